@@ -1,0 +1,28 @@
+"""freud_amd: freud's pair-analysis path (neighbour queries, RDF, Steinhardt,
+SolidLiquid, Hexatic, Cluster) on one AMD Instinct MI355X.
+
+The Python API mirrors freud 3.5.0's ``freud.box``, ``freud.locality``,
+``freud.density``, ``freud.order`` and ``freud.cluster`` for this path; the work
+runs in hand-written HIP kernels (``freud_amd/csrc``) behind the C-ABI declared
+in ``include/freud_amd.h``.
+"""
+
+from freud_amd import box, cluster, data, density, locality, order, parallel
+from freud_amd.box import Box
+from freud_amd.locality import AABBQuery, LinkCell, NeighborList
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "AABBQuery",
+    "Box",
+    "LinkCell",
+    "NeighborList",
+    "box",
+    "cluster",
+    "data",
+    "density",
+    "locality",
+    "order",
+    "parallel",
+]

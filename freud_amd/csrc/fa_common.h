@@ -1,0 +1,410 @@
+// fa_common.h -- shared device/host definitions of the freud_amd HIP library.
+//
+// All translation units are compiled with -ffp-contract=off and
+// -fhip-fp32-correctly-rounded-divide-sqrt: bond vectors, distances and
+// histogram bins are reproduced bit for bit from the reference's IEEE-single
+// arithmetic, which is formed without FMA and with correctly rounded divide and
+// square root (freud/util/VectorMath.h, freud/box/Box.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/freud_amd.h"
+
+namespace fa {
+
+// ------------------------------------------------------------------ errors
+// Status codes returned through the C-ABI; the Python layer maps them to the
+// exception types the reference raises (nanobind's default translation of
+// std::invalid_argument, std::runtime_error, std::domain_error, std::out_of_range).
+void set_error(int code, const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define FA_HIP(call)                                                                        \
+    do                                                                                      \
+    {                                                                                       \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return ::fa::fail(FA_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+hipStream_t stream();
+
+// Per-kernel event timing on the library stream (off unless enabled through
+// fa_profile_enable); each FA_PROF_BEGIN/END pair brackets one launch.
+bool profiling();
+void prof_begin(const char* name);
+void prof_end();
+#define FA_PROF_BEGIN(name)     \
+    if (::fa::profiling())      \
+    ::fa::prof_begin(name)
+#define FA_PROF_END()           \
+    if (::fa::profiling())      \
+    ::fa::prof_end()
+// Kernel launch on the library stream, timed when profiling is enabled.
+#define FA_KLAUNCH(kernel, ...)                    \
+    do                                             \
+    {                                              \
+        FA_PROF_BEGIN(#kernel);                    \
+        hipLaunchKernelGGL(kernel, __VA_ARGS__);   \
+        FA_PROF_END();                             \
+    } while (0)
+
+// Device scratch that persists between calls (grown on demand, never shrunk).
+int scratch(size_t slot, size_t bytes, void** ptr);
+int device_alloc(void** p, size_t bytes);
+void device_free(void* p);
+
+// ------------------------------------------------------------------ box
+// Box with the 27 periodic image vectors precomputed on the host with the
+// reference's float expression float(i)*a + float(j)*b + float(k)*c
+// (freud/locality/NeighborQuery.h:549-556).  Image (i,j,k) in {-1,0,1}^3 is
+// stored at (i+1)*9 + (j+1)*3 + (k+1).
+struct DevBox
+{
+    float L[3];
+    float lo[3];
+    float xy, xz, yz;
+    int is2D;
+    float img[27][3];
+};
+
+DevBox make_devbox(const fa_box& b);
+// freud/box/Box.h:489-497
+void nearest_plane_distance(const fa_box& b, float out[3]);
+float box_volume(const fa_box& b);
+
+// freud/box/Box.h:243-255
+__host__ __device__ inline void make_fractional(const DevBox& b, float vx, float vy, float vz, float& fx, float& fy,
+                                                float& fz)
+{
+    float dx = vx - b.lo[0];
+    float dy = vy - b.lo[1];
+    float dz = vz - b.lo[2];
+    float t1 = b.yz * b.xy;
+    float t2 = b.xz - t1;
+    float t3 = t2 * vz;
+    float t4 = b.xy * vy;
+    dx = dx - (t3 + t4);
+    dy = dy - b.yz * vz;
+    fx = dx / b.L[0];
+    fy = dy / b.L[1];
+    fz = dz / b.L[2];
+    if (b.is2D)
+        fz = 0.0f;
+}
+
+// freud/box/Box.h:212-222
+__host__ __device__ inline void make_absolute(const DevBox& b, float fx, float fy, float fz, float& vx, float& vy,
+                                              float& vz)
+{
+    float ax = b.lo[0] + fx * b.L[0];
+    float ay = b.lo[1] + fy * b.L[1];
+    float az = b.lo[2] + fz * b.L[2];
+    float s = b.xy * ay;
+    float u = b.xz * az;
+    ax = ax + (s + u);
+    ay = ay + b.yz * az;
+    if (b.is2D)
+        az = 0.0f;
+    vx = ax;
+    vy = ay;
+    vz = az;
+}
+
+// freud/util/utils.h:29-32
+__host__ __device__ inline float modulus_positive(float a)
+{
+    return fmodf(fmodf(a, 1.0f) + 1.0f, 1.0f);
+}
+
+// freud/box/Box.h:307-329 (every dimension periodic)
+__host__ __device__ inline void wrap(const DevBox& b, float vx, float vy, float vz, float& ox, float& oy, float& oz)
+{
+    float fx, fy, fz;
+    make_fractional(b, vx, vy, vz, fx, fy, fz);
+    fx = modulus_positive(fx);
+    fy = modulus_positive(fy);
+    fz = modulus_positive(fz);
+    make_absolute(b, fx, fy, fz, ox, oy, oz);
+}
+
+// ------------------------------------------------------------------ grid
+// Cell grid over fractional coordinates.  Points are stored cell-sorted as
+// float4 (x, y, z, bits(j)); cells are x-fastest so the cells of one stencil
+// row are contiguous in memory.  `wrapc` packs each point's integer image
+// offset floor(f) (10 bits per axis) and is only read when some point lies
+// outside the box.
+struct DevGrid
+{
+    const float4* cpos;
+    const int* wrapc;
+    const uint32_t* cstart; // ncells + 1
+    int dim[3];
+    int any_outside;
+};
+
+__device__ __host__ inline int floor_div(int a, int b)
+{
+    int q = a / b;
+    return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+
+__device__ inline void cell_of(const DevBox& b, const int dim[3], float x, float y, float z, int c[3], int w[3])
+{
+    float f[3];
+    make_fractional(b, x, y, z, f[0], f[1], f[2]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        float s = floorf(f[k] * float(dim[k]));
+        s = fminf(fmaxf(s, -1.0e6f), 1.0e6f); // NaN-free clamp for far-outside points
+        int C = int(s);
+        int ww = floor_div(C, dim[k]);
+        c[k] = C - ww * dim[k];
+        w[k] = ww;
+    }
+}
+
+__device__ inline int pack_wrap(const int w[3])
+{
+    auto cl = [](int v) { return (v < -511 ? -511 : (v > 511 ? 511 : v)) & 1023; };
+    return cl(w[0]) | (cl(w[1]) << 10) | (cl(w[2]) << 20);
+}
+__device__ inline void unpack_wrap(int p, int w[3])
+{
+    w[0] = (p << 22) >> 22;
+    w[1] = (p << 12) >> 22;
+    w[2] = (p << 2) >> 22;
+}
+
+enum Formula
+{
+    AABB_FORMULA = 0,     // r = p_j - (q_i + image)   freud/locality/AABBQuery.cc:93,125
+    LINKCELL_FORMULA = 1, // r = box.wrap(p_j - q_i)   freud/locality/LinkCell.cc:522,616
+};
+
+// Per-dimension stencil interval [lo, hi] of cell offsets.  For the AABB
+// formula every offset is a distinct (cell, image) pair, so the interval is
+// always [-R, R].  For the LinkCell formula a cell must be visited once, so an
+// interval wider than the grid is clipped to `dim` distinct cells (the
+// reference deduplicates with an unordered_set, LinkCell.cc:546-556).
+template<int FORMULA> __device__ inline void stencil_interval(int R, int dim, int& lo, int& hi)
+{
+    if (FORMULA == LINKCELL_FORMULA && 2 * R + 1 >= dim)
+    {
+        lo = -((dim - 1) / 2);
+        hi = dim - 1 + lo;
+    }
+    else
+    {
+        lo = -R;
+        hi = R;
+    }
+}
+
+template<int FORMULA, typename F>
+__device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, float qy, float qzz, float qz,
+                                   const int wq[3], int rowbase, int ux, int ux_end, int sy, int sz, F&& f)
+{
+    // cells ux..ux_end (unwrapped x indices) share one x image shift
+    int const sx = floor_div(ux, g.dim[0]);
+    int const cx = ux - sx * g.dim[0];
+    int const cxe = cx + (ux_end - ux);
+    uint32_t const beg = g.cstart[rowbase + cx];
+    uint32_t const end = g.cstart[rowbase + cxe + 1];
+    if (FORMULA == AABB_FORMULA)
+    {
+        int const nb0 = -wq[0] - sx, nb1 = -wq[1] - sy, nb2 = -wq[2] - sz;
+        if (!g.any_outside)
+        {
+            if (nb0 < -1 || nb0 > 1 || nb1 < -1 || nb1 > 1 || nb2 < -1 || nb2 > 1)
+                return;
+            int const code = (nb0 + 1) * 9 + (nb1 + 1) * 3 + (nb2 + 1);
+            float const ix = qx + b.img[code][0];
+            float const iy = qy + b.img[code][1];
+            float const iz = qzz + b.img[code][2];
+            for (uint32_t t = beg; t < end; ++t)
+            {
+                float4 const p = g.cpos[t];
+                float const pz = b.is2D ? 0.0f : p.z;
+                float const rx = p.x - ix;
+                float const ry = p.y - iy;
+                float const rz = pz - iz;
+                float const r_sq = (rx * rx + ry * ry) + rz * rz;
+                f(__float_as_uint(p.w), rx, ry, rz, r_sq, code);
+            }
+        }
+        else
+        {
+            for (uint32_t t = beg; t < end; ++t)
+            {
+                int wj[3];
+                unpack_wrap(g.wrapc[t], wj);
+                int const n0 = nb0 + wj[0], n1 = nb1 + wj[1], n2 = b.is2D ? 0 : nb2 + wj[2];
+                if (n0 < -1 || n0 > 1 || n1 < -1 || n1 > 1 || n2 < -1 || n2 > 1)
+                    continue;
+                int const code = (n0 + 1) * 9 + (n1 + 1) * 3 + (n2 + 1);
+                float4 const p = g.cpos[t];
+                float const pz = b.is2D ? 0.0f : p.z;
+                float const rx = p.x - (qx + b.img[code][0]);
+                float const ry = p.y - (qy + b.img[code][1]);
+                float const rz = pz - (qzz + b.img[code][2]);
+                float const r_sq = (rx * rx + ry * ry) + rz * rz;
+                f(__float_as_uint(p.w), rx, ry, rz, r_sq, code);
+            }
+        }
+    }
+    else
+    {
+        for (uint32_t t = beg; t < end; ++t)
+        {
+            float4 const p = g.cpos[t];
+            float rx, ry, rz;
+            wrap(b, p.x - qx, p.y - qy, p.z - qz, rx, ry, rz);
+            float const r_sq = (rx * rx + ry * ry) + rz * rz;
+            f(__float_as_uint(p.w), rx, ry, rz, r_sq, 13);
+        }
+    }
+}
+
+// Visit every candidate point of the stencil of half-width R around q, row by
+// row (a row's cells are contiguous in memory).  For each candidate calls
+//   f(j, rx, ry, rz, r_sq, code)
+// with the reference's bond vector for FORMULA; `code` is the image index
+// (0..26) used (AABB) or 13 (LinkCell).  `ring_only` restricts the visit to
+// cells at Chebyshev offset exactly R (AABB formula only).
+template<int FORMULA, typename F>
+__device__ inline void for_each_candidate(const DevBox& b, const DevGrid& g, float qx, float qy, float qz, int R,
+                                          bool ring_only, F&& f)
+{
+    int cq[3], wq[3];
+    cell_of(b, g.dim, qx, qy, qz, cq, wq);
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        stencil_interval<FORMULA>(R, g.dim[k], lo[k], hi[k]);
+    if (b.is2D)
+        lo[2] = hi[2] = 0;
+    float const qzz = (FORMULA == AABB_FORMULA && b.is2D) ? 0.0f : qz;
+    for (int dz = lo[2]; dz <= hi[2]; ++dz)
+    {
+        int const uz = cq[2] + dz;
+        int const sz = floor_div(uz, g.dim[2]);
+        int const cz = uz - sz * g.dim[2];
+        for (int dy = lo[1]; dy <= hi[1]; ++dy)
+        {
+            int const uy = cq[1] + dy;
+            int const sy = floor_div(uy, g.dim[1]);
+            int const cy = uy - sy * g.dim[1];
+            int const rowbase = (cz * g.dim[1] + cy) * g.dim[0];
+            bool const inner = ring_only && R > 0 && (dy > -R && dy < R) && (b.is2D || (dz > -R && dz < R));
+            if (inner)
+            {
+                visit_piece<FORMULA>(b, g, qx, qy, qzz, qz, wq, rowbase, cq[0] - R, cq[0] - R, sy, sz, f);
+                visit_piece<FORMULA>(b, g, qx, qy, qzz, qz, wq, rowbase, cq[0] + R, cq[0] + R, sy, sz, f);
+                continue;
+            }
+            int ux = cq[0] + lo[0];
+            int const uxend = cq[0] + hi[0];
+            while (ux <= uxend)
+            {
+                int const sx = floor_div(ux, g.dim[0]);
+                int const piece_end = min(uxend, (sx + 1) * g.dim[0] - 1);
+                visit_piece<FORMULA>(b, g, qx, qy, qzz, qz, wq, rowbase, ux, piece_end, sy, sz, f);
+                ux = piece_end + 1;
+            }
+        }
+    }
+}
+
+// Recompute the bond vector of (q, p_j) for an image code (AABB) or by wrap
+// (LinkCell); identical arithmetic to for_each_candidate.
+template<int FORMULA>
+__device__ inline void bond_vector(const DevBox& b, float qx, float qy, float qz, float px, float py, float pz,
+                                   int code, float& rx, float& ry, float& rz)
+{
+    if (FORMULA == AABB_FORMULA)
+    {
+        float const qzz = b.is2D ? 0.0f : qz;
+        float const pzz = b.is2D ? 0.0f : pz;
+        rx = px - (qx + b.img[code][0]);
+        ry = py - (qy + b.img[code][1]);
+        rz = pzz - (qzz + b.img[code][2]);
+    }
+    else
+    {
+        wrap(b, px - qx, py - qy, pz - qz, rx, ry, rz);
+    }
+}
+
+// ------------------------------------------------------------------ host-side objects
+struct Grid
+{
+    fa_box box;
+    DevBox dbox;
+    int dim[3];
+    float width; // requested minimum cell extent along every plane normal
+    uint32_t n;
+    float4* cpos = nullptr;
+    int* wrapc = nullptr;
+    uint32_t* cstart = nullptr;
+    uint32_t* cell_of_point = nullptr; // scratch
+    int any_outside = 0;
+    DevGrid dev() const
+    {
+        DevGrid g;
+        g.cpos = cpos;
+        g.wrapc = wrapc;
+        g.cstart = cstart;
+        g.dim[0] = dim[0];
+        g.dim[1] = dim[1];
+        g.dim[2] = dim[2];
+        g.any_outside = any_outside;
+        return g;
+    }
+    ~Grid();
+};
+
+// Build (or rebuild) a grid over device points with cells at least `width`
+// wide along each plane normal (dims clamped to >= 1, z = 1 in 2D).
+int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, const int dims[3]);
+
+// Exclusive scan of n uint32 counts into n+1 offsets (offsets[n] = total);
+// returns the 64-bit total through *total (host) when total != nullptr.
+int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total);
+int sort_u64(uint64_t* d_keys, uint32_t n);
+
+} // namespace fa
+
+struct fa_nq
+{
+    fa_box box;
+    const float* d_points; // caller-owned device array, N x 3
+    uint32_t n;
+    int kind;         // FA_NQ_AABB or FA_NQ_LINKCELL
+    float cell_width; // LinkCell's cell width (reference semantics)
+    int lc_dim[3];    // LinkCell's grid dims (LinkCell.cc:278-314)
+    // small cache of grids (one per query shape: e.g. ball query, RDF, kNN);
+    // `cur` is the grid selected by the last ensure_grid
+    static constexpr int kGrids = 3;
+    fa::Grid grids[kGrids];
+    int grid_dims[kGrids][3] = {};
+    bool grid_valid[kGrids] = {};
+    uint64_t grid_used[kGrids] = {};
+    uint64_t tick = 0;
+    fa::Grid* cur = nullptr;
+};
+
+namespace fa {
+// Shared by the query, RDF and analysis translation units.
+int validate_query_args(const fa_nq* nq, fa_query_args& a, uint32_t n_query);
+int ensure_grid(fa_nq* nq, const int dims[3]);
+void grid_dims_for(const fa_box& box, uint32_t n, float r, int R, int dims[3]);
+int nlist_reserve(fa_nlist* nl, uint64_t nb, uint32_t nq);
+int nlist_finish_counts(fa_nlist* nl);
+} // namespace fa

@@ -1,0 +1,121 @@
+// fa_grid.hip -- cell grid over fractional coordinates (the GPU counterpart of
+// freud's LinkCell / CellQuery cell lists, freud/locality/LinkCell.cc:316-367,
+// freud/locality/CellQuery.cc): counting sort of the points into cells, stored
+// as cell-ordered float4 (x, y, z, bits(j)) for coalesced stencil reads.
+#include "fa_common.h"
+
+namespace {
+
+__global__ void grid_assign_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, int d0, int d1, int d2,
+                                   uint32_t* __restrict__ cell_of_point, uint32_t* __restrict__ counts,
+                                   int* __restrict__ any_outside)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int const dim[3] = {d0, d1, d2};
+    int c[3], w[3];
+    fa::cell_of(b, dim, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], c, w);
+    uint32_t const cell = uint32_t((c[2] * d1 + c[1]) * d0 + c[0]);
+    cell_of_point[i] = cell;
+    if (w[0] != 0 || w[1] != 0 || w[2] != 0)
+        atomicOr(any_outside, 1);
+    atomicAdd(&counts[cell], 1u);
+}
+
+__global__ void grid_scatter_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, int d0, int d1, int d2,
+                                    const uint32_t* __restrict__ cell_of_point, const uint32_t* __restrict__ cstart,
+                                    uint32_t* __restrict__ fill, float4* __restrict__ cpos, int* __restrict__ wrapc)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t const cell = cell_of_point[i];
+    uint32_t const slot = cstart[cell] + atomicAdd(&fill[cell], 1u);
+    float const x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    cpos[slot] = make_float4(x, y, z, __uint_as_float(i));
+    int const dim[3] = {d0, d1, d2};
+    int c[3], w[3];
+    fa::cell_of(b, dim, x, y, z, c, w);
+    wrapc[slot] = fa::pack_wrap(w);
+}
+
+} // namespace
+
+namespace fa {
+
+Grid::~Grid()
+{
+    device_free(cpos);
+    device_free(wrapc);
+    device_free(cstart);
+    device_free(cell_of_point);
+}
+
+int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, const int dims[3])
+{
+    hipStream_t s = stream();
+    uint64_t const ncell64 = uint64_t(dims[0]) * dims[1] * dims[2];
+    if (ncell64 > (1ull << 30))
+        return fail(FA_ERR_RUNTIME, "cell grid too large");
+    uint32_t const ncell = uint32_t(ncell64);
+    bool const realloc_pts = (g.n < n) || !g.cpos;
+    bool const realloc_cells = !g.cstart || (uint64_t(g.dim[0]) * g.dim[1] * g.dim[2] < ncell64);
+    if (realloc_pts)
+    {
+        device_free(g.cpos);
+        device_free(g.wrapc);
+        device_free(g.cell_of_point);
+        g.cpos = nullptr;
+        g.wrapc = nullptr;
+        g.cell_of_point = nullptr;
+        if (int rc = device_alloc((void**) &g.cpos, sizeof(float4) * size_t(n)))
+            return rc;
+        if (int rc = device_alloc((void**) &g.wrapc, sizeof(int) * size_t(n)))
+            return rc;
+        if (int rc = device_alloc((void**) &g.cell_of_point, sizeof(uint32_t) * size_t(n)))
+            return rc;
+    }
+    if (realloc_cells)
+    {
+        device_free(g.cstart);
+        g.cstart = nullptr;
+        if (int rc = device_alloc((void**) &g.cstart, sizeof(uint32_t) * (size_t(ncell) + 1)))
+            return rc;
+    }
+    g.box = box;
+    g.dbox = make_devbox(box);
+    g.dim[0] = dims[0];
+    g.dim[1] = dims[1];
+    g.dim[2] = dims[2];
+    g.n = n;
+    // scratch: counts (ncell) + fill (ncell) + any_outside flag
+    void* tmp = nullptr;
+    size_t const cb = sizeof(uint32_t) * size_t(ncell);
+    if (int rc = scratch(2, 2 * cb + 256, &tmp))
+        return rc;
+    uint32_t* counts = static_cast<uint32_t*>(tmp);
+    uint32_t* fill = counts + ncell;
+    int* d_any = reinterpret_cast<int*>(fill + ncell);
+    FA_HIP(hipMemsetAsync(tmp, 0, 2 * cb + sizeof(int), s));
+    unsigned const blocks = (n + 255) / 256;
+    if (n)
+    {
+        FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, g.dbox, d_points, n, dims[0], dims[1],
+                           dims[2], g.cell_of_point, counts, d_any);
+        FA_HIP(hipGetLastError());
+    }
+    if (int rc = exclusive_scan_u32(counts, g.cstart, ncell, nullptr))
+        return rc;
+    if (n)
+    {
+        FA_KLAUNCH(grid_scatter_kernel, dim3(blocks), dim3(256), 0, s, g.dbox, d_points, n, dims[0], dims[1],
+                           dims[2], g.cell_of_point, g.cstart, fill, g.cpos, g.wrapc);
+        FA_HIP(hipGetLastError());
+    }
+    FA_HIP(hipMemcpyAsync(&g.any_outside, d_any, sizeof(int), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    return FA_OK;
+}
+
+} // namespace fa

@@ -1,0 +1,818 @@
+// fa_order.hip -- Steinhardt q_l / w_l (freud/order/Steinhardt.cc), SolidLiquid
+// (freud/order/SolidLiquid.cc), Cluster (freud/cluster/Cluster.cc) and Hexatic
+// (freud/order/HexaticTranslational.cc) over a device NeighborList.
+//
+// Spherical harmonics: the reference evaluates Y_lm with the fsph submodule
+// from (theta, phi) = (acos(z/d), atan2(y, x)).  Here they are evaluated
+// straight from the bond direction (cos theta = z/d, e^{i phi} = (x + i y)/rho)
+// with the normalised associated-Legendre recurrence
+//   P(m,m)   = -sqrt((2m+1)/(2m)) sin(t) P(m-1,m-1),   P(0,0) = 1/sqrt(4 pi)
+//   P(m+1,m) = sqrt(2m+3) cos(t) P(m,m)
+//   P(l,m)   = a(l,m) (cos(t) P(l-1,m) - b(l,m) P(l-2,m))
+// (Condon-Shortley phase included, the convention Steinhardt.cc:31-52 stores).
+// Only m >= 0 is accumulated: for real bond weights the m < 0 sums follow from
+// q_{l,-m} = (-1)^m conj(q_{l,m}).  No transcendental function is evaluated per
+// bond.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "fa_common.h"
+
+extern "C" double fa_wigner3j_lll(int l, int m1, int m2, int m3);
+
+namespace {
+
+constexpr int kLMax = 20;
+constexpr int kBlock = 256;
+
+struct YlmCoef
+{
+    float amm[kLMax + 1];                 // -sqrt((2m+1)/(2m)), m >= 1
+    float bm1[kLMax + 1];                 // sqrt(2m+3)
+    float a[kLMax + 1][kLMax + 1];        // a(l, m)
+    float b[kLMax + 1][kLMax + 1];        // b(l, m)
+};
+__constant__ YlmCoef c_ylm;
+bool g_ylm_ready = false;
+
+int ensure_ylm_coef()
+{
+    if (g_ylm_ready)
+        return FA_OK;
+    YlmCoef h{};
+    for (int m = 0; m <= kLMax; ++m)
+    {
+        h.amm[m] = m ? float(-std::sqrt((2.0 * m + 1.0) / (2.0 * m))) : 0.0f;
+        h.bm1[m] = float(std::sqrt(2.0 * m + 3.0));
+        for (int l = 0; l <= kLMax; ++l)
+        {
+            if (l >= m + 2)
+            {
+                h.a[l][m] = float(std::sqrt((4.0 * l * l - 1.0) / (double(l) * l - double(m) * m)));
+                h.b[l][m] = float(std::sqrt(((l - 1.0) * (l - 1.0) - double(m) * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0)));
+            }
+        }
+    }
+    FA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ylm), &h, sizeof(h)));
+    g_ylm_ready = true;
+    return FA_OK;
+}
+
+// Bond direction in the reference's terms: delta = box.wrap(p_j - p_i)
+// (Steinhardt.cc:155), cos(theta) = clamp(delta.z / d, -1, 1) with theta = 0
+// for d == 0 (Steinhardt.cc:167-180), e^{i phi} with phi = atan2(delta.y, delta.x).
+__device__ inline void bond_angles(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i, uint32_t j,
+                                   float d, float& ct, float& st, float& cp, float& sp)
+{
+    float dx, dy, dz;
+    fa::wrap(b, pts[3 * j] - pts[3 * i], pts[3 * j + 1] - pts[3 * i + 1], pts[3 * j + 2] - pts[3 * i + 2], dx, dy, dz);
+    if (d == 0.0f)
+    {
+        ct = 1.0f;
+        st = 0.0f;
+    }
+    else
+    {
+        ct = fminf(fmaxf(dz / d, -1.0f), 1.0f);
+        st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+    }
+    float const rho2 = dx * dx + dy * dy;
+    if (rho2 > 0.0f)
+    {
+        float const inv = 1.0f / sqrtf(rho2);
+        cp = dx * inv;
+        sp = dy * inv;
+    }
+    else
+    {
+        cp = signbit(dx) ? -1.0f : 1.0f;
+        sp = 0.0f;
+    }
+}
+
+// Accumulate w * Y_L^m (m = 0..L) of one bond into acc.
+template<int L> __device__ inline void accumulate_ylm(float ct, float st, float cp, float sp, float w, float2* acc)
+{
+    float pmm = 0.28209479177387814f;
+    float er = 1.0f, ei = 0.0f;
+#pragma unroll
+    for (int m = 0; m <= L; ++m)
+    {
+        if (m > 0)
+        {
+            pmm = c_ylm.amm[m] * st * pmm;
+            float const nr = er * cp - ei * sp;
+            float const ni = er * sp + ei * cp;
+            er = nr;
+            ei = ni;
+        }
+        float plm;
+        if (m == L)
+            plm = pmm;
+        else
+        {
+            float p0 = pmm;
+            float p1 = c_ylm.bm1[m] * ct * pmm;
+#pragma unroll
+            for (int l = m + 2; l <= L; ++l)
+            {
+                float const p2 = c_ylm.a[l][m] * (ct * p1 - c_ylm.b[l][m] * p0);
+                p0 = p1;
+                p1 = p2;
+            }
+            plm = p1;
+        }
+        float const wp = w * plm;
+        acc[m].x += wp * er;
+        acc[m].y += wp * ei;
+    }
+}
+
+// Steinhardt.cc:120-222: q_lm(i) = sum_b w Y_lm / sum_b w and
+// q_l(i) = sqrt(4 pi / (2l+1) sum_m |q_lm|^2); also the block partial sums of
+// q_lm(i) / N for the system-wide order.
+template<int L>
+__global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
+                                                     const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
+                                                     const float* __restrict__ wts, const uint32_t* __restrict__ offsets,
+                                                     int weighted, int want_sys, float2* __restrict__ qlm,
+                                                     float* __restrict__ ql, int ql_stride, int ql_col,
+                                                     float2* __restrict__ sys_partial)
+{
+    constexpr int NM = 2 * L + 1;
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    float2 acc[L + 1];
+#pragma unroll
+    for (int m = 0; m <= L; ++m)
+        acc[m] = make_float2(0.0f, 0.0f);
+    float q[2 * NM];
+    if (i < n)
+    {
+        float total_w = 0.0f;
+        uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+        for (uint32_t e = e0; e < e1; ++e)
+        {
+            uint32_t const j = nbr[2 * e + 1];
+            float const w = weighted ? wts[e] : 1.0f;
+            float ct, st, cp, sp;
+            bond_angles(b, pts, i, j, dist[e], ct, st, cp, sp);
+            accumulate_ylm<L>(ct, st, cp, sp, w, acc);
+            total_w += w;
+        }
+        float norm = 0.0f;
+#pragma unroll
+        for (int m = 0; m <= L; ++m)
+        {
+            float const re = acc[m].x / total_w, im = acc[m].y / total_w;
+            q[2 * m] = re;
+            q[2 * m + 1] = im;
+            if (m > 0)
+            {
+                float const sgn = (m & 1) ? -1.0f : 1.0f;
+                q[2 * (L + m)] = sgn * re;
+                q[2 * (L + m) + 1] = -sgn * im;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NM; ++t)
+            norm += q[2 * t] * q[2 * t] + q[2 * t + 1] * q[2 * t + 1];
+        float2* out = qlm + uint64_t(i) * NM;
+#pragma unroll
+        for (int t = 0; t < NM; ++t)
+            out[t] = make_float2(q[2 * t], q[2 * t + 1]);
+        float const nf = float(4.0 * M_PI / NM);
+        ql[uint64_t(i) * ql_stride + ql_col] = sqrtf(norm * nf);
+    }
+    if (!want_sys)
+        return;
+    // block partial sum of q_lm / N (system q_lm, Steinhardt.cc:213-216)
+    __shared__ float2 s_red[kBlock];
+    float const invn = 1.0f / float(n);
+    for (int t = 0; t < NM; ++t)
+    {
+        s_red[threadIdx.x] = (i < n) ? make_float2(q[2 * t] * invn, q[2 * t + 1] * invn) : make_float2(0.0f, 0.0f);
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1)
+        {
+            if (threadIdx.x < s)
+            {
+                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
+                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            sys_partial[uint64_t(blockIdx.x) * NM + t] = s_red[0];
+        __syncthreads();
+    }
+}
+
+// Steinhardt.cc:224-289: qbar_lm(i) = (sum_j q_lm(j) + q_lm(i)) / (n_b + 1)
+template<int L>
+__global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint32_t* __restrict__ nbr,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const float2* __restrict__ qlm, float2* __restrict__ qlmave,
+                                                         float* __restrict__ qlave, int ql_stride, int ql_col,
+                                                         float2* __restrict__ sys_partial)
+{
+    constexpr int NM = 2 * L + 1;
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    float2 acc[NM];
+#pragma unroll
+    for (int t = 0; t < NM; ++t)
+        acc[t] = make_float2(0.0f, 0.0f);
+    if (i < n)
+    {
+        uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+        for (uint32_t e = e0; e < e1; ++e)
+        {
+            const float2* src = qlm + uint64_t(nbr[2 * e + 1]) * NM;
+#pragma unroll
+            for (int t = 0; t < NM; ++t)
+            {
+                float2 const v = src[t];
+                acc[t].x += v.x;
+                acc[t].y += v.y;
+            }
+        }
+        float const cnt = float(e1 - e0 + 1);
+        const float2* own = qlm + uint64_t(i) * NM;
+        float norm = 0.0f;
+        float2* out = qlmave + uint64_t(i) * NM;
+#pragma unroll
+        for (int t = 0; t < NM; ++t)
+        {
+            float2 const v = own[t];
+            acc[t].x = (acc[t].x + v.x) / cnt;
+            acc[t].y = (acc[t].y + v.y) / cnt;
+            out[t] = acc[t];
+            norm += acc[t].x * acc[t].x + acc[t].y * acc[t].y;
+        }
+        float const nf = float(4.0 * M_PI / NM);
+        qlave[uint64_t(i) * ql_stride + ql_col] = sqrtf(norm * nf);
+    }
+    __shared__ float2 s_red[kBlock];
+    float const invn = 1.0f / float(n);
+    for (int t = 0; t < NM; ++t)
+    {
+        s_red[threadIdx.x] = (i < n) ? make_float2(acc[t].x * invn, acc[t].y * invn) : make_float2(0.0f, 0.0f);
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1)
+        {
+            if (threadIdx.x < s)
+            {
+                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
+                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            sys_partial[uint64_t(blockIdx.x) * NM + t] = s_red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void sum_partials_kernel(const float2* __restrict__ part, uint32_t nblocks, int nm, float2* __restrict__ out)
+{
+    __shared__ float2 s_red[kBlock];
+    for (int t = 0; t < nm; ++t)
+    {
+        float2 acc = make_float2(0.0f, 0.0f);
+        for (uint32_t bl = threadIdx.x; bl < nblocks; bl += blockDim.x)
+        {
+            float2 const v = part[uint64_t(bl) * nm + t];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        s_red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1)
+        {
+            if (threadIdx.x < s)
+            {
+                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
+                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            out[t] = s_red[0];
+        __syncthreads();
+    }
+}
+
+// Wigner3j.cc:22-56 reduction for every particle, w_l normalisation of
+// Steinhardt.cc:345-355.
+__device__ inline int lm_index(int l, int m) { return m < 0 ? l - m : m; }
+
+__global__ __launch_bounds__(kBlock) void wl_kernel(uint32_t n, int l, const float2* __restrict__ src,
+                                                    const float* __restrict__ w3j, const float* __restrict__ qlnorm,
+                                                    int wl_normalize, float* __restrict__ wl, int stride, int col)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int const nm = 2 * l + 1;
+    const float2* q = src + uint64_t(i) * nm;
+    float result = 0.0f;
+    unsigned counter = 0;
+    for (int m1 = -l; m1 <= l; ++m1)
+    {
+        float2 const a = q[lm_index(l, m1)];
+        int const m2lo = max(-l - m1, -l), m2hi = min(l - m1, l);
+        for (int m2 = m2lo; m2 <= m2hi; ++m2)
+        {
+            float2 const bq = q[lm_index(l, m2)];
+            float2 const c = q[lm_index(l, -m1 - m2)];
+            float const w = w3j[counter++];
+            // ((w * a) * b) * c, real part, in std::complex<float> order
+            float const ar = w * a.x, ai = w * a.y;
+            float const abr = ar * bq.x - ai * bq.y;
+            float const abi = ar * bq.y + ai * bq.x;
+            result += abr * c.x - abi * c.y;
+        }
+    }
+    if (wl_normalize)
+    {
+        float const nf = float(4.0 * M_PI / nm);
+        float const norm = sqrtf(nf) / qlnorm[uint64_t(i) * stride + col];
+        result *= norm * norm * norm;
+    }
+    wl[uint64_t(i) * stride + col] = result;
+}
+
+// SolidLiquid.cc:51-78 (bond dot products) and :90-96 (solid-bond counts)
+__global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, const uint32_t* __restrict__ nbr,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const float2* __restrict__ qlm,
+                                                             const float* __restrict__ ql, int normalize_q,
+                                                             float q_threshold, float* __restrict__ ql_ij,
+                                                             uint32_t* __restrict__ num_conn)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int const nm = 2 * l + 1;
+    float const nf = float(4.0 * M_PI / nm);
+    const float2* qi = qlm + uint64_t(i) * nm;
+    uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+    uint32_t cnt = 0;
+    for (uint32_t e = e0; e < e1; ++e)
+    {
+        uint32_t const j = nbr[2 * e + 1];
+        const float2* qj = qlm + uint64_t(j) * nm;
+        float re = 0.0f;
+        for (int t = 0; t < nm; ++t)
+        {
+            float2 const a = qi[t], c = qj[t];
+            re += a.x * c.x + a.y * c.y;
+        }
+        if (normalize_q)
+            re *= nf / (ql[i] * ql[j]);
+        ql_ij[e] = re;
+        cnt += (re > q_threshold) ? 1u : 0u;
+    }
+    num_conn[i] = cnt;
+}
+
+// SolidLiquid.cc:100-110: solid-like bonds between particles with at least
+// solid_threshold solid-like bonds each
+__global__ void solid_keep_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, const float* __restrict__ ql_ij,
+                                  const uint32_t* __restrict__ num_conn, float q_threshold, uint32_t solid_threshold,
+                                  uint8_t* __restrict__ keep)
+{
+    uint64_t const e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nb)
+        return;
+    uint2 const ij = reinterpret_cast<const uint2*>(nbr)[e];
+    keep[e] = (ql_ij[e] > q_threshold) && num_conn[ij.x] >= solid_threshold && num_conn[ij.y] >= solid_threshold;
+}
+
+// ------------------------------------------------------------------ Cluster
+// Lock-free union-find: roots always hook to the smaller index, so every root
+// is the smallest member of its component (the "min id" of Cluster.cc:62-66).
+__device__ inline uint32_t uf_find(uint32_t* parent, uint32_t x)
+{
+    uint32_t p = parent[x];
+    while (p != x)
+    {
+        uint32_t const gp = parent[p];
+        if (gp != p)
+            parent[x] = gp; // path halving (benign race: any ancestor is valid)
+        x = p;
+        p = parent[x];
+    }
+    return x;
+}
+
+__global__ void uf_init_kernel(uint32_t n, uint32_t* parent, uint32_t* size)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        parent[i] = i;
+        size[i] = 0;
+    }
+}
+
+__global__ void uf_union_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ keep,
+                                uint32_t* parent)
+{
+    uint64_t const e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nb || (keep && !keep[e]))
+        return;
+    uint2 const ij = reinterpret_cast<const uint2*>(nbr)[e];
+    uint32_t a = ij.x, c = ij.y;
+    while (true)
+    {
+        a = uf_find(parent, a);
+        c = uf_find(parent, c);
+        if (a == c)
+            break;
+        if (a < c)
+        {
+            uint32_t const t = a;
+            a = c;
+            c = t;
+        }
+        // a > c: hook a under c if a is still a root
+        uint32_t const old = atomicCAS(&parent[a], a, c);
+        if (old == a)
+            break;
+        a = old;
+    }
+}
+
+__global__ void uf_flatten_kernel(uint32_t n, uint32_t* parent, uint32_t* size)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t const r = uf_find(parent, i);
+    parent[i] = r;
+    atomicAdd(&size[r], 1u);
+}
+
+// key = (n - size) << 32 | root sorts clusters by size descending, then by
+// smallest member ascending (Cluster.cc:110-128)
+__global__ void uf_keys_kernel(uint32_t n, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ size,
+                               uint32_t* __restrict__ counter, uint64_t* __restrict__ keys)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || parent[i] != i)
+        return;
+    uint32_t const slot = atomicAdd(counter, 1u);
+    keys[slot] = (uint64_t(n - size[i]) << 32) | i;
+}
+
+__global__ void uf_rank_kernel(uint32_t n, uint32_t ncl, const uint64_t* __restrict__ keys,
+                               uint32_t* __restrict__ rank_of_root, uint32_t* __restrict__ sizes)
+{
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ncl)
+    {
+        rank_of_root[uint32_t(keys[t])] = t;
+        if (sizes)
+            sizes[t] = n - uint32_t(keys[t] >> 32);
+    }
+}
+
+__global__ void uf_label_kernel(uint32_t n, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ rank_of_root,
+                                uint32_t* __restrict__ cluster_idx)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        cluster_idx[i] = rank_of_root[parent[i]];
+}
+
+// ------------------------------------------------------------------ Hexatic
+__global__ __launch_bounds__(kBlock) void hexatic_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
+                                                         const uint32_t* __restrict__ nbr,
+                                                         const float* __restrict__ wts,
+                                                         const uint32_t* __restrict__ offsets, float k, int weighted,
+                                                         float2* __restrict__ psi)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float re = 0.0f, im = 0.0f, total = 0.0f;
+    for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e)
+    {
+        uint32_t const j = nbr[2 * e + 1];
+        float dx, dy, dz;
+        fa::wrap(b, pts[3 * j] - pts[3 * i], pts[3 * j + 1] - pts[3 * i + 1], pts[3 * j + 2] - pts[3 * i + 2], dx, dy,
+                 dz);
+        float const w = weighted ? wts[e] : 1.0f;
+        float const th = atan2f(dy, dx);
+        float s, c;
+        sincosf(k * th, &s, &c);
+        re += w * c;
+        im += w * s;
+        total += w;
+    }
+    psi[i] = make_float2(re / total, im / total);
+}
+
+// ------------------------------------------------------------------ host
+std::vector<std::vector<float>> g_w3j_host(kLMax + 1);
+float* g_w3j_dev[kLMax + 1] = {nullptr};
+
+int wigner_table(int l, float** d)
+{
+    if (l > kLMax)
+        return fa::fail(FA_ERR_INVALID, "w_l is available for l <= 20 (Wigner3j.cc:59)");
+    if (!g_w3j_dev[l])
+    {
+        std::vector<float> v;
+        for (int m1 = -l; m1 <= l; ++m1)
+            for (int m2 = std::max(-l - m1, -l); m2 <= std::min(l - m1, l); ++m2)
+                v.push_back(float(fa_wigner3j_lll(l, m1, m2, -m1 - m2)));
+        g_w3j_host[l] = v;
+        if (int rc = fa::device_alloc((void**) &g_w3j_dev[l], sizeof(float) * v.size()))
+            return rc;
+        FA_HIP(hipMemcpy(g_w3j_dev[l], v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
+    }
+    *d = g_w3j_dev[l];
+    return FA_OK;
+}
+
+// host reduction of Wigner3j.cc:22-56 in std::complex<float>
+float reduce_wigner3j_host(const std::complex<float>* src, int l)
+{
+    const auto& w = g_w3j_host[l];
+    float result = 0;
+    unsigned counter = 0;
+    auto idx = [l](int m) { return m < 0 ? l - m : m; };
+    for (int m1 = -l; m1 <= l; ++m1)
+        for (int m2 = std::max(-l - m1, -l); m2 <= std::min(l - m1, l); ++m2)
+        {
+            result += (w[counter] * src[idx(m1)] * src[idx(m2)] * src[idx(-m1 - m2)]).real();
+            ++counter;
+        }
+    return result;
+}
+
+template<int L>
+void launch_qlm_l(const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted, int want_sys,
+                  float2* qlm, float* ql, int stride, int col, float2* part)
+{
+    unsigned const blocks = (n + kBlock - 1) / kBlock;
+    FA_KLAUNCH(qlm_kernel<L>, dim3(blocks), dim3(kBlock), 0, fa::stream(), db, pts, n, nl->neighbors,
+                       nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part);
+}
+
+template<int L>
+void launch_ave_l(uint32_t n, const fa_nlist* nl, const float2* qlm, float2* qlmave, float* qlave, int stride, int col,
+                  float2* part)
+{
+    unsigned const blocks = (n + kBlock - 1) / kBlock;
+    FA_KLAUNCH(qlm_ave_kernel<L>, dim3(blocks), dim3(kBlock), 0, fa::stream(), n, nl->neighbors, nl->offsets,
+                       qlm, qlmave, qlave, stride, col, part);
+}
+
+template<int L>
+int dispatch_l(int l, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted,
+               int want_sys, float2* qlm, float* ql, int stride, int col, float2* part)
+{
+    if (l == L)
+    {
+        launch_qlm_l<L>(db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part);
+        return 1;
+    }
+    if constexpr (L < kLMax)
+        return dispatch_l<L + 1>(l, db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part);
+    return 0;
+}
+
+template<int L>
+int dispatch_ave(int l, uint32_t n, const fa_nlist* nl, const float2* qlm, float2* qlmave, float* qlave, int stride,
+                 int col, float2* part)
+{
+    if (l == L)
+    {
+        launch_ave_l<L>(n, nl, qlm, qlmave, qlave, stride, col, part);
+        return 1;
+    }
+    if constexpr (L < kLMax)
+        return dispatch_ave<L + 1>(l, n, nl, qlm, qlmave, qlave, stride, col, part);
+    return 0;
+}
+
+} // namespace
+
+namespace fa {
+// Steinhardt::compute for one NeighborList; see the C-ABI below.
+int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, const uint32_t* ls,
+                   int n_l, int flags, float* d_qlm, float* d_ql, float* d_qlave, float* d_qlmave, float* d_wl,
+                   float* order)
+{
+    if (int rc = ensure_ylm_coef())
+        return rc;
+    if (nl->num_query_points != n)
+        return fail(FA_ERR_RUNTIME, "NeighborList found inconsistent array sizes.");
+    bool const average = flags & FA_ST_AVERAGE, wl = flags & FA_ST_WL, weighted = flags & FA_ST_WEIGHTED,
+               wl_norm = flags & FA_ST_WL_NORMALIZE;
+    hipStream_t s = stream();
+    DevBox const db = make_devbox(*box);
+    unsigned const blocks = (n + kBlock - 1) / kBlock;
+    size_t moff = 0;
+    for (int k = 0; k < n_l; ++k)
+    {
+        int const l = int(ls[k]);
+        if (l > kLMax)
+            return fail(FA_ERR_INVALID, "freud_amd supports l <= 20");
+        int const nm = 2 * l + 1;
+        // partial sums: blocks x nm complex for base and average, + nm result
+        void* tmp = nullptr;
+        size_t const pb = sizeof(float2) * size_t(blocks) * nm;
+        if (int rc = scratch(6, 2 * pb + sizeof(float2) * nm + 256, &tmp))
+            return rc;
+        float2* part = static_cast<float2*>(tmp);
+        float2* part_ave = reinterpret_cast<float2*>(static_cast<char*>(tmp) + pb);
+        float2* sys = reinterpret_cast<float2*>(static_cast<char*>(tmp) + 2 * pb);
+        float2* qlm_l = reinterpret_cast<float2*>(d_qlm) + size_t(n) * moff;
+        if (n)
+        {
+            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part);
+            FA_HIP(hipGetLastError());
+        }
+        float2* src = qlm_l;
+        if (average && n)
+        {
+            float2* qlmave_l = reinterpret_cast<float2*>(d_qlmave) + size_t(n) * moff;
+            dispatch_ave<0>(l, n, nl, qlm_l, qlmave_l, d_qlave, n_l, k, part_ave);
+            FA_HIP(hipGetLastError());
+            src = qlmave_l;
+        }
+        if (wl && n)
+        {
+            float* w3 = nullptr;
+            if (int rc = wigner_table(l, &w3))
+                return rc;
+            FA_KLAUNCH(wl_kernel, dim3(blocks), dim3(kBlock), 0, s, n, l, src, w3, average ? d_qlave : d_ql,
+                               int(wl_norm), d_wl, n_l, k);
+            FA_HIP(hipGetLastError());
+        }
+        // system order: Steinhardt.cc:291-327
+        std::vector<std::complex<float>> hsys(nm, 0.0f);
+        if (n)
+        {
+            FA_KLAUNCH(sum_partials_kernel, dim3(1), dim3(kBlock), 0, s, average ? part_ave : part, blocks, nm,
+                               sys);
+            FA_HIP(hipGetLastError());
+            FA_HIP(hipMemcpyAsync(hsys.data(), sys, sizeof(float2) * nm, hipMemcpyDeviceToHost, s));
+            FA_HIP(hipStreamSynchronize(s));
+        }
+        float const nf = float(4.0 * M_PI / nm);
+        float calc = 0;
+        for (int t = 0; t < nm; ++t)
+            calc += std::norm(hsys[t]);
+        float const ql_sys = std::sqrt(calc * nf);
+        if (wl)
+        {
+            float* w3 = nullptr;
+            if (int rc = wigner_table(l, &w3))
+                return rc;
+            float v = reduce_wigner3j_host(hsys.data(), l);
+            if (wl_norm)
+            {
+                float const wn = std::sqrt(nf) / ql_sys;
+                v *= wn * wn * wn;
+            }
+            order[k] = v;
+        }
+        else
+            order[k] = ql_sys;
+        moff += nm;
+    }
+    return FA_OK;
+}
+} // namespace fa
+
+extern "C" {
+
+// (l l l; m1 m2 m3) by the Racah formula in long double (log-factorials)
+double fa_wigner3j_lll(int l, int m1, int m2, int m3)
+{
+    if (m1 + m2 + m3 != 0 || std::abs(m1) > l || std::abs(m2) > l || std::abs(m3) > l)
+        return 0.0;
+    auto lf = [](int v) { return std::lgamma((long double) v + 1.0L); };
+    int const j = l;
+    long double const ltri = 3.0L * lf(j) - lf(3 * j + 1);
+    long double const lpre = 0.5L * (ltri + lf(j + m1) + lf(j - m1) + lf(j + m2) + lf(j - m2) + lf(j + m3) + lf(j - m3));
+    int const kmin = std::max(0, std::max(-m1, m2));
+    int const kmax = std::min(j, std::min(j - m1, j + m2));
+    long double sum = 0.0L;
+    for (int k = kmin; k <= kmax; ++k)
+    {
+        long double const lt = lf(k) + lf(j - k) + lf(j - m1 - k) + lf(j + m2 - k) + lf(m1 + k) + lf(-m2 + k);
+        long double const term = std::exp(lpre - lt);
+        sum += (k & 1) ? -term : term;
+    }
+    return double((((-m3) % 2) != 0) ? -sum : sum);
+}
+
+int fa_steinhardt(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, const uint32_t* ls,
+                  int n_l, int flags, float* d_qlm, float* d_ql, float* d_qlave, float* d_qlmave, float* d_wl,
+                  float* order)
+{
+    if (int rc = fa_init())
+        return rc;
+    return fa::steinhardt_run(box, d_points, n, nl, ls, n_l, flags, d_qlm, d_ql, d_qlave, d_qlmave, d_wl, order);
+}
+
+int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* d_cluster_idx,
+               uint32_t* d_cluster_sizes, uint32_t* num_clusters)
+{
+    if (int rc = fa_init())
+        return rc;
+    hipStream_t s = fa::stream();
+    if (n == 0)
+    {
+        *num_clusters = 0;
+        return FA_OK;
+    }
+    void* tmp = nullptr;
+    size_t const nb4 = sizeof(uint32_t) * size_t(n);
+    if (int rc = fa::scratch(7, 3 * nb4 + sizeof(uint64_t) * size_t(n) + 512, &tmp))
+        return rc;
+    uint32_t* parent = static_cast<uint32_t*>(tmp);
+    uint32_t* size = parent + n;
+    uint32_t* rank = size + n;
+    uint32_t* counter = rank + n;
+    uint64_t* keys = reinterpret_cast<uint64_t*>((reinterpret_cast<uintptr_t>(counter + 4) + 15) & ~uintptr_t(15));
+    unsigned const blocks = (n + kBlock - 1) / kBlock;
+    FA_KLAUNCH(uf_init_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size);
+    if (nl->num_bonds)
+        FA_KLAUNCH(uf_union_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                           nl->num_bonds, nl->neighbors, d_keep, parent);
+    FA_KLAUNCH(uf_flatten_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size);
+    FA_HIP(hipMemsetAsync(counter, 0, sizeof(uint32_t), s));
+    FA_KLAUNCH(uf_keys_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size, counter, keys);
+    FA_HIP(hipGetLastError());
+    uint32_t ncl = 0;
+    FA_HIP(hipMemcpyAsync(&ncl, counter, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (int rc = fa::sort_u64(keys, ncl))
+        return rc;
+    FA_KLAUNCH(uf_rank_kernel, dim3((ncl + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, ncl, keys, rank,
+                       d_cluster_sizes);
+    FA_KLAUNCH(uf_label_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, rank, d_cluster_idx);
+    FA_HIP(hipGetLastError());
+    FA_HIP(hipStreamSynchronize(s));
+    *num_clusters = ncl;
+    return FA_OK;
+}
+
+int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, uint32_t l,
+                   float q_threshold, uint32_t solid_threshold, int normalize_q, float* d_qlm, float* d_ql,
+                   float* d_ql_ij, uint32_t* d_num_connections, uint32_t* d_cluster_idx, uint32_t* d_cluster_sizes,
+                   uint32_t* num_clusters)
+{
+    if (int rc = fa_init())
+        return rc;
+    // SolidLiquid.cc:20-29
+    if (q_threshold < 0.0f)
+        return fa::fail(FA_ERR_INVALID,
+                        "SolidLiquid requires that the dot product cutoff q_threshold must be non-negative.");
+    float order = 0.0f;
+    if (int rc = fa::steinhardt_run(box, d_points, n, nl, &l, 1, 0, d_qlm, d_ql, nullptr, nullptr, nullptr, &order))
+        return rc;
+    hipStream_t s = fa::stream();
+    unsigned const blocks = (n + kBlock - 1) / kBlock;
+    if (n)
+        FA_KLAUNCH(solid_bonds_kernel, dim3(blocks), dim3(kBlock), 0, s, n, int(l), nl->neighbors, nl->offsets,
+                           reinterpret_cast<const float2*>(d_qlm), d_ql, normalize_q, q_threshold, d_ql_ij,
+                           d_num_connections);
+    FA_HIP(hipGetLastError());
+    void* keep = nullptr;
+    if (int rc = fa::scratch(8, std::max<uint64_t>(nl->num_bonds, 1) + 64, &keep))
+        return rc;
+    if (nl->num_bonds)
+        FA_KLAUNCH(solid_keep_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           s, nl->num_bonds, nl->neighbors, d_ql_ij, d_num_connections, q_threshold, solid_threshold,
+                           static_cast<uint8_t*>(keep));
+    FA_HIP(hipGetLastError());
+    return fa_cluster(n, nl, static_cast<uint8_t*>(keep), d_cluster_idx, d_cluster_sizes, num_clusters);
+}
+
+int fa_hexatic(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, uint32_t k, int weighted,
+               float* d_psi)
+{
+    if (int rc = fa_init())
+        return rc;
+    // Box.h:571-577 enforce2D
+    if (!box->is2D)
+        return fa::fail(FA_ERR_INVALID, "A 3D box was provided to a class that only supports 2D systems.");
+    if (n == 0)
+        return FA_OK;
+    fa::DevBox const db = fa::make_devbox(*box);
+    FA_KLAUNCH(hexatic_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), db, d_points, n,
+                       nl->neighbors, nl->weights, nl->offsets, float(k), weighted, reinterpret_cast<float2*>(d_psi));
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,69 @@
+// fa_prim.hip -- scan and sort primitives (rocPRIM device algorithms) used to
+// turn per-query counts into NeighborList offsets and to order cluster labels.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "fa_common.h"
+
+namespace fa {
+
+namespace {
+struct widen
+{
+    __device__ __host__ uint64_t operator()(uint32_t v) const { return v; }
+};
+} // namespace
+
+int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total)
+{
+    hipStream_t s = stream();
+    FA_HIP(hipMemsetAsync(d_out, 0, sizeof(uint32_t), s));
+    if (n == 0)
+    {
+        if (total)
+            *total = 0;
+        return FA_OK;
+    }
+    size_t bytes_scan = 0, bytes_red = 0;
+    FA_HIP(rocprim::inclusive_scan(nullptr, bytes_scan, d_in, d_out + 1, size_t(n), rocprim::plus<uint32_t>(), s));
+    auto it = rocprim::make_transform_iterator(d_in, widen());
+    uint64_t* d_total = nullptr;
+    FA_HIP(rocprim::reduce(nullptr, bytes_red, it, d_total, uint64_t(0), size_t(n), rocprim::plus<uint64_t>(), s));
+    void* tmp = nullptr;
+    size_t const need = bytes_scan + bytes_red + 512;
+    if (int rc = scratch(0, need, &tmp))
+        return rc;
+    char* base = static_cast<char*>(tmp);
+    d_total = reinterpret_cast<uint64_t*>(base);
+    void* t_scan = base + 256;
+    void* t_red = base + 256 + ((bytes_scan + 255) / 256) * 256;
+    FA_HIP(rocprim::inclusive_scan(t_scan, bytes_scan, d_in, d_out + 1, size_t(n), rocprim::plus<uint32_t>(), s));
+    if (total)
+    {
+        FA_HIP(rocprim::reduce(t_red, bytes_red, it, d_total, uint64_t(0), size_t(n), rocprim::plus<uint64_t>(), s));
+        FA_HIP(hipMemcpyAsync(total, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+    }
+    return FA_OK;
+}
+
+int sort_u64(uint64_t* d_keys, uint32_t n)
+{
+    if (n < 2)
+        return FA_OK;
+    hipStream_t s = stream();
+    size_t bytes = 0;
+    FA_HIP(rocprim::radix_sort_keys(nullptr, bytes, d_keys, d_keys, size_t(n), 0, 64, s));
+    void* tmp = nullptr;
+    size_t const kb = size_t(n) * sizeof(uint64_t);
+    if (int rc = scratch(1, bytes + kb + 256, &tmp))
+        return rc;
+    uint64_t* alt = reinterpret_cast<uint64_t*>(static_cast<char*>(tmp));
+    void* t = static_cast<char*>(tmp) + ((kb + 255) / 256) * 256;
+    FA_HIP(rocprim::radix_sort_keys(t, bytes, d_keys, alt, size_t(n), 0, 64, s));
+    FA_HIP(hipMemcpyAsync(d_keys, alt, kb, hipMemcpyDeviceToDevice, s));
+    return FA_OK;
+}
+
+} // namespace fa

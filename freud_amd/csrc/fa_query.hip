@@ -1,0 +1,980 @@
+// fa_query.hip -- ball and k-nearest-neighbour queries producing freud's
+// NeighborList (freud/locality/NeighborQuery.h:434-481 toNeighborList,
+// AABBQuery.cc:77-281, LinkCell.cc:496-679).
+//
+// Ball query: a count pass (one thread per query point), an exclusive scan of
+// the counts into segment offsets, and a fill pass.  The fill pass stages each
+// block's keys (j << 5 | image) in LDS, sorts every query point's segment by
+// point index (the reference's (i, j, w, d) order, NeighborBond.h:80-95) and
+// then writes the block's contiguous bond range cooperatively, one bond per
+// lane, so the NeighborList stores are coalesced.
+//
+// kNN: one thread per query point walks Chebyshev rings of cells, keeping the
+// k best (r_sq, j) keys in registers, and stops once the k-th distance is
+// inside the completely searched radius.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "fa_common.h"
+
+namespace {
+
+using fa::AABB_FORMULA;
+using fa::DevBox;
+using fa::DevGrid;
+using fa::LINKCELL_FORMULA;
+
+constexpr int kBlock = 256;
+constexpr int kKeyBytes = 32768; // LDS staging per block
+
+__device__ inline uint32_t float_order_bits(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+template<int FORMULA>
+__global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                            uint32_t nq, float r_max_sq, float r_min_sq,
+                                                            int exclude_ii, int R, uint32_t* __restrict__ counts)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq)
+        return;
+    float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
+    uint32_t c = 0;
+    fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
+                                    [&](uint32_t j, float, float, float, float r_sq, int) {
+                                        if (exclude_ii && j == i)
+                                            return;
+                                        if (r_sq < r_max_sq && r_sq >= r_min_sq)
+                                            ++c;
+                                    });
+    counts[i] = c;
+}
+
+template<typename K> __device__ inline void insertion_sort(K* a, uint32_t n)
+{
+    for (uint32_t x = 1; x < n; ++x)
+    {
+        K const v = a[x];
+        uint32_t y = x;
+        while (y > 0 && a[y - 1] > v)
+        {
+            a[y] = a[y - 1];
+            --y;
+        }
+        a[y] = v;
+    }
+}
+
+// Writes bond t of a block given its query index i and key.
+template<int FORMULA, bool BYDIST>
+__device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpts, const float* __restrict__ pts,
+                                  uint32_t i, uint64_t key, uint64_t bidx, uint32_t* __restrict__ nb_out,
+                                  float* __restrict__ d_out, float* __restrict__ w_out, float* __restrict__ v_out)
+{
+    uint32_t const jc = uint32_t(key);
+    uint32_t const j = jc >> 5;
+    int const code = int(jc & 31u);
+    float rx, ry, rz;
+    fa::bond_vector<FORMULA>(b, qpts[3 * i], qpts[3 * i + 1], qpts[3 * i + 2], pts[3 * j], pts[3 * j + 1],
+                             pts[3 * j + 2], code, rx, ry, rz);
+    float const r_sq = (rx * rx + ry * ry) + rz * rz;
+    reinterpret_cast<uint2*>(nb_out)[bidx] = make_uint2(i, j);
+    d_out[bidx] = sqrtf(r_sq);
+    w_out[bidx] = 1.0f;
+    v_out[3 * bidx] = rx;
+    v_out[3 * bidx + 1] = ry;
+    v_out[3 * bidx + 2] = rz;
+}
+
+template<int FORMULA, bool BYDIST>
+__global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, const float* __restrict__ pts, float r_max_sq,
+                                                           float r_min_sq, int exclude_ii, int R,
+                                                           const uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                                           float* __restrict__ w_out, float* __restrict__ v_out,
+                                                           void* __restrict__ gkeys_raw)
+{
+    using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
+    constexpr uint32_t CAP = kKeyBytes / sizeof(K);
+    __shared__ K s_keys[CAP];
+    __shared__ uint32_t s_off[kBlock + 1];
+    uint32_t const tid = threadIdx.x;
+    uint32_t const i0 = blockIdx.x * kBlock;
+    uint32_t const nloc = min(uint32_t(kBlock), nq - i0);
+    uint32_t const base = offsets[i0];
+    if (tid <= nloc)
+        s_off[tid] = offsets[i0 + tid] - base;
+    __syncthreads();
+    uint32_t const total = s_off[nloc];
+    K* keys = (total <= CAP) ? s_keys : reinterpret_cast<K*>(gkeys_raw) + base;
+    if (tid < nloc)
+    {
+        uint32_t const i = i0 + tid;
+        float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
+        uint32_t const beg = s_off[tid];
+        uint32_t const end = s_off[tid + 1];
+        uint32_t k = beg;
+        fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
+                                        [&](uint32_t j, float, float, float, float r_sq, int code) {
+                                            if (exclude_ii && j == i)
+                                                return;
+                                            if (r_sq < r_max_sq && r_sq >= r_min_sq && k < end)
+                                            {
+                                                uint32_t const jc = (j << 5) | uint32_t(code);
+                                                if (BYDIST)
+                                                    keys[k] = K((uint64_t(float_order_bits(sqrtf(r_sq))) << 32) | jc);
+                                                else
+                                                    keys[k] = K(jc);
+                                                ++k;
+                                            }
+                                        });
+        insertion_sort(keys + beg, k - beg);
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < total; t += kBlock)
+    {
+        uint32_t lo = 0, hi = nloc;
+        while (hi - lo > 1)
+        {
+            uint32_t const mid = (lo + hi) >> 1;
+            if (s_off[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        write_bond<FORMULA, BYDIST>(b, qpts, pts, i0 + lo, uint64_t(keys[t]), uint64_t(base) + t, nb_out, d_out,
+                                    w_out, v_out);
+    }
+}
+
+// Segments with the reference's convention (0 for query points without bonds).
+__global__ void counts_segments_kernel(const uint32_t* __restrict__ offsets, uint32_t nq, uint32_t* __restrict__ counts,
+                                       uint32_t* __restrict__ segments)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq)
+        return;
+    uint32_t const c = offsets[i + 1] - offsets[i];
+    counts[i] = c;
+    segments[i] = c ? offsets[i] : 0u;
+}
+
+// ------------------------------------------------------------------ kNN
+template<int KMAX> struct TopK
+{
+    uint64_t v[KMAX];
+    __device__ void init()
+    {
+#pragma unroll
+        for (int m = 0; m < KMAX; ++m)
+            v[m] = ~0ull;
+    }
+    // keep the KMAX smallest keys in ascending order
+    __device__ void insert(uint64_t key)
+    {
+        if (key >= v[KMAX - 1])
+            return;
+#pragma unroll
+        for (int m = KMAX - 1; m > 0; --m)
+            v[m] = min(v[m], max(v[m - 1], key));
+        v[0] = min(v[0], key);
+    }
+    __device__ uint64_t at(uint32_t idx) const
+    {
+        uint64_t r = ~0ull;
+#pragma unroll
+        for (int m = 0; m < KMAX; ++m)
+            if (uint32_t(m) == idx)
+                r = v[m];
+        return r;
+    }
+};
+
+// AABB formula kNN (AABBQuery.cc:152-281).  The candidate set is every (j,
+// image) with r_sq < rlim_sq (rlim = the last ball radius of the reference's
+// expanding search) and sqrt(r_sq) >= r_min; the k smallest by (r_sq, j) are
+// kept.  A query whose answer needs radii beyond half the smallest plane
+// distance (where one point can appear through two images) is flagged for the
+// exhaustive kernel below.
+template<int KMAX>
+__global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                          uint32_t nq, uint32_t k, float rlim_sq, float rlim,
+                                                          float r_min, int exclude_ii, float wmin,
+                                                          float half_min_plane, int rmax_ring,
+                                                          uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
+                                                          uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq)
+        return;
+    float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
+    TopK<KMAX> best;
+    best.init();
+    uint32_t found = 0;
+    bool extended = false;
+    for (int R = 0; R <= rmax_ring; ++R)
+    {
+        fa::for_each_candidate<AABB_FORMULA>(b, g, qx, qy, qz, R, true,
+                                             [&](uint32_t j, float, float, float, float r_sq, int code) {
+                                                 if (exclude_ii && j == i)
+                                                     return;
+                                                 if (!(r_sq < rlim_sq))
+                                                     return;
+                                                 if (!(sqrtf(r_sq) >= r_min))
+                                                     return;
+                                                 ++found;
+                                                 best.insert((uint64_t(__float_as_uint(r_sq)) << 32)
+                                                             | ((j << 5) | uint32_t(code)));
+                                             });
+        float const complete = float(R) * wmin * (1.0f - 1e-4f);
+        if (found >= k)
+        {
+            float const kth = sqrtf(__uint_as_float(uint32_t(best.at(k - 1) >> 32)));
+            if (kth < complete)
+                break;
+        }
+        if (complete >= rlim)
+            break;
+        if (complete >= half_min_plane && rlim > half_min_plane)
+        {
+            extended = true;
+            break;
+        }
+    }
+    if (extended)
+    {
+        uint32_t const slot = atomicAdd(nflag, 1u);
+        flagged[slot] = i;
+        counts[i] = 0;
+        return;
+    }
+    uint32_t const n = min(found, k);
+    counts[i] = n;
+    for (uint32_t m = 0; m < n; ++m)
+        sel[uint64_t(i) * k + m] = best.at(m);
+}
+
+// LinkCell formula kNN (LinkCell.cc:575-679): every point in cells within
+// max_range - 1 Chebyshev shells of the reference's grid, each cell once.
+template<int KMAX>
+__global__ __launch_bounds__(kBlock) void knn_linkcell_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                              uint32_t nq, uint32_t k, float r_max_sq,
+                                                              float r_min_sq, int exclude_ii, int R,
+                                                              uint32_t* __restrict__ counts,
+                                                              uint64_t* __restrict__ sel)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq)
+        return;
+    float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
+    TopK<KMAX> best;
+    best.init();
+    uint32_t found = 0;
+    fa::for_each_candidate<LINKCELL_FORMULA>(b, g, qx, qy, qz, R, false,
+                                             [&](uint32_t j, float, float, float, float r_sq, int) {
+                                                 if (exclude_ii && j == i)
+                                                     return;
+                                                 if (r_sq < r_max_sq && r_sq >= r_min_sq)
+                                                 {
+                                                     ++found;
+                                                     best.insert((uint64_t(__float_as_uint(r_sq)) << 32)
+                                                                 | ((j << 5) | 13u));
+                                                 }
+                                             });
+    uint32_t const n = min(found, k);
+    counts[i] = n;
+    for (uint32_t m = 0; m < n; ++m)
+        sel[uint64_t(i) * k + m] = best.at(m);
+}
+
+// Exhaustive kNN for flagged queries (one block per query): every point and
+// every image in {-1,0,1}^d; per point the closest image is kept (the
+// reference's minimum-distance map, AABBQuery.cc:188-205); a point whose
+// closest image is nearer than r_min is dropped (AABBQuery.cc:197-201, 235-244).
+__global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const float* __restrict__ pts, uint32_t np,
+                                                                const float* __restrict__ qpts,
+                                                                const uint32_t* __restrict__ flagged, uint32_t k,
+                                                                float rlim_sq, float r_min, int exclude_ii,
+                                                                uint64_t* __restrict__ work,
+                                                                uint32_t* __restrict__ counts,
+                                                                uint64_t* __restrict__ sel)
+{
+    uint32_t const i = flagged[blockIdx.x];
+    uint64_t* w = work + uint64_t(blockIdx.x) * np;
+    float const qx = qpts[3 * i], qy = qpts[3 * i + 1];
+    float const qz = b.is2D ? 0.0f : qpts[3 * i + 2];
+    int const nimg_z = b.is2D ? 0 : 1;
+    for (uint32_t j = threadIdx.x; j < np; j += blockDim.x)
+    {
+        uint64_t bestk = ~0ull;
+        if (!(exclude_ii && j == i))
+        {
+            float const px = pts[3 * j], py = pts[3 * j + 1];
+            float const pz = b.is2D ? 0.0f : pts[3 * j + 2];
+            for (int ix = -1; ix <= 1; ++ix)
+                for (int iy = -1; iy <= 1; ++iy)
+                    for (int iz = -nimg_z; iz <= nimg_z; ++iz)
+                    {
+                        int const code = (ix + 1) * 9 + (iy + 1) * 3 + (iz + 1);
+                        float const rx = px - (qx + b.img[code][0]);
+                        float const ry = py - (qy + b.img[code][1]);
+                        float const rz = pz - (qz + b.img[code][2]);
+                        float const r_sq = (rx * rx + ry * ry) + rz * rz;
+                        if (r_sq < rlim_sq)
+                        {
+                            uint64_t const key = (uint64_t(__float_as_uint(r_sq)) << 32) | ((j << 5) | uint32_t(code));
+                            bestk = min(bestk, key);
+                        }
+                    }
+            if (bestk != ~0ull && !(sqrtf(__uint_as_float(uint32_t(bestk >> 32))) >= r_min))
+                bestk = ~0ull;
+        }
+        w[j] = bestk;
+    }
+    __syncthreads();
+    // k rounds of block-wide minimum extraction
+    __shared__ uint64_t s_red[kBlock];
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0)
+        s_cnt = 0;
+    __syncthreads();
+    for (uint32_t m = 0; m < k; ++m)
+    {
+        uint64_t lm = ~0ull;
+        for (uint32_t j = threadIdx.x; j < np; j += blockDim.x)
+            lm = min(lm, w[j]);
+        s_red[threadIdx.x] = lm;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1)
+        {
+            if (threadIdx.x < s)
+                s_red[threadIdx.x] = min(s_red[threadIdx.x], s_red[threadIdx.x + s]);
+            __syncthreads();
+        }
+        uint64_t const mn = s_red[0];
+        __syncthreads();
+        if (mn == ~0ull)
+            break;
+        if (threadIdx.x == 0)
+        {
+            sel[uint64_t(i) * k + m] = mn;
+            s_cnt = m + 1;
+        }
+        uint32_t const jm = uint32_t(mn) >> 5;
+        if (jm % blockDim.x == threadIdx.x)
+            w[jm] = ~0ull;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        counts[i] = s_cnt;
+}
+
+// Writes a kNN result: per query, its selected keys sorted into the
+// NeighborList order ((i, j) or (i, d, j)).
+template<int FORMULA, bool BYDIST>
+__global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, const float* __restrict__ qpts, uint32_t nq,
+                                                           const float* __restrict__ pts, uint32_t k,
+                                                           const uint64_t* __restrict__ sel,
+                                                           const uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                                           float* __restrict__ w_out, float* __restrict__ v_out,
+                                                           uint64_t* __restrict__ tmp)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq)
+        return;
+    uint32_t const off = offsets[i];
+    uint32_t const n = offsets[i + 1] - off;
+    uint64_t* keys = tmp + uint64_t(i) * k;
+    for (uint32_t m = 0; m < n; ++m)
+    {
+        uint64_t const s = sel[uint64_t(i) * k + m];
+        uint32_t const jc = uint32_t(s);
+        if (BYDIST)
+        {
+            float const d = sqrtf(__uint_as_float(uint32_t(s >> 32)));
+            keys[m] = (uint64_t(float_order_bits(d)) << 32) | jc;
+        }
+        else
+            keys[m] = jc;
+    }
+    insertion_sort(keys, n);
+    for (uint32_t m = 0; m < n; ++m)
+        write_bond<FORMULA, BYDIST>(b, qpts, pts, i, keys[m], uint64_t(off) + m, nb_out, d_out, w_out, v_out);
+}
+
+// ------------------------------------------------------------------ all pairs
+__global__ void all_pairs_kernel(DevBox b, const float* __restrict__ pts, uint32_t np, const float* __restrict__ qpts,
+                                 uint32_t nq, int exclude_ii, uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                 float* __restrict__ w_out, float* __restrict__ v_out)
+{
+    uint64_t const t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= uint64_t(np) * nq)
+        return;
+    uint32_t const i = uint32_t(t / np), j = uint32_t(t % np);
+    if (exclude_ii && i == j)
+        return;
+    // NeighborList.cc:103-125: row start i*np - min(i, np); the vector is wrap(q_i - p_j)
+    uint64_t idx = uint64_t(i) * np + j;
+    if (exclude_ii)
+        idx -= uint64_t(min(i, np)) + ((i < np && j > i) ? 1 : 0);
+    float rx, ry, rz;
+    fa::wrap(b, qpts[3 * i] - pts[3 * j], qpts[3 * i + 1] - pts[3 * j + 1], qpts[3 * i + 2] - pts[3 * j + 2], rx, ry,
+             rz);
+    reinterpret_cast<uint2*>(nb_out)[idx] = make_uint2(i, j);
+    d_out[idx] = sqrtf((rx * rx + ry * ry) + rz * rz);
+    w_out[idx] = 1.0f;
+    v_out[3 * idx] = rx;
+    v_out[3 * idx + 1] = ry;
+    v_out[3 * idx + 2] = rz;
+}
+
+} // namespace
+
+namespace fa {
+
+int nlist_reserve(fa_nlist* nl, uint64_t nb, uint32_t nq)
+{
+    if (nl->capacity < nb || !nl->neighbors)
+    {
+        device_free(nl->neighbors);
+        device_free(nl->distances);
+        device_free(nl->weights);
+        device_free(nl->vectors);
+        nl->neighbors = nullptr;
+        nl->distances = nl->weights = nl->vectors = nullptr;
+        nl->capacity = 0;
+        uint64_t const cap = std::max<uint64_t>(nb, 1);
+        if (int rc = device_alloc((void**) &nl->neighbors, sizeof(uint32_t) * 2 * cap))
+            return rc;
+        if (int rc = device_alloc((void**) &nl->distances, sizeof(float) * cap))
+            return rc;
+        if (int rc = device_alloc((void**) &nl->weights, sizeof(float) * cap))
+            return rc;
+        if (int rc = device_alloc((void**) &nl->vectors, sizeof(float) * 3 * cap))
+            return rc;
+        nl->capacity = cap;
+    }
+    if (!nl->counts || nl->num_query_points < nq)
+    {
+        device_free(nl->counts);
+        device_free(nl->segments);
+        device_free(nl->offsets);
+        nl->counts = nl->segments = nl->offsets = nullptr;
+        if (int rc = device_alloc((void**) &nl->counts, sizeof(uint32_t) * std::max<uint32_t>(nq, 1)))
+            return rc;
+        if (int rc = device_alloc((void**) &nl->segments, sizeof(uint32_t) * std::max<uint32_t>(nq, 1)))
+            return rc;
+        if (int rc = device_alloc((void**) &nl->offsets, sizeof(uint32_t) * (size_t(nq) + 1)))
+            return rc;
+    }
+    nl->num_query_points = nq;
+    return FA_OK;
+}
+
+int nlist_finish_counts(fa_nlist* nl)
+{
+    uint32_t const nq = nl->num_query_points;
+    if (nq == 0)
+        return FA_OK;
+    FA_KLAUNCH(counts_segments_kernel, dim3((nq + 255) / 256), dim3(256), 0, stream(), nl->offsets, nq,
+                       nl->counts, nl->segments);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+// Cell-grid shape for a stencil of half-width R covering r: every cell is at
+// least r/R (+0.01 %) wide along each plane normal, and the cell count stays
+// within a small multiple of the point count.
+void grid_dims_for(const fa_box& box, uint32_t n, float r, int R, int dims[3])
+{
+    float npd[3];
+    nearest_plane_distance(box, npd);
+    double const w = double(r) * (1.0 + 1e-4) / R;
+    for (int k = 0; k < 3; ++k)
+    {
+        double d = (k == 2 && box.is2D) ? 1.0 : std::floor(double(npd[k]) / w);
+        if (!(d >= 1.0))
+            d = 1.0;
+        dims[k] = int(std::min(d, 1.0e6));
+    }
+    double const max_cells = std::max<double>(8.0 * n, 64.0);
+    while (double(dims[0]) * dims[1] * dims[2] > max_cells)
+    {
+        for (int k = 0; k < 3; ++k)
+            if (!(k == 2 && box.is2D))
+                dims[k] = std::max(1, int(dims[k] * 0.8));
+    }
+}
+
+int ensure_grid(fa_nq* nq, const int dims[3])
+{
+    ++nq->tick;
+    int victim = 0;
+    for (int s = 0; s < fa_nq::kGrids; ++s)
+    {
+        if (nq->grid_valid[s] && nq->grid_dims[s][0] == dims[0] && nq->grid_dims[s][1] == dims[1]
+            && nq->grid_dims[s][2] == dims[2])
+        {
+            nq->grid_used[s] = nq->tick;
+            nq->cur = &nq->grids[s];
+            return FA_OK;
+        }
+        if (!nq->grid_valid[s] || (nq->grid_valid[victim] && nq->grid_used[s] < nq->grid_used[victim]))
+            victim = s;
+    }
+    nq->grid_valid[victim] = false;
+    if (int rc = build_grid(nq->grids[victim], nq->box, nq->d_points, nq->n, dims))
+        return rc;
+    nq->grid_valid[victim] = true;
+    nq->grid_used[victim] = nq->tick;
+    for (int k = 0; k < 3; ++k)
+        nq->grid_dims[victim][k] = dims[k];
+    nq->cur = &nq->grids[victim];
+    return FA_OK;
+}
+
+// NeighborQuery::validateQueryArgs / validateNearestNeighborArgs and the
+// per-point iterator checks (NeighborQuery.h:195-292, 317-328; AABBQuery.h:62-69).
+int validate_query_args(const fa_nq* nq, fa_query_args& a, uint32_t n_query)
+{
+    constexpr uint32_t kDefaultK = 0xffffffffu;
+    if (a.mode == FA_MODE_NONE)
+    {
+        if (a.num_neighbors != kDefaultK)
+            a.mode = FA_MODE_NEAREST;
+        else if (a.r_max != -1.0f)
+            a.mode = FA_MODE_BALL;
+    }
+    if (a.mode == FA_MODE_BALL)
+    {
+        if (a.r_max == -1.0f)
+            return fail(FA_ERR_RUNTIME, "You must set r_max in the query arguments when performing ball queries.");
+        if (a.num_neighbors != kDefaultK)
+            return fail(FA_ERR_RUNTIME,
+                        "You cannot set num_neighbors in the query arguments when performing ball queries.");
+    }
+    else if (a.mode == FA_MODE_NEAREST)
+    {
+        if (a.num_neighbors == kDefaultK)
+            return fail(FA_ERR_RUNTIME, "You must set num_neighbors in the query arguments when performing number "
+                                        "of neighbor queries.");
+        if (a.r_max == -1.0f)
+            a.r_max = std::numeric_limits<float>::infinity();
+        if (nq->kind == FA_NQ_AABB)
+        {
+            if (a.scale == -1.0f)
+                a.scale = 1.1f;
+            else if (a.scale <= 1.0f)
+                return fail(FA_ERR_RUNTIME, "The scale query argument must be greater than 1.");
+            if (a.r_guess == -1.0f)
+            {
+                float const vol = box_volume(nq->box);
+                float const r_guess = std::cbrt((3.0f * float(a.num_neighbors) * vol)
+                                                / (4.0f * float(M_PI) * float(nq->n)));
+                float npd[3];
+                nearest_plane_distance(nq->box, npd);
+                float mpd = std::min(npd[0], npd[1]);
+                if (!nq->box.is2D)
+                    mpd = std::min(mpd, npd[2]);
+                a.r_guess = std::min(r_guess, mpd / 2.0f);
+            }
+            a.r_guess = std::min(a.r_guess, a.r_max);
+        }
+    }
+    else
+        return fail(FA_ERR_RUNTIME, "Unknown mode");
+    if (n_query == 0)
+        return FA_OK;
+    if (a.r_max <= 0)
+        return fail(FA_ERR_INVALID, "NeighborQuery requires r_max to be positive.");
+    if (a.r_max <= a.r_min)
+        return fail(FA_ERR_INVALID, "NeighborQuery requires that r_max must be greater than r_min.");
+    if (nq->kind == FA_NQ_AABB && a.mode == FA_MODE_BALL)
+    {
+        float npd[3];
+        nearest_plane_distance(nq->box, npd);
+        if (npd[0] <= a.r_max * 2.0 || npd[1] <= a.r_max * 2.0 || (!nq->box.is2D && npd[2] <= a.r_max * 2.0))
+            return fail(FA_ERR_RUNTIME, "The AABBQuery r_max is too large for this box.");
+    }
+    return FA_OK;
+}
+
+} // namespace fa
+
+namespace {
+
+template<int KMAX>
+int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, uint32_t nq, uint32_t k, float rlim,
+                    float r_min, int exclude_ii, float wmin, float hmp, int rmax_ring, uint32_t* counts,
+                    uint64_t* sel, uint32_t* flagged, uint32_t* nflag)
+{
+    float const rlim_sq = rlim * rlim;
+    FA_KLAUNCH(knn_aabb_kernel<KMAX>, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), db, g,
+                       qpts, nq, k, rlim_sq, rlim, r_min, exclude_ii, wmin, hmp, rmax_ring, counts, sel, flagged,
+                       nflag);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+template<int KMAX>
+int launch_knn_lc(const fa::DevBox& db, const DevGrid& g, const float* qpts, uint32_t nq, uint32_t k, float r_max,
+                  float r_min, int exclude_ii, int R, uint32_t* counts, uint64_t* sel)
+{
+    FA_KLAUNCH(knn_linkcell_kernel<KMAX>, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(),
+                       db, g, qpts, nq, k, r_max * r_max, r_min * r_min, exclude_ii, R, counts, sel);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args& a, int by_distance, fa_nlist* out)
+{
+    hipStream_t s = fa::stream();
+    int const R = 1;
+    int dims[3];
+    fa::grid_dims_for(nq->box, nq->n, a.r_max, R, dims);
+    if (int rc = fa::ensure_grid(nq, dims))
+        return rc;
+    DevGrid const g = nq->cur->dev();
+    fa::DevBox const& db = nq->cur->dbox;
+    if (int rc = fa::nlist_reserve(out, out->capacity, n_query))
+        return rc;
+    float const r_max_sq = a.r_max * a.r_max, r_min_sq = a.r_min * a.r_min;
+    unsigned const blocks = (n_query + kBlock - 1) / kBlock;
+    if (nq->kind == FA_NQ_AABB)
+        FA_KLAUNCH(ball_count_kernel<AABB_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query,
+                           r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
+    else
+        FA_KLAUNCH(ball_count_kernel<LINKCELL_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts,
+                           n_query, r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
+    FA_HIP(hipGetLastError());
+    uint64_t total = 0;
+    if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
+        return rc;
+    if (total > 0xffffffffull)
+        return fa::fail(FA_ERR_RUNTIME, "NeighborList would exceed 2^32 bonds");
+    if (int rc = fa::nlist_reserve(out, total, n_query))
+        return rc;
+    out->num_bonds = total;
+    void* gkeys = nullptr;
+    if (int rc = fa::scratch(3, std::max<uint64_t>(total, 1) * (by_distance ? 8 : 4), &gkeys))
+        return rc;
+    auto launch = [&](auto kern) {
+        FA_PROF_BEGIN("ball_fill_kernel");
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query, nq->d_points, r_max_sq,
+                           r_min_sq, a.exclude_ii, R, out->offsets, out->neighbors, out->distances, out->weights,
+                           out->vectors, gkeys);
+        FA_PROF_END();
+    };
+    if (n_query)
+    {
+        if (nq->kind == FA_NQ_AABB)
+            by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true>) : launch(ball_fill_kernel<AABB_FORMULA, false>);
+        else
+            by_distance ? launch(ball_fill_kernel<LINKCELL_FORMULA, true>)
+                        : launch(ball_fill_kernel<LINKCELL_FORMULA, false>);
+        FA_HIP(hipGetLastError());
+    }
+    return fa::nlist_finish_counts(out);
+}
+
+// Radius of the last ball of the reference's expanding kNN search
+// (AABBQuery.cc:174-263): r_cur starts at r_guess and is multiplied by scale
+// until it reaches r_max or the largest plane distance.
+float knn_radius_limit(const fa_nq* nq, const fa_query_args& a)
+{
+    float npd[3];
+    fa::nearest_plane_distance(nq->box, npd);
+    float max_pd = std::max(npd[0], npd[1]);
+    if (!nq->box.is2D)
+        max_pd = std::max(max_pd, npd[2]);
+    float r_cur = a.r_guess;
+    float last = std::min(r_cur, a.r_max);
+    for (int it = 0; it < 100000; ++it)
+    {
+        last = std::min(r_cur, a.r_max);
+        r_cur *= a.scale;
+        if (r_cur >= a.r_max || r_cur >= max_pd)
+            break;
+    }
+    return last;
+}
+
+int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args& a, int by_distance, fa_nlist* out)
+{
+    hipStream_t s = fa::stream();
+    uint32_t const k = a.num_neighbors;
+    if (int rc = fa::nlist_reserve(out, out->capacity, n_query))
+        return rc;
+    if (n_query == 0 || k == 0)
+    {
+        FA_HIP(hipMemsetAsync(out->offsets, 0, sizeof(uint32_t) * (size_t(n_query) + 1), s));
+        out->num_bonds = 0;
+        return fa::nlist_finish_counts(out);
+    }
+    // scratch: selections (nq*k u64), write temp (nq*k u64), flagged list
+    void* tmp = nullptr;
+    uint64_t const selb = uint64_t(n_query) * k * sizeof(uint64_t);
+    if (int rc = fa::scratch(4, 2 * selb + sizeof(uint32_t) * (size_t(n_query) + 64), &tmp))
+        return rc;
+    uint64_t* sel = static_cast<uint64_t*>(tmp);
+    uint64_t* wtmp = sel + uint64_t(n_query) * k;
+    uint32_t* nflag = reinterpret_cast<uint32_t*>(wtmp + uint64_t(n_query) * k);
+    uint32_t* flagged = nflag + 16;
+    FA_HIP(hipMemsetAsync(nflag, 0, sizeof(uint32_t), s));
+    float npd[3];
+    fa::nearest_plane_distance(nq->box, npd);
+    float min_pd = std::min(npd[0], npd[1]);
+    if (!nq->box.is2D)
+        min_pd = std::min(min_pd, npd[2]);
+    const fa::DevBox* dbp = nullptr;
+    uint32_t h_nflag = 0;
+    float rlim = 0.0f;
+    if (nq->kind == FA_NQ_AABB)
+    {
+        rlim = knn_radius_limit(nq, a);
+        // cells sized so that the first ring typically holds ~2k points
+        float const vol = fa::box_volume(nq->box);
+        double const dens = double(nq->n) / std::max(double(vol), 1e-30);
+        double const dimn = nq->box.is2D ? 2.0 : 3.0;
+        double rk = nq->box.is2D ? std::sqrt(2.0 * k / (M_PI * dens)) : std::cbrt(3.0 * 2.0 * k / (4.0 * M_PI * dens));
+        (void) dimn;
+        float const wcell = float(std::max(rk, 1e-6));
+        int dims[3];
+        fa::grid_dims_for(nq->box, nq->n, wcell, 1, dims);
+        if (int rc = fa::ensure_grid(nq, dims))
+            return rc;
+        dbp = &nq->cur->dbox;
+        float wmin = std::numeric_limits<float>::infinity();
+        for (int d = 0; d < (nq->box.is2D ? 2 : 3); ++d)
+            wmin = std::min(wmin, npd[d] / float(dims[d]));
+        int const rmax_ring = std::max(dims[0], std::max(dims[1], dims[2])) * 2 + 2;
+        int rc;
+        if (k <= 8)
+            rc = launch_knn_aabb<8>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
+                                    min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+        else if (k <= 16)
+            rc = launch_knn_aabb<16>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
+                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+        else if (k <= 32)
+            rc = launch_knn_aabb<32>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
+                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+        else if (k <= 64)
+            rc = launch_knn_aabb<64>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
+                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+        else
+        {
+            // every query takes the exhaustive path
+            rc = FA_OK;
+            std::vector<uint32_t> all(n_query);
+            for (uint32_t t = 0; t < n_query; ++t)
+                all[t] = t;
+            FA_HIP(hipMemcpyAsync(flagged, all.data(), sizeof(uint32_t) * n_query, hipMemcpyHostToDevice, s));
+            FA_HIP(hipMemcpyAsync(nflag, &n_query, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            FA_HIP(hipStreamSynchronize(s));
+        }
+        if (rc)
+            return rc;
+        FA_HIP(hipMemcpyAsync(&h_nflag, nflag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+        if (h_nflag)
+        {
+            // bounded work space: at most ~256 MiB of per-query candidate keys at once
+            uint64_t const per_q = uint64_t(nq->n) * sizeof(uint64_t);
+            uint32_t const chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (256ull << 20) / per_q)));
+            for (uint32_t c0 = 0; c0 < h_nflag; c0 += chunk)
+            {
+                uint32_t const nc = std::min(chunk, h_nflag - c0);
+                void* work = nullptr;
+                if (int rc2 = fa::scratch(5, uint64_t(nc) * nq->n * sizeof(uint64_t) + 64, &work))
+                    return rc2;
+                FA_KLAUNCH(knn_exhaustive_kernel, dim3(nc), dim3(kBlock), 0, s, *dbp, nq->d_points, nq->n,
+                                   qpts, flagged + c0, k, rlim * rlim, a.r_min, a.exclude_ii,
+                                   static_cast<uint64_t*>(work), out->counts, sel);
+                FA_HIP(hipGetLastError());
+            }
+        }
+    }
+    else
+    {
+        // the reference's LinkCell grid and shell range (LinkCell.cc:584-585)
+        int dims[3] = {nq->lc_dim[0], nq->lc_dim[1], nq->lc_dim[2]};
+        if (int rc = fa::ensure_grid(nq, dims))
+            return rc;
+        dbp = &nq->cur->dbox;
+        int const max_range = int(std::ceil(min_pd / (2 * nq->cell_width))) + 1;
+        int const R = std::max(0, max_range - 1);
+        int rc;
+        if (k <= 8)
+            rc = launch_knn_lc<8>(*dbp, nq->cur->dev(), qpts, n_query, k, a.r_max, a.r_min, a.exclude_ii, R,
+                                  out->counts, sel);
+        else if (k <= 16)
+            rc = launch_knn_lc<16>(*dbp, nq->cur->dev(), qpts, n_query, k, a.r_max, a.r_min, a.exclude_ii, R,
+                                   out->counts, sel);
+        else if (k <= 32)
+            rc = launch_knn_lc<32>(*dbp, nq->cur->dev(), qpts, n_query, k, a.r_max, a.r_min, a.exclude_ii, R,
+                                   out->counts, sel);
+        else if (k <= 64)
+            rc = launch_knn_lc<64>(*dbp, nq->cur->dev(), qpts, n_query, k, a.r_max, a.r_min, a.exclude_ii, R,
+                                   out->counts, sel);
+        else
+            return fa::fail(FA_ERR_RUNTIME, "LinkCell nearest-neighbour queries support num_neighbors <= 64");
+        if (rc)
+            return rc;
+    }
+    uint64_t total = 0;
+    if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
+        return rc;
+    if (int rc = fa::nlist_reserve(out, total, n_query))
+        return rc;
+    out->num_bonds = total;
+    unsigned const blocks = (n_query + kBlock - 1) / kBlock;
+    auto launch = [&](auto kern) {
+        FA_PROF_BEGIN("knn_write_kernel");
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, *dbp, qpts, n_query, nq->d_points, k, sel,
+                           out->offsets, out->neighbors, out->distances, out->weights, out->vectors, wtmp);
+        FA_PROF_END();
+    };
+    if (nq->kind == FA_NQ_AABB)
+        by_distance ? launch(knn_write_kernel<AABB_FORMULA, true>) : launch(knn_write_kernel<AABB_FORMULA, false>);
+    else
+        by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true>)
+                    : launch(knn_write_kernel<LINKCELL_FORMULA, false>);
+    FA_HIP(hipGetLastError());
+    return fa::nlist_finish_counts(out);
+}
+
+} // namespace
+
+extern "C" {
+
+int fa_nq_create(const fa_box* box, const float* d_points, uint32_t n, int kind, float cell_width, fa_nq** out)
+{
+    *out = nullptr;
+    if (int rc = fa_init())
+        return rc;
+    // NeighborQuery.h:97-100
+    if (n == 0)
+        return fa::fail(FA_ERR_INVALID, "Cannot create a NeighborQuery with 0 particles.");
+    if (n >= (1u << 27))
+        return fa::fail(FA_ERR_INVALID, "freud_amd supports fewer than 2^27 points per NeighborQuery");
+    fa_nq* nq = new fa_nq();
+    nq->box = *box;
+    nq->d_points = d_points;
+    nq->n = n;
+    nq->kind = kind;
+    nq->cell_width = cell_width;
+    if (kind == FA_NQ_LINKCELL)
+    {
+        // LinkCell.cc:222-260, 278-314
+        if (cell_width == 0.0f)
+        {
+            unsigned const desired = std::max(n / 10u, 1u);
+            nq->cell_width = std::cbrt(fa::box_volume(*box) / float(desired));
+        }
+        float npd[3];
+        fa::nearest_plane_distance(*box, npd);
+        float const w = nq->cell_width;
+        nq->lc_dim[0] = std::max(1, int((unsigned) (npd[0] / w)));
+        nq->lc_dim[1] = std::max(1, int((unsigned) (npd[1] / w)));
+        nq->lc_dim[2] = box->is2D ? 1 : std::max(1, int((unsigned) (npd[2] / w)));
+        if ((w * 2.0 > npd[0]) || (w * 2.0 > npd[1]) || (!box->is2D && w * 2.0 > npd[2]))
+        {
+            delete nq;
+            return fa::fail(FA_ERR_RUNTIME, "Cannot generate a cell list where cell_width is larger than half the box.");
+        }
+    }
+    *out = nq;
+    return FA_OK;
+}
+
+int fa_nq_destroy(fa_nq* nq)
+{
+    if (nq)
+    {
+        (void) hipStreamSynchronize(fa::stream());
+        delete nq;
+    }
+    return FA_OK;
+}
+
+int fa_nq_cell_width(const fa_nq* nq, float* width)
+{
+    *width = nq->cell_width;
+    return FA_OK;
+}
+
+int fa_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, const fa_query_args* args,
+             int sort_by_distance, fa_nlist* out)
+{
+    if (int rc = fa_init())
+        return rc;
+    fa_query_args a = *args;
+    if (int rc = fa::validate_query_args(nq, a, n_query))
+        return rc;
+    out->num_points = nq->n;
+    int rc = (a.mode == FA_MODE_BALL) ? run_ball(nq, d_query_points, n_query, a, sort_by_distance, out)
+                                      : run_knn(nq, d_query_points, n_query, a, sort_by_distance, out);
+    if (rc)
+        return rc;
+    FA_HIP(hipStreamSynchronize(fa::stream()));
+    return FA_OK;
+}
+
+int fa_nlist_free(fa_nlist* nl)
+{
+    if (!nl)
+        return FA_OK;
+    (void) hipStreamSynchronize(fa::stream());
+    fa::device_free(nl->neighbors);
+    fa::device_free(nl->distances);
+    fa::device_free(nl->weights);
+    fa::device_free(nl->vectors);
+    fa::device_free(nl->counts);
+    fa::device_free(nl->segments);
+    fa::device_free(nl->offsets);
+    *nl = fa_nlist{};
+    return FA_OK;
+}
+
+int fa_nlist_all_pairs(const fa_box* box, const float* d_points, uint32_t n_points, const float* d_query_points,
+                       uint32_t n_query, int exclude_ii, fa_nlist* out)
+{
+    if (int rc = fa_init())
+        return rc;
+    uint32_t const nii = exclude_ii ? std::min(n_points, n_query) : 0;
+    uint64_t const nb = uint64_t(n_points) * n_query - nii;
+    if (nb > 0xffffffffull)
+        return fa::fail(FA_ERR_RUNTIME, "NeighborList would exceed 2^32 bonds");
+    if (int rc = fa::nlist_reserve(out, nb, n_query))
+        return rc;
+    out->num_bonds = nb;
+    out->num_points = n_points;
+    fa::DevBox const db = fa::make_devbox(*box);
+    uint64_t const total = uint64_t(n_points) * n_query;
+    if (total)
+    {
+        FA_KLAUNCH(all_pairs_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, fa::stream(), db,
+                           d_points, n_points, d_query_points, n_query, exclude_ii, out->neighbors, out->distances,
+                           out->weights, out->vectors);
+        FA_HIP(hipGetLastError());
+    }
+    // offsets: row i starts at i*np - min(i, np)
+    std::vector<uint32_t> off(size_t(n_query) + 1);
+    for (uint32_t i = 0; i <= n_query; ++i)
+        off[i] = uint32_t(uint64_t(i) * n_points - (exclude_ii ? std::min(i, n_points) : 0));
+    FA_HIP(hipMemcpyAsync(out->offsets, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice,
+                          fa::stream()));
+    if (int rc = fa::nlist_finish_counts(out))
+        return rc;
+    FA_HIP(hipStreamSynchronize(fa::stream()));
+    return FA_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,217 @@
+/* freud_amd.h -- C-ABI of the MI355X implementation of freud's pair-analysis path.
+ *
+ * Plain pointers and sizes only.  Arrays named d_* live in device memory (HBM);
+ * everything else is host memory.  Every function returns FA_OK (0) or a
+ * negative FA_ERR_* status; fa_last_error() returns the message of the last
+ * failure on the calling thread.  The status codes map onto the exception types
+ * the reference raises through nanobind: FA_ERR_INVALID -> ValueError
+ * (std::invalid_argument / std::domain_error), FA_ERR_RUNTIME -> RuntimeError
+ * (std::runtime_error), FA_ERR_RANGE -> IndexError (std::out_of_range).
+ *
+ * Citations name files of glotzerlab/freud 3.5.0: each entry point replaces the
+ * C++ seam cited beside it (which the reference binds with nanobind in the
+ * export-*.cc file named there).
+ */
+#ifndef FREUD_AMD_H
+#define FREUD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FA_OK 0
+#define FA_ERR_INVALID (-1) /* ValueError   */
+#define FA_ERR_RUNTIME (-2) /* RuntimeError */
+#define FA_ERR_RANGE (-3)   /* IndexError   */
+#define FA_ERR_DEVICE (-4)  /* RuntimeError: a HIP call failed */
+
+#define FA_NQ_AABB 0     /* freud.locality.AABBQuery: r = p_j - (q_i + image) */
+#define FA_NQ_LINKCELL 1 /* freud.locality.LinkCell:  r = box.wrap(p_j - q_i)  */
+
+#define FA_MODE_NONE 0
+#define FA_MODE_BALL 1
+#define FA_MODE_NEAREST 2
+
+/* Steinhardt flags (freud/order/Steinhardt.h:59-90 constructor arguments) */
+#define FA_ST_AVERAGE 1
+#define FA_ST_WL 2
+#define FA_ST_WEIGHTED 4
+#define FA_ST_WL_NORMALIZE 8
+
+/* freud::box::Box (freud/box/Box.h:44-606): lengths, tilt factors, 2D flag.
+ * Every dimension is periodic (the reference's pair queries reject other boxes,
+ * freud/locality/NeighborQuery.h:134-138). */
+typedef struct fa_box
+{
+    float Lx, Ly, Lz, xy, xz, yz;
+    int32_t is2D;
+} fa_box;
+
+/* freud::locality::QueryArgs (freud/locality/NeighborQuery.h:37-73) with the
+ * reference's sentinels: num_neighbors 0xffffffff, r_max -1, r_guess -1,
+ * scale -1 mean "unset". */
+typedef struct fa_query_args
+{
+    int32_t mode;
+    uint32_t num_neighbors;
+    float r_max;
+    float r_min;
+    float r_guess;
+    float scale;
+    int32_t exclude_ii;
+} fa_query_args;
+
+/* freud::locality::NeighborList storage (freud/locality/NeighborList.h:29-158),
+ * device resident, in the reference's layout:
+ *   neighbors  uint32 (num_bonds, 2) row-major  [i0, j0, i1, j1, ...]
+ *   distances  float  (num_bonds)
+ *   weights    float  (num_bonds)
+ *   vectors    float  (num_bonds, 3)
+ *   counts     uint32 (num_query_points)
+ *   segments   uint32 (num_query_points) -- first bond of each query point, 0
+ *              for query points without bonds (NeighborList.cc:199-232)
+ *   offsets    uint32 (num_query_points + 1) -- exclusive scan of counts
+ * Buffers are owned by the library and released with fa_nlist_free. */
+typedef struct fa_nlist
+{
+    uint32_t* neighbors;
+    float* distances;
+    float* weights;
+    float* vectors;
+    uint32_t* counts;
+    uint32_t* segments;
+    uint32_t* offsets;
+    uint64_t num_bonds;
+    uint32_t num_query_points;
+    uint32_t num_points;
+    uint64_t capacity;
+} fa_nlist;
+
+typedef struct fa_nq fa_nq;
+
+/* ---------------------------------------------------------------- runtime */
+const char* fa_last_error(void);
+int fa_init(void);
+int fa_device_name(char* buf, int len);
+int fa_malloc(void** d_ptr, size_t bytes);
+int fa_free(void* d_ptr);
+int fa_memcpy_h2d(void* d_dst, const void* src, size_t bytes);
+int fa_memcpy_d2h(void* dst, const void* d_src, size_t bytes);
+int fa_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes);
+int fa_memset(void* d_dst, int value, size_t bytes);
+int fa_synchronize(void);
+/* Event timing on the library's stream (milliseconds between two records). */
+int fa_timer_start(void);
+int fa_timer_stop(float* ms);
+/* Per-kernel timing: when enabled, every kernel launch of the library is
+ * bracketed by HIP events on the library's stream; fa_profile_get returns the
+ * launch count and summed milliseconds per kernel name.  (No counterpart in
+ * the reference; used by the benchmark to time the dominant kernel.) */
+int fa_profile_enable(int on);
+int fa_profile_reset(void);
+int fa_profile_count(void);
+int fa_profile_get(int idx, char* name, int len, uint64_t* launches, double* total_ms);
+
+/* ---------------------------------------------------------------- box
+ * freud/box/Box.h:212-329: op 0 makeFractional, 1 makeAbsolute, 2 wrap over n
+ * device vec3s (in place allowed). */
+int fa_box_transform(const fa_box* box, int op, const float* d_in, float* d_out, uint64_t n);
+
+/* ---------------------------------------------------------------- locality
+ * NeighborQuery constructors: AABBQuery (freud/locality/AABBQuery.cc:18-26) and
+ * LinkCell (freud/locality/LinkCell.cc:222-260; cell_width 0 = the reference's
+ * default width).  d_points (n x 3) is caller-owned and must outlive the handle
+ * (the reference also keeps a raw pointer, NeighborQuery.h:93-95). */
+int fa_nq_create(const fa_box* box, const float* d_points, uint32_t n, int kind, float cell_width, fa_nq** out);
+int fa_nq_destroy(fa_nq* nq);
+/* LinkCell::getCellWidth (LinkCell.h) */
+int fa_nq_cell_width(const fa_nq* nq, float* width);
+
+/* NeighborQuery::query(...)->toNeighborList(sort_by_distance)
+ * (freud/locality/NeighborQuery.h:131-142, 434-481; binding
+ * export-NeighborQuery.cc:26-41).  On entry *out may hold buffers of an
+ * earlier list (capacity reused) or be zeroed. */
+int fa_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, const fa_query_args* args,
+             int sort_by_distance, fa_nlist* out);
+
+int fa_nlist_free(fa_nlist* nl);
+/* NeighborList(num_bonds, query_point_index, ...) from device arrays
+ * (freud/locality/NeighborList.cc:51-82): validates sorting and index ranges,
+ * distances = sqrt(dot(v, v)). */
+int fa_nlist_from_arrays(uint32_t num_query_points, uint32_t num_points, const uint32_t* d_query_point_indices,
+                         const uint32_t* d_point_indices, const float* d_vectors, const float* d_weights,
+                         uint64_t num_bonds, fa_nlist* out);
+/* NeighborList::NeighborList(points, query_points, box, exclude_ii, ...)
+ * all_pairs (freud/locality/NeighborList.cc:84-128). */
+int fa_nlist_all_pairs(const fa_box* box, const float* d_points, uint32_t n_points, const float* d_query_points,
+                       uint32_t n_query, int exclude_ii, fa_nlist* out);
+/* NeighborList::filter (NeighborList.cc:239-277): keep bonds whose d_keep byte is nonzero. */
+int fa_nlist_filter(fa_nlist* nl, const uint8_t* d_keep);
+/* NeighborList::filter_r (NeighborList.cc:286-307): keep r_min <= d < r_max. */
+int fa_nlist_filter_r(fa_nlist* nl, float r_max, float r_min);
+/* NeighborList::sort (NeighborList.cc:371-399). */
+int fa_nlist_sort(fa_nlist* nl, int by_distance);
+/* NeighborList::copy (NeighborList.cc:360-370) */
+int fa_nlist_copy(const fa_nlist* src, fa_nlist* dst);
+
+/* ---------------------------------------------------------------- density
+ * RDF::accumulate without a NeighborList (freud/density/RDF.cc:101-110 via
+ * BondHistogramCompute::accumulateGeneral, BondHistogramCompute.h:115-126): a
+ * fused ball query (query radius q_r_max/q_r_min) + RegularAxis binning
+ * (freud/util/Histogram.h:152-174) into d_counts (bins uint32, accumulated). */
+int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, float q_r_max, float q_r_min,
+                            int exclude_ii, uint32_t bins, float r_min, float r_max, uint32_t* d_counts);
+/* Same, binning the distances of an existing NeighborList. */
+int fa_rdf_accumulate_nlist(const fa_nlist* nl, uint32_t bins, float r_min, float r_max, uint32_t* d_counts);
+/* RDF::reduce (freud/density/RDF.cc:73-99) and the shell volumes of the RDF
+ * constructor (RDF.cc:44-63), host arrays.  Arithmetic in the reference's float
+ * order. */
+int fa_rdf_reduce(const fa_box* box, const uint32_t* counts, uint32_t bins, float r_min, float r_max,
+                  uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, int finite_size, float* rdf,
+                  float* n_r);
+
+/* ---------------------------------------------------------------- order
+ * Steinhardt::compute over a NeighborList (freud/order/Steinhardt.cc:85-359;
+ * binding export-Steinhardt.cc:19-34).  ls[n_l] are the l values.  Outputs
+ * (device, caller-allocated):
+ *   d_qlm     complex64 interleaved, l-major blocks: block k is (n, 2l_k+1) with
+ *             m = 0..l, -1..-l
+ *   d_ql      float (n, n_l)
+ *   d_qlave   float (n, n_l)            (FA_ST_AVERAGE; may be NULL otherwise)
+ *   d_qlmave  like d_qlm                (FA_ST_AVERAGE; may be NULL)
+ *   d_wl      float (n, n_l)            (FA_ST_WL; may be NULL otherwise)
+ *   order     host float (n_l)          system-wide order (normalizeSystem) */
+int fa_steinhardt(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, const uint32_t* ls,
+                  int n_l, int flags, float* d_qlm, float* d_ql, float* d_qlave, float* d_qlmave, float* d_wl,
+                  float* order);
+
+/* SolidLiquid::compute (freud/order/SolidLiquid.cc:31-114; binding
+ * export-SolidLiquid.cc:11-28) over a NeighborList: q_lm of Steinhardt(l),
+ * per-bond ql_ij, num_connections and solid clusters.  d_qlm (n, 2l+1)
+ * complex64 and d_ql (n) are written too. */
+int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, uint32_t l,
+                   float q_threshold, uint32_t solid_threshold, int normalize_q, float* d_qlm, float* d_ql,
+                   float* d_ql_ij, uint32_t* d_num_connections, uint32_t* d_cluster_idx, uint32_t* d_cluster_sizes,
+                   uint32_t* num_clusters);
+
+/* Cluster::compute over the bonds of a NeighborList (freud/cluster/Cluster.cc:21-128):
+ * labels sorted by cluster size (descending) then smallest member index.
+ * d_keep (optional, per bond) selects the bonds used.  d_cluster_sizes
+ * (optional, n entries) receives the sizes in label order
+ * (SolidLiquid.h:95-109 getClusterSizes). */
+int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* d_cluster_idx,
+               uint32_t* d_cluster_sizes, uint32_t* num_clusters);
+
+/* Hexatic::compute (freud/order/HexaticTranslational.cc:22-74): psi_k per
+ * particle, complex64 interleaved. */
+int fa_hexatic(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, uint32_t k, int weighted,
+               float* d_psi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FREUD_AMD_H */

@@ -1,0 +1,344 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Each function mirrors one freud entry point and returns numpy arrays in the
+reference's layouts.  See ``freud_oracle.cpp`` for the citations of the
+reference arithmetic each function restates.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+__all__ = [
+    "build",
+    "box_op",
+    "nearest_plane_distance",
+    "query",
+    "all_pairs",
+    "rdf_accumulate",
+    "rdf_bin",
+    "rdf_reduce",
+    "regular_axis",
+    "ylm",
+    "wigner3j_table",
+    "steinhardt",
+    "solidliquid",
+    "cluster",
+    "hexatic",
+    "OracleError",
+]
+
+
+class OracleError(Exception):
+    pass
+
+
+class _QueryParams(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int),
+        ("k", ctypes.c_uint32),
+        ("r_max", ctypes.c_float),
+        ("r_min", ctypes.c_float),
+        ("r_guess", ctypes.c_float),
+        ("scale", ctypes.c_float),
+        ("exclude_ii", ctypes.c_int),
+    ]
+
+
+class _Nlist(ctypes.Structure):
+    _fields_ = [
+        ("ij", ctypes.POINTER(ctypes.c_uint32)),
+        ("dist", ctypes.POINTER(ctypes.c_float)),
+        ("w", ctypes.POINTER(ctypes.c_float)),
+        ("vec", ctypes.POINTER(ctypes.c_float)),
+        ("n", ctypes.c_uint64),
+    ]
+
+
+def build():
+    """Compile the oracle library with its Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.orc_last_error.restype = ctypes.c_char_p
+        _lib.orc_wigner3j_table.restype = ctypes.c_uint32
+        _lib.orc_cluster.restype = ctypes.c_uint32
+        _lib.orc_solidliquid.restype = ctypes.c_uint32
+    return _lib
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _boxp(box):
+    """box: (Lx, Ly, Lz, xy, xz, yz, is2D) tuple or an object with those fields."""
+    if hasattr(box, "Lx"):
+        vals = [box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz]
+        is2d = bool(box.is2D)
+    else:
+        vals = list(box[:6])
+        is2d = bool(box[6]) if len(box) > 6 else False
+    return np.array(vals, dtype=np.float32), int(is2d)
+
+
+def _check(rc):
+    if rc == 0:
+        return
+    msg = _load().orc_last_error().decode()
+    if rc == -1:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def box_op(box, vecs, op):
+    """op: 'fractional' | 'absolute' | 'wrap' (Box.h:212-329)."""
+    lib = _load()
+    b, is2d = _boxp(box)
+    v = _f32(vecs).reshape(-1, 3)
+    out = np.empty_like(v)
+    code = {"fractional": 0, "absolute": 1, "wrap": 2}[op]
+    lib.orc_box_op(_p(b), is2d, code, _p(v), ctypes.c_uint64(len(v)), _p(out))
+    return out
+
+
+def nearest_plane_distance(box):
+    lib = _load()
+    b, is2d = _boxp(box)
+    out = np.empty(3, np.float32)
+    lib.orc_nearest_plane_distance(_p(b), is2d, _p(out))
+    return out
+
+
+def _qparams(mode=None, num_neighbors=None, r_max=None, r_min=None, r_guess=None, scale=None, exclude_ii=False):
+    qp = _QueryParams()
+    qp.mode = {None: 0, "ball": 1, "nearest": 2}[mode]
+    qp.k = 0xFFFFFFFF if num_neighbors is None else int(num_neighbors)
+    qp.r_max = -1.0 if r_max is None else float(r_max)
+    qp.r_min = 0.0 if r_min is None else float(r_min)
+    qp.r_guess = -1.0 if r_guess is None else float(r_guess)
+    qp.scale = -1.0 if scale is None else float(scale)
+    qp.exclude_ii = int(bool(exclude_ii))
+    return qp
+
+
+def _take_nlist(nl):
+    lib = _load()
+    n = int(nl.n)
+    if n:
+        ij = np.ctypeslib.as_array(nl.ij, shape=(n, 2)).copy()
+        d = np.ctypeslib.as_array(nl.dist, shape=(n,)).copy()
+        w = np.ctypeslib.as_array(nl.w, shape=(n,)).copy()
+        v = np.ctypeslib.as_array(nl.vec, shape=(n, 3)).copy()
+    else:
+        ij = np.zeros((0, 2), np.uint32)
+        d = np.zeros(0, np.float32)
+        w = np.zeros(0, np.float32)
+        v = np.zeros((0, 3), np.float32)
+    lib.orc_nlist_free(ctypes.byref(nl))
+    return {"ij": ij, "distances": d, "weights": w, "vectors": v}
+
+
+def query(box, points, query_points, qargs, nq_type="aabb", cell_width=0.0, sort_by_distance=False, nthreads=1):
+    """Ball / kNN query -> dict(ij, distances, weights, vectors) in NeighborList order.
+
+    nq_type: 'aabb' (AABBQuery formula r = p_j - (q_i + image)) or 'linkcell'
+    (LinkCell formula r = box.wrap(p_j - q_i)).
+    """
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    qpts = _f32(query_points).reshape(-1, 3)
+    qp = _qparams(**qargs)
+    nl = _Nlist()
+    rc = lib.orc_query(
+        _p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(qpts), ctypes.c_uint32(len(qpts)),
+        {"aabb": 0, "linkcell": 1}[nq_type], ctypes.c_float(cell_width), qp, int(bool(sort_by_distance)),
+        int(nthreads), ctypes.byref(nl))
+    if rc:
+        _check(rc)
+    return _take_nlist(nl)
+
+
+def all_pairs(box, points, query_points, exclude_ii=True):
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    qpts = _f32(query_points).reshape(-1, 3)
+    nl = _Nlist()
+    lib.orc_all_pairs(_p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(qpts), ctypes.c_uint32(len(qpts)),
+                      int(bool(exclude_ii)), ctypes.byref(nl))
+    return _take_nlist(nl)
+
+
+def regular_axis(bins, r_min, r_max):
+    lib = _load()
+    edges = np.empty(bins + 1, np.float32)
+    centers = np.empty(bins, np.float32)
+    w = ctypes.c_float()
+    inv = ctypes.c_float()
+    lib.orc_regular_axis(ctypes.c_uint32(bins), ctypes.c_float(r_min), ctypes.c_float(r_max), _p(edges),
+                         _p(centers), ctypes.byref(w), ctypes.byref(inv))
+    return edges, centers
+
+
+def rdf_bin(distances, bins, r_min, r_max, counts=None):
+    lib = _load()
+    d = _f32(distances)
+    if counts is None:
+        counts = np.zeros(bins, np.uint32)
+    lib.orc_rdf_bin(_p(d), ctypes.c_uint64(len(d)), ctypes.c_uint32(bins), ctypes.c_float(r_min),
+                    ctypes.c_float(r_max), _p(counts))
+    return counts
+
+
+def rdf_accumulate(box, points, query_points, bins, r_max, r_min=0.0, exclude_ii=True, nq_type="aabb",
+                   cell_width=0.0, query_r_max=None, query_r_min=0.0, counts=None, nthreads=1):
+    """Fused ball query + RegularAxis binning (RDF.cc:101-110).  Adds to `counts`."""
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    qpts = _f32(query_points).reshape(-1, 3)
+    if counts is None:
+        counts = np.zeros(bins, np.uint32)
+    qrm = r_max if query_r_max is None else query_r_max
+    rc = lib.orc_rdf_accumulate(
+        _p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(qpts), ctypes.c_uint32(len(qpts)),
+        {"aabb": 0, "linkcell": 1}[nq_type], ctypes.c_float(cell_width), ctypes.c_float(qrm),
+        ctypes.c_float(query_r_min), int(bool(exclude_ii)), ctypes.c_uint32(bins), ctypes.c_float(r_min),
+        ctypes.c_float(r_max), int(nthreads), _p(counts))
+    _check(rc)
+    return counts
+
+
+def rdf_reduce(box, counts, bins, r_min, r_max, n_points, n_query_points, n_frames, finite_size=False):
+    lib = _load()
+    b, is2d = _boxp(box)
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    rdf = np.empty(bins, np.float32)
+    n_r = np.empty(bins, np.float32)
+    lib.orc_rdf_reduce(_p(b), is2d, _p(c), ctypes.c_uint32(bins), ctypes.c_float(r_min), ctypes.c_float(r_max),
+                       ctypes.c_uint32(n_points), ctypes.c_uint32(n_query_points), ctypes.c_uint32(n_frames),
+                       int(bool(finite_size)), _p(rdf), _p(n_r))
+    return rdf, n_r
+
+
+def ylm(l, theta, phi, f64=True):
+    """Y_l^m for each (theta, phi), m order [0..l, -1..-l]; complex128 (n, 2l+1)."""
+    lib = _load()
+    t = np.ascontiguousarray(theta, dtype=np.float64).ravel()
+    p = np.ascontiguousarray(phi, dtype=np.float64).ravel()
+    re = np.empty((len(t), 2 * l + 1))
+    im = np.empty((len(t), 2 * l + 1))
+    lib.orc_ylm(int(l), _p(t), _p(p), ctypes.c_uint64(len(t)), int(bool(f64)), _p(re), _p(im))
+    return re + 1j * im
+
+
+def wigner3j_table(l):
+    lib = _load()
+    n = lib.orc_wigner3j_table(int(l), None)
+    out = np.empty(n, np.float64)
+    lib.orc_wigner3j_table(int(l), _p(out))
+    return out
+
+
+def steinhardt(box, points, nlist, ls, average=False, wl=False, weighted=False, wl_normalize=False, f64=False,
+               nthreads=1):
+    """Steinhardt.cc:85-359 over an i-sorted NeighborList.
+
+    Returns dict with per-l lists 'qlm' ((N, 2l+1) complex), 'qlm_ave', and
+    arrays 'ql' (N, n_l), 'ql_ave', 'wl', 'order' (n_l,), 'particle_order'.
+    """
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    n = len(pts)
+    ij = np.ascontiguousarray(nlist["ij"], dtype=np.uint32)
+    d = _f32(nlist["distances"])
+    w = _f32(nlist["weights"])
+    ls = np.atleast_1d(np.asarray(ls, dtype=np.uint32))
+    nl = len(ls)
+    dt = np.float64 if f64 else np.float32
+    total_m = int(np.sum(2 * ls + 1))
+    qlm = np.zeros(2 * n * total_m, dt)
+    qlm_ave = np.zeros(2 * n * total_m, dt)
+    ql = np.zeros((n, nl), dt)
+    ql_ave = np.zeros((n, nl), dt)
+    wlv = np.zeros((n, nl), dt)
+    order = np.zeros(nl, dt)
+    lib.orc_steinhardt(_p(b), is2d, _p(pts), ctypes.c_uint32(n), _p(ij), _p(d), _p(w), ctypes.c_uint64(len(d)),
+                       _p(ls), nl, int(average), int(wl), int(weighted), int(wl_normalize), int(f64), int(nthreads),
+                       _p(qlm), _p(ql), _p(ql_ave), _p(wlv), _p(order), _p(qlm_ave))
+    out_qlm, out_ave = [], []
+    off = 0
+    for l in ls:
+        nm = 2 * int(l) + 1
+        blk = qlm[2 * n * off: 2 * n * (off + nm)].reshape(n, nm, 2)
+        out_qlm.append(blk[..., 0] + 1j * blk[..., 1])
+        blk = qlm_ave[2 * n * off: 2 * n * (off + nm)].reshape(n, nm, 2)
+        out_ave.append(blk[..., 0] + 1j * blk[..., 1])
+        off += nm
+    if wl:
+        po = wlv
+    elif average:
+        po = ql_ave
+    else:
+        po = ql
+    return {"qlm": out_qlm, "qlm_ave": out_ave, "ql": ql, "ql_ave": ql_ave, "wl": wlv, "order": order,
+            "particle_order": po}
+
+
+def solidliquid(nlist, qlm, ql, l, q_threshold, solid_threshold, normalize_q=True):
+    """SolidLiquid.cc:31-114 given the Steinhardt(l) q_lm / q_l of the same nlist."""
+    lib = _load()
+    ij = np.ascontiguousarray(nlist["ij"], dtype=np.uint32)
+    nb = len(ij)
+    q = np.ascontiguousarray(qlm, dtype=np.complex64)
+    n = q.shape[0]
+    qlv = _f32(ql).ravel()
+    ql_ij = np.empty(nb, np.float32)
+    nc = np.empty(n, np.uint32)
+    cidx = np.empty(n, np.uint32)
+    ncl = lib.orc_solidliquid(ctypes.c_uint32(n), _p(ij), ctypes.c_uint64(nb), int(l), _p(q), _p(qlv),
+                              ctypes.c_float(q_threshold), ctypes.c_uint32(int(solid_threshold)),
+                              int(bool(normalize_q)), _p(ql_ij), _p(nc), _p(cidx))
+    return {"ql_ij": ql_ij, "num_connections": nc, "cluster_idx": cidx, "num_clusters": int(ncl)}
+
+
+def cluster(n_points, ij):
+    lib = _load()
+    ij = np.ascontiguousarray(ij, dtype=np.uint32).reshape(-1, 2)
+    cidx = np.empty(n_points, np.uint32)
+    ncl = lib.orc_cluster(ctypes.c_uint32(n_points), _p(ij), ctypes.c_uint64(len(ij)), _p(cidx))
+    return cidx, int(ncl)
+
+
+def hexatic(box, points, nlist, k=6, weighted=False):
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    ij = np.ascontiguousarray(nlist["ij"], dtype=np.uint32)
+    w = _f32(nlist["weights"])
+    psi = np.empty(2 * len(pts), np.float32)
+    lib.orc_hexatic(_p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(ij), _p(w), ctypes.c_uint64(len(ij)),
+                    ctypes.c_uint32(k), int(bool(weighted)), _p(psi))
+    return psi.reshape(-1, 2)[:, 0] + 1j * psi.reshape(-1, 2)[:, 1]

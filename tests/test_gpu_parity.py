@@ -1,0 +1,364 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
+same seeded inputs.  Integer and index outputs and the bond vectors/distances
+must be bit-identical; floating-point order parameters match within the
+tolerances stated in each test."""
+
+import os
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+import freud_amd
+from conftest import GOLDEN
+from freud_amd.box import Box
+from freud_amd.gsd import open_gsd
+
+pytestmark = pytest.mark.gpu
+
+
+def _bt(box):
+    return (box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D)
+
+
+def _systems():
+    rng = np.random.default_rng(11)
+    out = []
+    b = Box.cube(16.0)
+    out.append(("cubic_uniform", b, ((rng.random((4096, 3)) - 0.5) * 16).astype(np.float32)))
+    b = Box(12.0, 13.0, 14.0, 0.3, -0.2, 0.25)
+    out.append(("triclinic", b, b.make_absolute(rng.random((3000, 3)).astype(np.float32))))
+    b = Box.square(20.0)
+    p = b.make_absolute(rng.random((2000, 3)).astype(np.float32))
+    p[:, 2] = 0
+    out.append(("2d", b, p))
+    bx, pf = freud_amd.data.fcc_jittered(6, 1.5874, 0.05 * 1.5874, seed=1)
+    out.append(("fcc", bx, pf))
+    return out
+
+
+SYSTEMS = _systems()
+
+
+def _assert_nlist_equal(nl, ref):
+    npt.assert_array_equal(nl[:], ref["ij"])
+    npt.assert_array_equal(nl.distances, ref["distances"])
+    npt.assert_array_equal(nl.vectors, ref["vectors"])
+    npt.assert_array_equal(nl.weights, 1.0)
+
+
+@pytest.mark.parametrize("name,box,pts", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+@pytest.mark.parametrize("kind", ["aabb", "linkcell"])
+def test_ball_query_bit_exact(oracle, name, box, pts, kind, r_max=1.9):
+    if kind == "aabb":
+        nq = freud_amd.AABBQuery(box, pts)
+    else:
+        nq = freud_amd.LinkCell(box, pts, cell_width=r_max)
+    for qargs in (dict(r_max=r_max, exclude_ii=True), dict(r_max=r_max, r_min=0.8), dict(r_max=1.1)):
+        nl = nq.query(pts, qargs).toNeighborList()
+        ref = oracle.query(_bt(box), pts, pts, qargs, nq_type=kind, cell_width=r_max, nthreads=8)
+        _assert_nlist_equal(nl, ref)
+        counts = np.bincount(ref["ij"][:, 0], minlength=len(pts)) if len(ref["ij"]) else np.zeros(len(pts))
+        npt.assert_array_equal(nl.neighbor_counts, counts)
+        seg = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        npt.assert_array_equal(nl.segments, np.where(counts > 0, seg, 0))
+
+
+def test_ball_query_distinct_query_points(oracle):
+    rng = np.random.default_rng(2)
+    box = Box(10.0, 11.0, 12.0, 0.1, 0.2, -0.1)
+    pts = box.make_absolute(rng.random((1500, 3)).astype(np.float32))
+    qp = box.make_absolute(rng.random((700, 3)).astype(np.float32))
+    for cls, kind in ((freud_amd.AABBQuery, "aabb"), (freud_amd.LinkCell, "linkcell")):
+        nq = cls(box, pts) if kind == "aabb" else cls(box, pts, 2.0)
+        nl = nq.query(qp, dict(r_max=2.0)).toNeighborList()
+        ref = oracle.query(_bt(box), pts, qp, dict(r_max=2.0), nq_type=kind, cell_width=2.0, nthreads=8)
+        _assert_nlist_equal(nl, ref)
+
+
+def test_ball_query_sort_by_distance(oracle):
+    name, box, pts = SYSTEMS[0]
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=2.5, exclude_ii=True)).toNeighborList(True)
+    ref = oracle.query(_bt(box), pts, pts, dict(r_max=2.5, exclude_ii=True), sort_by_distance=True, nthreads=8)
+    _assert_nlist_equal(nl, ref)
+
+
+def test_points_outside_box(oracle):
+    """Points exactly on +L/2 and slightly outside the box keep the reference's
+    image arithmetic (cells are assigned through the wrapped coordinate)."""
+    rng = np.random.default_rng(4)
+    box = Box.cube(8.0)
+    pts = ((rng.random((800, 3)) - 0.5) * 8).astype(np.float32)
+    pts[:50, 0] = 4.0
+    pts[50:80] *= 1.01
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
+    ref = oracle.query(_bt(box), pts, pts, dict(r_max=1.5, exclude_ii=True), nthreads=8)
+    _assert_nlist_equal(nl, ref)
+
+
+@pytest.mark.parametrize("name,box,pts", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+@pytest.mark.parametrize("k", [1, 6, 12, 20])
+def test_knn(oracle, name, box, pts, k):
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(num_neighbors=k, exclude_ii=True)).toNeighborList()
+    ref = oracle.query(_bt(box), pts, pts, dict(num_neighbors=k, exclude_ii=True), nthreads=8)
+    # same neighbour sets (ties at the k-th distance are broken by index in both)
+    _assert_nlist_equal(nl, ref)
+
+
+def test_knn_linkcell(oracle):
+    name, box, pts = SYSTEMS[1]
+    nq = freud_amd.LinkCell(box, pts, cell_width=2.5)
+    for k in (4, 12):
+        nl = nq.query(pts, dict(num_neighbors=k, exclude_ii=True)).toNeighborList()
+        ref = oracle.query(_bt(box), pts, pts, dict(num_neighbors=k, exclude_ii=True), nq_type="linkcell",
+                           cell_width=2.5, nthreads=8)
+        _assert_nlist_equal(nl, ref)
+
+
+def test_knn_sparse_extended(oracle):
+    # tests/test_locality_neighbor_query.py:253-300 of the reference (search past L/2)
+    box = Box.cube(10)
+    pts = np.array([[0, 0, 0], [1, 0, 0], [3, 0, 0], [2, 0, 0]], np.float32)
+    for k in (3, 5):
+        nl = freud_amd.AABBQuery(box, pts).query(pts, dict(num_neighbors=k, exclude_ii=True)).toNeighborList()
+        ref = oracle.query(_bt(box), pts, pts, dict(num_neighbors=k, exclude_ii=True))
+        _assert_nlist_equal(nl, ref)
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(num_neighbors=3, r_max=1.9, exclude_ii=True)).toNeighborList()
+    assert [set(nl.point_indices[nl.query_point_indices == i]) for i in range(4)] == [{1}, {0, 3}, {3}, {1, 2}]
+
+
+@pytest.mark.parametrize("name,box,pts", SYSTEMS, ids=[s[0] for s in SYSTEMS])
+def test_rdf_counts_bit_exact(oracle, name, box, pts):
+    r_max = 3.0 if name != "fcc" else 2.9
+    rdf = freud_amd.density.RDF(bins=100, r_max=r_max).compute((box, pts))
+    counts = oracle.rdf_accumulate(_bt(box), pts, pts, 100, r_max, exclude_ii=True, nthreads=8)
+    npt.assert_array_equal(rdf.bin_counts, counts)
+    ref_rdf, ref_nr = oracle.rdf_reduce(_bt(box), counts, 100, 0.0, r_max, len(pts), len(pts), 1)
+    npt.assert_allclose(rdf.rdf, ref_rdf, rtol=1e-6)
+    npt.assert_allclose(rdf.n_r, ref_nr, rtol=1e-6)
+    # LinkCell formula through a LinkCell system
+    lc = freud_amd.LinkCell(box, pts, cell_width=r_max)
+    rdf2 = freud_amd.density.RDF(bins=100, r_max=r_max, r_min=0.5).compute(lc)
+    counts2 = oracle.rdf_accumulate(_bt(box), pts, pts, 100, r_max, r_min=0.5, exclude_ii=True, nq_type="linkcell",
+                                    cell_width=r_max, nthreads=8)
+    npt.assert_array_equal(rdf2.bin_counts, counts2)
+
+
+def test_rdf_lj_golden():
+    # tests/integration/test_reader_integrations.py:27-40 of the reference
+    traj = open_gsd(os.path.join(GOLDEN, "lj.gsd"))
+    rdf = freud_amd.density.RDF(bins=100, r_max=5)
+    for fr in traj:
+        rdf.compute(fr, reset=False)
+    gold = np.load(os.path.join(GOLDEN, "rdf.npz"))["rdf"]
+    npt.assert_allclose(rdf.rdf, gold, rtol=1e-5, atol=1e-5)
+
+
+def test_rdf_nlist_and_query_points(oracle):
+    rng = np.random.default_rng(7)
+    box = Box.cube(12.0)
+    pts = ((rng.random((2000, 3)) - 0.5) * 12).astype(np.float32)
+    qp = ((rng.random((500, 3)) - 0.5) * 12).astype(np.float32)
+    aq = freud_amd.AABBQuery(box, pts)
+    rdf = freud_amd.density.RDF(50, 3.0).compute(aq, query_points=qp)
+    counts = oracle.rdf_accumulate(_bt(box), pts, qp, 50, 3.0, exclude_ii=False, nthreads=8)
+    npt.assert_array_equal(rdf.bin_counts, counts)
+    nl = aq.query(qp, dict(r_max=3.0)).toNeighborList()
+    rdf2 = freud_amd.density.RDF(50, 3.0).compute(aq, query_points=qp, neighbors=nl)
+    npt.assert_array_equal(rdf2.bin_counts, counts)
+    # kNN neighbours are binned from the NeighborList
+    rdf3 = freud_amd.density.RDF(50, 3.0).compute(aq, query_points=qp, neighbors=dict(num_neighbors=8))
+    ref = oracle.query(_bt(box), pts, qp, dict(num_neighbors=8), nthreads=8)
+    npt.assert_array_equal(rdf3.bin_counts, oracle.rdf_bin(ref["distances"], 50, 0.0, 3.0))
+
+
+# --------------------------------------------------------------------- Steinhardt
+@pytest.mark.parametrize("name,box,pts", SYSTEMS[:2] + SYSTEMS[3:], ids=[s[0] for s in SYSTEMS[:2] + SYSTEMS[3:]])
+def test_steinhardt(oracle, name, box, pts):
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.6, exclude_ii=True)).toNeighborList()
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.6, exclude_ii=True), nthreads=8)
+    ls = [4, 6, 8, 10, 12]
+    for kw in (dict(), dict(average=True), dict(wl=True), dict(wl=True, wl_normalize=True),
+               dict(average=True, wl=True, wl_normalize=True)):
+        st = freud_amd.order.Steinhardt(ls, **kw).compute((box, pts), neighbors=nl)
+        ref = oracle.steinhardt(_bt(box), pts, ref_nl, ls, nthreads=8, **kw)
+        ref64 = oracle.steinhardt(_bt(box), pts, ref_nl, ls, nthreads=8, f64=True, **kw)
+        ok = np.isfinite(ref64["particle_order"])
+        po = st.particle_order
+        # tolerance 1e-5 on q_l / w_l; against the FP64 re-evaluation too
+        npt.assert_allclose(po[ok], ref["particle_order"][ok], atol=1e-5, rtol=0)
+        npt.assert_allclose(po[ok], ref64["particle_order"][ok], atol=1e-5, rtol=0)
+        npt.assert_allclose(st.ql, ref["ql"], atol=1e-5)
+        npt.assert_allclose(st.order, ref["order"], atol=1e-5)
+        for a, b in zip(st.particle_harmonics, ref["qlm"]):
+            npt.assert_allclose(a, b, atol=1e-5)
+
+
+def test_steinhardt_gc_golden():
+    # tests/validation/test_steinhardt_average.py:82-109 of the reference
+    fr = open_gsd(os.path.join(GOLDEN, "steinhardt_average", "Test_Configuration.gsd"))[-1]
+    aq = freud_amd.AABBQuery(fr.freud_box(), fr.positions)
+    nl = aq.query(fr.positions, dict(mode="ball", r_max=1.4, exclude_ii=True)).toNeighborList()
+    for average, name in ((False, "GC_rc1.4_q4q6w4w6"), (True, "GC_rc1.4_avq4avq6avw4avw6")):
+        gc = np.genfromtxt(os.path.join(GOLDEN, "steinhardt_average", f"{name}.txt"))
+        for col, (l, wl) in enumerate([(4, False), (6, False), (4, True), (6, True)]):
+            st = freud_amd.order.Steinhardt(l, average=average, wl=wl, wl_normalize=wl).compute(aq, neighbors=nl)
+            npt.assert_allclose(st.particle_order, gc[:, col], atol=2e-5)
+
+
+def test_steinhardt_fcc_constants():
+    box, pts = freud_amd.data.UnitCell.fcc().generate_system(4, scale=2)
+    for nb in (dict(r_max=1.5), dict(num_neighbors=12)):
+        st = freud_amd.order.Steinhardt(6).compute((box, pts), neighbors=nb)
+        npt.assert_allclose(st.particle_order, 0.57452416, atol=1e-5)
+        npt.assert_allclose(st.order, 0.57452416, atol=1e-5)
+        st = freud_amd.order.Steinhardt(6, wl=True).compute((box, pts), neighbors=nb)
+        npt.assert_allclose(st.particle_order, -0.00262604, atol=1e-5)
+
+
+@pytest.mark.parametrize("l", [1, 2, 5, 6, 11, 12, 19])
+def test_steinhardt_axis_aligned(l):
+    box = Box.cube(10)
+    pts = np.array([[0, 0, -1], [0, 0, 0], [0, 0, 1]], np.float32)
+    st = freud_amd.order.Steinhardt(l).compute((box, pts), neighbors={"num_neighbors": 2})
+    npt.assert_allclose(st.particle_order, [1, 1, 1] if l % 2 == 0 else [1, 0, 1], atol=1e-5)
+
+
+# --------------------------------------------------------------------- SolidLiquid, Cluster, Hexatic
+def test_solidliquid(oracle):
+    box, pts, _ = freud_amd.data.two_phase(5, seed=3)
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.5, exclude_ii=True), nthreads=8)
+    sl = freud_amd.order.SolidLiquid(6, 0.7, 6).compute((box, pts), neighbors=nl)
+    st = oracle.steinhardt(_bt(box), pts, ref_nl, [6], nthreads=8)
+    ref = oracle.solidliquid(ref_nl, st["qlm"][0], st["ql"][:, 0], 6, 0.7, 6)
+    npt.assert_allclose(sl.ql_ij, ref["ql_ij"], atol=1e-5)
+    near = np.abs(ref["ql_ij"] - 0.7) < 1e-5
+    assert not near.any(), "a bond dot product lies within the tolerance of the threshold"
+    npt.assert_array_equal(sl.num_connections, ref["num_connections"])
+    npt.assert_array_equal(sl.cluster_idx, ref["cluster_idx"])
+    assert len(sl.cluster_sizes) == ref["num_clusters"]
+    npt.assert_array_equal(sl.cluster_sizes, np.bincount(ref["cluster_idx"]))
+
+
+def test_solidliquid_fcc():
+    # tests/test_order_solid_liquid.py:56-75 of the reference
+    box, pts = freud_amd.data.UnitCell.fcc().generate_system(4, scale=2)
+    sl = freud_amd.order.SolidLiquid(6, q_threshold=0.7, solid_threshold=6).compute((box, pts),
+                                                                                    neighbors=dict(r_max=1.5))
+    npt.assert_array_equal(sl.num_connections, 12)
+    assert sl.largest_cluster_size == len(pts)
+    npt.assert_array_equal(sl.cluster_idx, 0)
+
+
+def test_cluster(oracle):
+    rng = np.random.default_rng(9)
+    box = Box.cube(20.0)
+    pts = ((rng.random((3000, 3)) - 0.5) * 20).astype(np.float32)
+    cl = freud_amd.cluster.Cluster().compute((box, pts), neighbors=dict(r_max=0.9))
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=0.9, exclude_ii=True), nthreads=8)
+    idx, ncl = oracle.cluster(len(pts), ref_nl["ij"])
+    assert cl.num_clusters == ncl
+    npt.assert_array_equal(cl.cluster_idx, idx)
+
+
+def test_hexatic(oracle):
+    box, pts = freud_amd.data.hex2d_jittered(3000, jitter=0.08, seed=1)
+    hx = freud_amd.order.Hexatic(6).compute((box, pts))
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(num_neighbors=6, exclude_ii=True), nthreads=8)
+    ref = oracle.hexatic(_bt(box), pts, ref_nl, k=6)
+    npt.assert_allclose(hx.particle_order, ref, atol=1e-5)
+    with pytest.raises(ValueError):
+        freud_amd.order.Hexatic(6).compute((Box.cube(10), np.zeros((4, 3), np.float32)))
+
+
+# --------------------------------------------------------------------- NeighborList ops
+def test_nlist_ops(oracle):
+    name, box, pts = SYSTEMS[1]
+    aq = freud_amd.AABBQuery(box, pts)
+    nl = aq.query(pts, dict(r_max=2.0, exclude_ii=True)).toNeighborList()
+    ref = oracle.query(_bt(box), pts, pts, dict(r_max=2.0, exclude_ii=True), nthreads=8)
+    # filter_r keeps r_min <= d < r_max (NeighborList.cc:286-307)
+    c = nl.copy().filter_r(1.5, 0.5)
+    keep = (ref["distances"] >= 0.5) & (ref["distances"] < 1.5)
+    npt.assert_array_equal(c[:], ref["ij"][keep])
+    npt.assert_array_equal(c.distances, ref["distances"][keep])
+    # filter by mask
+    mask = (ref["ij"][:, 1] % 3) == 0
+    c2 = nl.copy().filter(mask)
+    npt.assert_array_equal(c2[:], ref["ij"][mask])
+    npt.assert_array_equal(c2.vectors, ref["vectors"][mask])
+    # sort by distance then back (NeighborList.cc:371-399)
+    sd = nl.copy().sort(by_distance=True)
+    order = np.lexsort((ref["ij"][:, 1], ref["distances"], ref["ij"][:, 0]))
+    npt.assert_array_equal(sd[:], ref["ij"][order])
+    npt.assert_array_equal(sd.copy().sort()[:], ref["ij"])
+    # find_first_index
+    for i in (0, 1, 17, len(pts) - 1):
+        assert nl.find_first_index(i) == int(np.searchsorted(ref["ij"][:, 0], i))
+    # from_arrays round trip and validation (NeighborList.cc:51-82)
+    fa = freud_amd.NeighborList.from_arrays(len(pts), len(pts), ref["ij"][:, 0], ref["ij"][:, 1], ref["vectors"])
+    npt.assert_array_equal(fa[:], ref["ij"])
+    npt.assert_array_equal(fa.distances, ref["distances"])
+    npt.assert_array_equal(fa.neighbor_counts, nl.neighbor_counts)
+    with pytest.raises(ValueError, match="sorted"):
+        freud_amd.NeighborList.from_arrays(4, 4, [1, 0], [0, 1], np.zeros((2, 3)))
+    with pytest.raises(ValueError, match="num_points"):
+        freud_amd.NeighborList.from_arrays(4, 4, [0, 1], [0, 9], np.zeros((2, 3)))
+
+
+def test_all_pairs(oracle):
+    rng = np.random.default_rng(1)
+    box = Box.cube(5.0)
+    pts = ((rng.random((60, 3)) - 0.5) * 5).astype(np.float32)
+    qp = ((rng.random((40, 3)) - 0.5) * 5).astype(np.float32)
+    for q, ex in ((None, True), (qp, False), (qp, True)):
+        nl = freud_amd.NeighborList.all_pairs((box, pts), query_points=q, exclude_ii=ex)
+        ref = oracle.all_pairs(_bt(box), pts, pts if q is None else q, exclude_ii=ex)
+        _assert_nlist_equal(nl, ref)
+
+
+def test_query_errors():
+    box = Box.cube(10)
+    pts = np.zeros((4, 3), np.float32)
+    aq = freud_amd.AABBQuery(box, pts)
+    with pytest.raises(RuntimeError, match="too large"):
+        aq.query(pts, dict(r_max=5.0)).toNeighborList()
+    with pytest.raises(ValueError):
+        aq.query(pts, dict(r_max=1.0, r_min=2.0)).toNeighborList()
+    with pytest.raises(RuntimeError):
+        aq.query(pts, dict(mode="ball")).toNeighborList()
+    with pytest.raises(ValueError):
+        freud_amd.AABBQuery(box, np.zeros((0, 3), np.float32))
+    with pytest.raises(RuntimeError, match="half the box"):
+        freud_amd.LinkCell(box, pts, cell_width=6.0)
+    with pytest.raises(ValueError, match="z != 0"):
+        freud_amd.AABBQuery(Box.square(10), np.ones((3, 3), np.float32))
+    with pytest.raises(NotImplementedError):
+        freud_amd.order.Steinhardt(6).compute((box, pts))
+
+
+# --------------------------------------------------------------------- full size properties
+def test_large_frame_properties():
+    """At the benchmark size (N = 1,048,576 FCC): ball bonds are reciprocal
+    (the AABB formula gives d_ij == d_ji for pairs that do not cross the box
+    edge), the RDF counts below r_max of the query equal the NeighborList
+    histogram, and every particle of a jittered FCC crystal has q6 near the
+    perfect value."""
+    box, pts = freud_amd.data.fcc_jittered(64, 1.5874, 0.05 * 1.5874, seed=1)
+    aq = freud_amd.AABBQuery(box, pts)
+    nl = aq.query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
+    ij = nl[:].astype(np.int64)
+    fwd = ij[:, 0] * len(pts) + ij[:, 1]
+    bwd = ij[:, 1] * len(pts) + ij[:, 0]
+    # pairs present in one direction only can only sit within rounding of r_max
+    odd = np.setxor1d(fwd, bwd)
+    d_of = dict(zip(fwd.tolist(), nl.distances.tolist()))
+    assert all(abs(d_of.get(int(k), 1.5) - 1.5) < 1e-5 for k in odd)
+    assert np.all(np.diff(ij[:, 0]) >= 0)
+    rdf = freud_amd.density.RDF(60, 1.5).compute(aq, neighbors=dict(r_max=1.5))
+    inv = np.float32(1) / (np.float32(1.5) / np.float32(60))
+    binned = np.minimum((nl.distances * inv).astype(np.int64), 59)
+    npt.assert_array_equal(rdf.bin_counts, np.bincount(binned, minlength=60))
+    st = freud_amd.order.Steinhardt(6).compute(aq, neighbors=nl)
+    assert np.all(np.abs(st.particle_order - 0.5745) < 0.2)

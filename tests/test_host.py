@@ -1,0 +1,140 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports every
+symbol of include/freud_amd.h, the host Box matches the oracle bit for bit, and
+the Python argument handling mirrors the reference."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+from conftest import ROOT
+from freud_amd import _lib
+from freud_amd.box import Box
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "freud_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*|double)\s+(fa_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_header_symbols():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    handle = ctypes.CDLL(_lib.LIB_PATH)
+    syms = _header_symbols()
+    assert len(syms) > 30
+    missing = [s for s in syms if not hasattr(handle, s)]
+    assert not missing, missing
+    # every header symbol has a Python signature
+    assert set(syms) <= set(_lib.exported_symbols())
+
+
+def test_box_bit_exact_vs_oracle(oracle):
+    rng = np.random.default_rng(0)
+    for p in [(10, 12, 14, 0.3, -0.2, 0.1, False), (9, 9, 0, 0.2, 0, 0, True), (3, 3, 3, 0, 0, 0, False)]:
+        box = Box(*p)
+        v = ((rng.random((500, 3)) - 0.5) * 40).astype(np.float32)
+        if box.is2D:
+            v[:, 2] = 0
+        npt.assert_array_equal(box.wrap(v), oracle.box_op(p, v, "wrap"))
+        npt.assert_array_equal(box.make_fractional(v), oracle.box_op(p, v, "fractional"))
+        f = rng.random((100, 3)).astype(np.float32)
+        npt.assert_array_equal(box.make_absolute(f), oracle.box_op(p, f, "absolute"))
+        npt.assert_array_equal(box.nearest_plane_distance(), oracle.nearest_plane_distance(p))
+
+
+def test_box_api():
+    box = Box(2, 2, 2, 1, 0, 0)
+    npt.assert_allclose(box.wrap([[10, -5, -5], [0, 0.5, 0]]), [[-2, -1, -1], [0, 0.5, 0]], rtol=1e-6)
+    assert Box.cube(3).volume == 27
+    assert Box.square(3).is2D and Box.square(3).dimensions == 2
+    assert Box.from_box([1, 2, 3]) == Box(1, 2, 3)
+    assert Box.from_box({"Lx": 1, "Ly": 2}).is2D
+    b = Box.from_matrix(Box(4, 5, 6, 0.1, 0.2, 0.3).to_matrix())
+    npt.assert_allclose([b.Lx, b.Ly, b.Lz, b.xy, b.xz, b.yz], [4, 5, 6, 0.1, 0.2, 0.3], rtol=1e-6)
+    with pytest.raises(ValueError):
+        Box(0, 1, 1)
+    img = Box.cube(2).get_images([[1.5, 0, -2.5]])
+    npt.assert_array_equal(img, [[1, 0, -1]])
+
+
+def test_query_args():
+    from freud_amd.locality import _QueryArgs
+
+    qa = _QueryArgs.from_dict(dict(r_max=3, exclude_ii=True))
+    assert qa.mode is None and qa.r_max == 3 and qa.exclude_ii
+    qa = _QueryArgs(mode="nearest", num_neighbors=4)
+    assert qa.mode == "nearest" and qa.num_neighbors == 4
+    with pytest.raises(ValueError):
+        _QueryArgs(bad=1)
+    with pytest.raises(ValueError):
+        _QueryArgs(mode="sphere")
+
+
+def test_rdf_arguments_and_axis():
+    from freud_amd.density import RDF
+
+    for args in [(0, 1.0), (10, -1.0), (10, 1.0, -0.5), (10, 1.0, 2.0)]:
+        with pytest.raises(ValueError):
+            RDF(*args)
+    with pytest.raises(ValueError):
+        RDF(10, 1.0, normalization_mode="bad")
+    r = RDF(37, 3.3, 0.5)
+    npt.assert_allclose(r.bin_edges, np.histogram_bin_edges([0.5, 3.3], bins=37, range=(0.5, 3.3)), rtol=1e-5,
+                        atol=1e-6)
+    assert r.nbins == 37 and r.bounds == pytest.approx((0.5, 3.3))
+    with pytest.raises(AttributeError):
+        r.rdf
+
+
+def test_rdf_reduce_matches_oracle(oracle):
+    # RDF.cc:73-99 normalisation in the reference's float order
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    rng = np.random.default_rng(0)
+    for p, fs in [((10, 10, 10, 0, 0, 0, False), 0), ((8, 9, 0, 0, 0, 0, True), 1)]:
+        counts = rng.integers(0, 5000, size=64).astype(np.uint32)
+        box = _lib.fa_box(*p[:6], int(p[6]))
+        rdf = np.empty(64, np.float32)
+        n_r = np.empty(64, np.float32)
+        rc = lib.fa_rdf_reduce(ctypes.byref(box), counts.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint32(64),
+                               ctypes.c_float(0.25), ctypes.c_float(4.0), ctypes.c_uint32(1000), ctypes.c_uint32(900),
+                               ctypes.c_uint32(3), fs, rdf.ctypes.data_as(ctypes.c_void_p),
+                               n_r.ctypes.data_as(ctypes.c_void_p))
+        assert rc == 0
+        ref_rdf, ref_nr = oracle.rdf_reduce(p, counts, 64, 0.25, 4.0, 1000, 900, 3, finite_size=bool(fs))
+        npt.assert_array_equal(rdf, ref_rdf)
+        npt.assert_array_equal(n_r, ref_nr)
+
+
+def test_data_generators():
+    from freud_amd import data
+
+    box, pts = data.fcc_jittered(4, 1.5874, 0.0)
+    assert pts.shape == (256, 3) and pts.dtype == np.float32
+    f = box.make_fractional(pts)
+    assert np.all((f >= 0) & (f < 1))
+    box, pts = data.triclinic_uniform(1000, 0.85)
+    assert np.all(np.abs(box.make_fractional(pts) - 0.5) <= 0.5)
+    box, pts = data.hex2d_jittered(500)
+    assert box.is2D and np.all(pts[:, 2] == 0)
+    box, pts, lab = data.two_phase(3)
+    assert len(pts) == 2 * 4 * 27 and lab.sum() == 4 * 27
+
+
+def test_gsd_reader():
+    from freud_amd.gsd import open_gsd
+
+    from conftest import GOLDEN
+
+    t = open_gsd(os.path.join(GOLDEN, "lj.gsd"))
+    assert len(t) >= 2
+    fr = t[0]
+    assert fr.positions.shape == (1000, 3)
+    npt.assert_allclose(fr.box[:3], 20.0)
+    fr = open_gsd(os.path.join(GOLDEN, "steinhardt_average", "Test_Configuration.gsd"))[-1]
+    assert fr.positions.shape == (3288, 3)
