@@ -445,14 +445,23 @@ __global__ void uf_union_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, c
     }
 }
 
-__global__ void uf_flatten_kernel(uint32_t n, uint32_t* parent, uint32_t* size)
+// Roots into a separate array with a read-only walk: path-halving writes here
+// could overwrite another thread's finished entry with a non-root ancestor.
+__global__ void uf_flatten_kernel(uint32_t n, const uint32_t* __restrict__ parent, uint32_t* __restrict__ root,
+                                  uint32_t* size)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    uint32_t const r = uf_find(parent, i);
-    parent[i] = r;
-    atomicAdd(&size[r], 1u);
+    uint32_t x = i;
+    uint32_t p = parent[x];
+    while (p != x)
+    {
+        x = p;
+        p = parent[x];
+    }
+    root[i] = x;
+    atomicAdd(&size[x], 1u);
 }
 
 // key = (n - size) << 32 | root sorts clusters by size descending, then by
@@ -736,30 +745,33 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
     }
     void* tmp = nullptr;
     size_t const nb4 = sizeof(uint32_t) * size_t(n);
-    if (int rc = fa::scratch(7, 3 * nb4 + sizeof(uint64_t) * size_t(n) + 512, &tmp))
+    if (int rc = fa::scratch(7, 4 * nb4 + sizeof(uint64_t) * size_t(n) + 512, &tmp))
         return rc;
     uint32_t* parent = static_cast<uint32_t*>(tmp);
     uint32_t* size = parent + n;
     uint32_t* rank = size + n;
-    uint32_t* counter = rank + n;
+    uint32_t* root = rank + n;
+    uint32_t* counter = root + n;
     uint64_t* keys = reinterpret_cast<uint64_t*>((reinterpret_cast<uintptr_t>(counter + 4) + 15) & ~uintptr_t(15));
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     FA_KLAUNCH(uf_init_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size);
     if (nl->num_bonds)
         FA_KLAUNCH(uf_union_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            nl->num_bonds, nl->neighbors, d_keep, parent);
-    FA_KLAUNCH(uf_flatten_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size);
+    FA_KLAUNCH(uf_flatten_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, root, size);
     FA_HIP(hipMemsetAsync(counter, 0, sizeof(uint32_t), s));
     FA_KLAUNCH(uf_keys_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size, counter, keys);
     FA_HIP(hipGetLastError());
     uint32_t ncl = 0;
     FA_HIP(hipMemcpyAsync(&ncl, counter, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
+    if (ncl == 0)
+        return fa::fail(FA_ERR_RUNTIME, "cluster labelling found no roots");
     if (int rc = fa::sort_u64(keys, ncl))
         return rc;
     FA_KLAUNCH(uf_rank_kernel, dim3((ncl + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n, ncl, keys, rank,
                        d_cluster_sizes);
-    FA_KLAUNCH(uf_label_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, rank, d_cluster_idx);
+    FA_KLAUNCH(uf_label_kernel, dim3(blocks), dim3(kBlock), 0, s, n, root, rank, d_cluster_idx);
     FA_HIP(hipGetLastError());
     FA_HIP(hipStreamSynchronize(s));
     *num_clusters = ncl;

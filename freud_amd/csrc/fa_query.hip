@@ -108,8 +108,10 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
     uint32_t const i0 = blockIdx.x * kBlock;
     uint32_t const nloc = min(uint32_t(kBlock), nq - i0);
     uint32_t const base = offsets[i0];
-    if (tid <= nloc)
+    if (tid < nloc)
         s_off[tid] = offsets[i0 + tid] - base;
+    if (tid == 0)
+        s_off[nloc] = offsets[i0 + nloc] - base;
     __syncthreads();
     uint32_t const total = s_off[nloc];
     K* keys = (total <= CAP) ? s_keys : reinterpret_cast<K*>(gkeys_raw) + base;
@@ -246,6 +248,16 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
             extended = true;
             break;
         }
+    }
+    // Two images of one point are at least a plane distance apart, so only a
+    // selection reaching half the smallest plane distance can hold a point
+    // twice; such queries take the exhaustive, deduplicating kernel.
+    uint32_t const nsel = min(found, k);
+    if (!extended && rlim > half_min_plane && nsel > 0)
+    {
+        float const last = sqrtf(__uint_as_float(uint32_t(best.at(nsel - 1) >> 32)));
+        if (last >= half_min_plane)
+            extended = true;
     }
     if (extended)
     {
@@ -647,12 +659,14 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
         return rc;
     float const r_max_sq = a.r_max * a.r_max, r_min_sq = a.r_min * a.r_min;
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
-    if (nq->kind == FA_NQ_AABB)
+    if (n_query == 0)
+        ;
+    else if (nq->kind == FA_NQ_AABB)
         FA_KLAUNCH(ball_count_kernel<AABB_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query,
-                           r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
+                   r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
     else
-        FA_KLAUNCH(ball_count_kernel<LINKCELL_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts,
-                           n_query, r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
+        FA_KLAUNCH(ball_count_kernel<LINKCELL_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query,
+                   r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
     FA_HIP(hipGetLastError());
     uint64_t total = 0;
     if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))

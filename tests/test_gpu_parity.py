@@ -361,4 +361,11 @@ def test_large_frame_properties():
     binned = np.minimum((nl.distances * inv).astype(np.int64), 59)
     npt.assert_array_equal(rdf.bin_counts, np.bincount(binned, minlength=60))
     st = freud_amd.order.Steinhardt(6).compute(aq, neighbors=nl)
-    assert np.all(np.abs(st.particle_order - 0.5745) < 0.2)
+    # thermal jitter (sigma = 0.05 a) lowers q6 from the perfect 0.5745 but the
+    # crystal stays clearly ordered (a liquid gives ~0.1-0.3 per particle)
+    q6 = st.particle_order
+    assert np.all(np.isfinite(q6)) and 0.3 < q6.mean() < 0.5745
+    # q_l is rotation invariant: the same frame rotated by 90 degrees about z
+    rot = np.ascontiguousarray(pts[:, [1, 0, 2]] * np.array([-1, 1, 1], np.float32))
+    st2 = freud_amd.order.Steinhardt(6).compute((box, rot), neighbors=dict(r_max=1.5))
+    npt.assert_allclose(np.sort(st2.particle_order), np.sort(q6), atol=1e-5)
