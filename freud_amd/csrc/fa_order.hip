@@ -343,6 +343,50 @@ __global__ __launch_bounds__(kBlock) void wl_kernel(uint32_t n, int l, const flo
     wl[uint64_t(i) * stride + col] = result;
 }
 
+// The same reduction with q_lm held in registers and the (m1, m2) loops
+// unrolled at compile time (coefficients are wave-uniform loads).
+template<int L>
+__global__ __launch_bounds__(kBlock) void wl_kernel_t(uint32_t n, const float2* __restrict__ src,
+                                                      const float* __restrict__ w3j, const float* __restrict__ qlnorm,
+                                                      int wl_normalize, float* __restrict__ wl, int stride, int col)
+{
+    constexpr int NM = 2 * L + 1;
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float2 q[NM];
+    const float2* qi = src + uint64_t(i) * NM;
+#pragma unroll
+    for (int t = 0; t < NM; ++t)
+        q[t] = qi[t];
+    float result = 0.0f;
+    int counter = 0;
+#pragma unroll
+    for (int m1 = -L; m1 <= L; ++m1)
+    {
+        float2 const a = q[m1 < 0 ? L - m1 : m1];
+#pragma unroll
+        for (int m2 = (-L - m1 > -L ? -L - m1 : -L); m2 <= (L - m1 < L ? L - m1 : L); ++m2)
+        {
+            int const m3 = -m1 - m2;
+            float2 const bq = q[m2 < 0 ? L - m2 : m2];
+            float2 const c = q[m3 < 0 ? L - m3 : m3];
+            float const w = w3j[counter++];
+            float const ar = w * a.x, ai = w * a.y;
+            float const abr = ar * bq.x - ai * bq.y;
+            float const abi = ar * bq.y + ai * bq.x;
+            result += abr * c.x - abi * c.y;
+        }
+    }
+    if (wl_normalize)
+    {
+        float const nf = float(4.0 * M_PI / NM);
+        float const norm = sqrtf(nf) / qlnorm[uint64_t(i) * stride + col];
+        result *= norm * norm * norm;
+    }
+    wl[uint64_t(i) * stride + col] = result;
+}
+
 // SolidLiquid.cc:51-78 (bond dot products) and :90-96 (solid-bond counts)
 __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, const uint32_t* __restrict__ nbr,
                                                              const uint32_t* __restrict__ offsets,
@@ -660,8 +704,32 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
             float* w3 = nullptr;
             if (int rc = wigner_table(l, &w3))
                 return rc;
-            FA_KLAUNCH(wl_kernel, dim3(blocks), dim3(kBlock), 0, s, n, l, src, w3, average ? d_qlave : d_ql,
-                               int(wl_norm), d_wl, n_l, k);
+            const float* qn = average ? d_qlave : d_ql;
+            switch (l)
+            {
+#define FA_WL_CASE(LL)                                                                                        \
+    case LL:                                                                                                  \
+        FA_KLAUNCH(wl_kernel_t<LL>, dim3(blocks), dim3(kBlock), 0, s, n, src, w3, qn, int(wl_norm), d_wl, n_l, \
+                   k);                                                                                        \
+        break;
+                FA_WL_CASE(0)
+                FA_WL_CASE(1)
+                FA_WL_CASE(2)
+                FA_WL_CASE(3)
+                FA_WL_CASE(4)
+                FA_WL_CASE(5)
+                FA_WL_CASE(6)
+                FA_WL_CASE(7)
+                FA_WL_CASE(8)
+                FA_WL_CASE(9)
+                FA_WL_CASE(10)
+                FA_WL_CASE(11)
+                FA_WL_CASE(12)
+#undef FA_WL_CASE
+            default:
+                FA_KLAUNCH(wl_kernel, dim3(blocks), dim3(kBlock), 0, s, n, l, src, w3, qn, int(wl_norm), d_wl, n_l,
+                           k);
+            }
             FA_HIP(hipGetLastError());
         }
         // system order: Steinhardt.cc:291-327

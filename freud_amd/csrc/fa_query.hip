@@ -35,15 +35,57 @@ __device__ inline uint32_t float_order_bits(float v)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-template<int FORMULA>
+// Query point at processing position p: in SELF mode (query points are the
+// grid's own points) positions run in cell order through the grid's sorted
+// copy, so neighbouring lanes share stencil cells; otherwise in caller order.
+template<bool SELF>
+__device__ inline uint32_t query_at(const DevGrid& g, const float* __restrict__ qpts, uint32_t p, float& qx, float& qy,
+                                    float& qz)
+{
+    if (SELF)
+    {
+        float4 const q = g.cpos[p];
+        qx = q.x;
+        qy = q.y;
+        qz = q.z;
+        return __float_as_uint(q.w);
+    }
+    qx = qpts[3 * p];
+    qy = qpts[3 * p + 1];
+    qz = qpts[3 * p + 2];
+    return p;
+}
+
+// exclusive prefix sum over a block of kBlock threads; returns the total
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& excl)
+{
+    uint32_t const tid = threadIdx.x;
+    s_tmp[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1)
+    {
+        uint32_t const add = (tid >= uint32_t(off)) ? s_tmp[tid - off] : 0u;
+        __syncthreads();
+        s_tmp[tid] += add;
+        __syncthreads();
+    }
+    uint32_t const incl = s_tmp[tid];
+    uint32_t const total = s_tmp[kBlock - 1];
+    __syncthreads();
+    excl = incl - v;
+    return total;
+}
+
+template<int FORMULA, bool SELF>
 __global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
                                                             uint32_t nq, float r_max_sq, float r_min_sq,
                                                             int exclude_ii, int R, uint32_t* __restrict__ counts)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq)
+    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nq)
         return;
-    float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
+    float qx, qy, qz;
+    uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
     uint32_t c = 0;
     fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
                                     [&](uint32_t j, float, float, float, float r_sq, int) {
@@ -91,7 +133,34 @@ __device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpt
     v_out[3 * bidx + 2] = rz;
 }
 
-template<int FORMULA, bool BYDIST>
+// Block-cooperative store of a block's bonds: keys[t] for t < total belong to
+// the block's query points in processing order (exclusive offsets s_off,
+// point indices s_i); bond t of point lo goes to offsets[s_i[lo]] + t - s_off[lo].
+template<int FORMULA, bool BYDIST, typename K>
+__device__ inline void store_block_bonds(const DevBox& b, const float* __restrict__ qpts, const float* __restrict__ pts,
+                                         const K* keys, uint32_t total, uint32_t nloc, const uint32_t* s_off,
+                                         const uint32_t* s_i, const uint32_t* __restrict__ offsets,
+                                         uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                         float* __restrict__ w_out, float* __restrict__ v_out)
+{
+    for (uint32_t t = threadIdx.x; t < total; t += kBlock)
+    {
+        uint32_t lo = 0, hi = nloc;
+        while (hi - lo > 1)
+        {
+            uint32_t const mid = (lo + hi) >> 1;
+            if (s_off[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        uint32_t const i = s_i[lo];
+        write_bond<FORMULA, BYDIST>(b, qpts, pts, i, uint64_t(keys[t]), uint64_t(offsets[i]) + (t - s_off[lo]),
+                                    nb_out, d_out, w_out, v_out);
+    }
+}
+
+template<int FORMULA, bool BYDIST, bool SELF>
 __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
                                                            uint32_t nq, const float* __restrict__ pts, float r_max_sq,
                                                            float r_min_sq, int exclude_ii, int R,
@@ -104,23 +173,33 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
     constexpr uint32_t CAP = kKeyBytes / sizeof(K);
     __shared__ K s_keys[CAP];
     __shared__ uint32_t s_off[kBlock + 1];
+    __shared__ uint32_t s_i[kBlock];
+    __shared__ uint32_t s_tmp[kBlock];
     uint32_t const tid = threadIdx.x;
-    uint32_t const i0 = blockIdx.x * kBlock;
-    uint32_t const nloc = min(uint32_t(kBlock), nq - i0);
-    uint32_t const base = offsets[i0];
-    if (tid < nloc)
-        s_off[tid] = offsets[i0 + tid] - base;
-    if (tid == 0)
-        s_off[nloc] = offsets[i0 + nloc] - base;
-    __syncthreads();
-    uint32_t const total = s_off[nloc];
-    K* keys = (total <= CAP) ? s_keys : reinterpret_cast<K*>(gkeys_raw) + base;
+    uint32_t const p0 = blockIdx.x * kBlock;
+    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
+    float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+    uint32_t i = 0, cnt = 0;
     if (tid < nloc)
     {
-        uint32_t const i = i0 + tid;
-        float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
-        uint32_t const beg = s_off[tid];
-        uint32_t const end = s_off[tid + 1];
+        i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
+        cnt = offsets[i + 1] - offsets[i];
+        s_i[tid] = i;
+    }
+    uint32_t excl;
+    uint32_t const total = block_exclusive_scan(cnt, s_tmp, excl);
+    if (tid < nloc)
+        s_off[tid] = excl;
+    if (tid == 0)
+        s_off[nloc] = total;
+    __syncthreads();
+    bool const in_lds = total <= CAP;
+    // overflow: keys go to the point's own segment of a global scratch array
+    K* keys = in_lds ? s_keys : reinterpret_cast<K*>(gkeys_raw);
+    if (tid < nloc)
+    {
+        uint32_t const beg = in_lds ? excl : offsets[i];
+        uint32_t const end = beg + cnt;
         uint32_t k = beg;
         fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
                                         [&](uint32_t j, float, float, float, float r_sq, int code) {
@@ -137,22 +216,15 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
                                             }
                                         });
         insertion_sort(keys + beg, k - beg);
+        if (!in_lds)
+            for (uint32_t m = 0; m < k - beg; ++m)
+                write_bond<FORMULA, BYDIST>(b, qpts, pts, i, uint64_t(keys[beg + m]), uint64_t(offsets[i]) + m,
+                                            nb_out, d_out, w_out, v_out);
     }
     __syncthreads();
-    for (uint32_t t = tid; t < total; t += kBlock)
-    {
-        uint32_t lo = 0, hi = nloc;
-        while (hi - lo > 1)
-        {
-            uint32_t const mid = (lo + hi) >> 1;
-            if (s_off[mid] <= t)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        write_bond<FORMULA, BYDIST>(b, qpts, pts, i0 + lo, uint64_t(keys[t]), uint64_t(base) + t, nb_out, d_out,
-                                    w_out, v_out);
-    }
+    if (in_lds)
+        store_block_bonds<FORMULA, BYDIST>(b, qpts, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+                                           w_out, v_out);
 }
 
 // Segments with the reference's convention (0 for query points without bonds).
@@ -204,7 +276,7 @@ template<int KMAX> struct TopK
 // kept.  A query whose answer needs radii beyond half the smallest plane
 // distance (where one point can appear through two images) is flagged for the
 // exhaustive kernel below.
-template<int KMAX>
+template<int KMAX, bool SELF>
 __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
                                                           uint32_t nq, uint32_t k, float rlim_sq, float rlim,
                                                           float r_min, int exclude_ii, float wmin,
@@ -212,10 +284,11 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
                                                           uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
                                                           uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq)
+    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nq)
         return;
-    float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
+    float qx, qy, qz;
+    uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
     TopK<KMAX> best;
     best.init();
     uint32_t found = 0;
@@ -389,36 +462,68 @@ __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const 
 
 // Writes a kNN result: per query, its selected keys sorted into the
 // NeighborList order ((i, j) or (i, d, j)).
-template<int FORMULA, bool BYDIST>
-__global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, const float* __restrict__ qpts, uint32_t nq,
-                                                           const float* __restrict__ pts, uint32_t k,
+// Writes a kNN result: per query, its selected keys sorted into the
+// NeighborList order ((i, j) or (i, d, j)), staged per block in LDS and stored
+// cooperatively (query points in the same processing order as the search).
+template<int FORMULA, bool BYDIST, bool SELF>
+__global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, const float* __restrict__ pts, uint32_t k,
                                                            const uint64_t* __restrict__ sel,
                                                            const uint32_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                                            float* __restrict__ w_out, float* __restrict__ v_out,
                                                            uint64_t* __restrict__ tmp)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq)
-        return;
-    uint32_t const off = offsets[i];
-    uint32_t const n = offsets[i + 1] - off;
-    uint64_t* keys = tmp + uint64_t(i) * k;
-    for (uint32_t m = 0; m < n; ++m)
+    using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
+    constexpr uint32_t CAP = kKeyBytes / sizeof(K);
+    __shared__ K s_keys[CAP];
+    __shared__ uint32_t s_off[kBlock + 1];
+    __shared__ uint32_t s_i[kBlock];
+    __shared__ uint32_t s_tmp[kBlock];
+    uint32_t const tid = threadIdx.x;
+    uint32_t const p0 = blockIdx.x * kBlock;
+    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
+    uint32_t i = 0, n = 0;
+    if (tid < nloc)
     {
-        uint64_t const s = sel[uint64_t(i) * k + m];
-        uint32_t const jc = uint32_t(s);
-        if (BYDIST)
-        {
-            float const d = sqrtf(__uint_as_float(uint32_t(s >> 32)));
-            keys[m] = (uint64_t(float_order_bits(d)) << 32) | jc;
-        }
-        else
-            keys[m] = jc;
+        float qx, qy, qz;
+        i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
+        n = offsets[i + 1] - offsets[i];
+        s_i[tid] = i;
     }
-    insertion_sort(keys, n);
-    for (uint32_t m = 0; m < n; ++m)
-        write_bond<FORMULA, BYDIST>(b, qpts, pts, i, keys[m], uint64_t(off) + m, nb_out, d_out, w_out, v_out);
+    uint32_t excl;
+    uint32_t const total = block_exclusive_scan(n, s_tmp, excl);
+    if (tid < nloc)
+        s_off[tid] = excl;
+    if (tid == 0)
+        s_off[nloc] = total;
+    __syncthreads();
+    bool const in_lds = total <= CAP;
+    if (tid < nloc)
+    {
+        K* keys = in_lds ? s_keys + excl : reinterpret_cast<K*>(tmp + uint64_t(i) * k);
+        for (uint32_t m = 0; m < n; ++m)
+        {
+            uint64_t const s = sel[uint64_t(i) * k + m];
+            uint32_t const jc = uint32_t(s);
+            if (BYDIST)
+            {
+                float const d = sqrtf(__uint_as_float(uint32_t(s >> 32)));
+                keys[m] = K((uint64_t(float_order_bits(d)) << 32) | jc);
+            }
+            else
+                keys[m] = K(jc);
+        }
+        insertion_sort(keys, n);
+        if (!in_lds)
+            for (uint32_t m = 0; m < n; ++m)
+                write_bond<FORMULA, BYDIST>(b, qpts, pts, i, uint64_t(keys[m]), uint64_t(offsets[i]) + m, nb_out,
+                                            d_out, w_out, v_out);
+    }
+    __syncthreads();
+    if (in_lds)
+        store_block_bonds<FORMULA, BYDIST>(b, qpts, pts, s_keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+                                           w_out, v_out);
 }
 
 // ------------------------------------------------------------------ all pairs
@@ -625,12 +730,20 @@ namespace {
 template<int KMAX>
 int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, uint32_t nq, uint32_t k, float rlim,
                     float r_min, int exclude_ii, float wmin, float hmp, int rmax_ring, uint32_t* counts,
-                    uint64_t* sel, uint32_t* flagged, uint32_t* nflag)
+                    uint64_t* sel, uint32_t* flagged, uint32_t* nflag, const float* self_points, uint32_t self_n)
 {
     float const rlim_sq = rlim * rlim;
-    FA_KLAUNCH(knn_aabb_kernel<KMAX>, dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), db, g,
-                       qpts, nq, k, rlim_sq, rlim, r_min, exclude_ii, wmin, hmp, rmax_ring, counts, sel, flagged,
-                       nflag);
+    bool const self = (qpts == self_points) && (nq == self_n);
+    FA_PROF_BEGIN("knn_aabb_kernel");
+    if (self)
+        hipLaunchKernelGGL((knn_aabb_kernel<KMAX, true>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, r_min, exclude_ii, wmin, hmp, rmax_ring,
+                           counts, sel, flagged, nflag);
+    else
+        hipLaunchKernelGGL((knn_aabb_kernel<KMAX, false>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, r_min, exclude_ii, wmin, hmp, rmax_ring,
+                           counts, sel, flagged, nflag);
+    FA_PROF_END();
     FA_HIP(hipGetLastError());
     return FA_OK;
 }
@@ -659,14 +772,20 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
         return rc;
     float const r_max_sq = a.r_max * a.r_max, r_min_sq = a.r_min * a.r_min;
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
+    // query points == the grid's points: process them in cell order
+    bool const self = (qpts == nq->d_points) && (n_query == nq->n);
+    auto count = [&](auto kern) {
+        FA_PROF_BEGIN("ball_count_kernel");
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query, r_max_sq, r_min_sq,
+                           a.exclude_ii, R, out->counts);
+        FA_PROF_END();
+    };
     if (n_query == 0)
         ;
     else if (nq->kind == FA_NQ_AABB)
-        FA_KLAUNCH(ball_count_kernel<AABB_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query,
-                   r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
+        self ? count(ball_count_kernel<AABB_FORMULA, true>) : count(ball_count_kernel<AABB_FORMULA, false>);
     else
-        FA_KLAUNCH(ball_count_kernel<LINKCELL_FORMULA>, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query,
-                   r_max_sq, r_min_sq, a.exclude_ii, R, out->counts);
+        self ? count(ball_count_kernel<LINKCELL_FORMULA, true>) : count(ball_count_kernel<LINKCELL_FORMULA, false>);
     FA_HIP(hipGetLastError());
     uint64_t total = 0;
     if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
@@ -689,10 +808,23 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
     if (n_query)
     {
         if (nq->kind == FA_NQ_AABB)
-            by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true>) : launch(ball_fill_kernel<AABB_FORMULA, false>);
+        {
+            if (self)
+                by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true, true>)
+                            : launch(ball_fill_kernel<AABB_FORMULA, false, true>);
+            else
+                by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true, false>)
+                            : launch(ball_fill_kernel<AABB_FORMULA, false, false>);
+        }
         else
-            by_distance ? launch(ball_fill_kernel<LINKCELL_FORMULA, true>)
-                        : launch(ball_fill_kernel<LINKCELL_FORMULA, false>);
+        {
+            if (self)
+                by_distance ? launch(ball_fill_kernel<LINKCELL_FORMULA, true, true>)
+                            : launch(ball_fill_kernel<LINKCELL_FORMULA, false, true>);
+            else
+                by_distance ? launch(ball_fill_kernel<LINKCELL_FORMULA, true, false>)
+                            : launch(ball_fill_kernel<LINKCELL_FORMULA, false, false>);
+        }
         FA_HIP(hipGetLastError());
     }
     return fa::nlist_finish_counts(out);
@@ -772,16 +904,16 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         int rc;
         if (k <= 8)
             rc = launch_knn_aabb<8>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                    min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+                                    min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
         else if (k <= 16)
             rc = launch_knn_aabb<16>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
         else if (k <= 32)
             rc = launch_knn_aabb<32>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
         else if (k <= 64)
             rc = launch_knn_aabb<64>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag);
+                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
         else
         {
             // every query takes the exhaustive path
@@ -849,17 +981,32 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         return rc;
     out->num_bonds = total;
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
+    bool const self = (qpts == nq->d_points) && (n_query == nq->n);
+    DevGrid const g = nq->cur->dev();
     auto launch = [&](auto kern) {
         FA_PROF_BEGIN("knn_write_kernel");
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, *dbp, qpts, n_query, nq->d_points, k, sel,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, *dbp, g, qpts, n_query, nq->d_points, k, sel,
                            out->offsets, out->neighbors, out->distances, out->weights, out->vectors, wtmp);
         FA_PROF_END();
     };
     if (nq->kind == FA_NQ_AABB)
-        by_distance ? launch(knn_write_kernel<AABB_FORMULA, true>) : launch(knn_write_kernel<AABB_FORMULA, false>);
+    {
+        if (self)
+            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, true>)
+                        : launch(knn_write_kernel<AABB_FORMULA, false, true>);
+        else
+            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, false>)
+                        : launch(knn_write_kernel<AABB_FORMULA, false, false>);
+    }
     else
-        by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true>)
-                    : launch(knn_write_kernel<LINKCELL_FORMULA, false>);
+    {
+        if (self)
+            by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, true>)
+                        : launch(knn_write_kernel<LINKCELL_FORMULA, false, true>);
+        else
+            by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, false>)
+                        : launch(knn_write_kernel<LINKCELL_FORMULA, false, false>);
+    }
     FA_HIP(hipGetLastError());
     return fa::nlist_finish_counts(out);
 }

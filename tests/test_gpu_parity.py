@@ -144,6 +144,34 @@ def test_rdf_counts_bit_exact(oracle, name, box, pts):
     npt.assert_array_equal(rdf2.bin_counts, counts2)
 
 
+@pytest.mark.parametrize("L,r_max,n,layout", [
+    (8.0, 3.9, 600, "uniform"),     # 4 cells per side: stencil wraps onto itself
+    (7.0, 3.2, 500, "uniform"),     # triclinic (see below)
+    (6.0, 2.9, 300, "clumped"),     # ragged cells, many empty
+    (20.0, 4.0, 5000, "clumped"),
+    (10.0, 2.0, 1, "uniform"),      # a single point
+    (10.0, 4.9, 1500, "outside"),   # points outside the box: generic path
+    (12.0, 5.9, 900, "2d"),
+])
+def test_rdf_self_cluster_cases(oracle, L, r_max, n, layout):
+    rng = np.random.default_rng(int(L * 10) + n)
+    box = Box(L, L * 1.1, L * 0.95, 0.2, -0.1, 0.15) if n == 500 else Box.cube(L)
+    if layout == "2d":
+        box = Box.square(L)
+    f = rng.random((n, 3))
+    if layout == "clumped":
+        f = (np.floor(f * 3) / 3 + rng.random((n, 3)) * 0.05) % 1.0
+    pts = box.make_absolute(f.astype(np.float32))
+    if layout == "outside":
+        pts = pts + np.float32(0.3 * L) * (rng.random((n, 3)) > 0.8).astype(np.float32)
+    if layout == "2d":
+        pts[:, 2] = 0
+    bins = 37
+    rdf = freud_amd.density.RDF(bins=bins, r_max=r_max, r_min=0.25).compute((box, pts))
+    counts = oracle.rdf_accumulate(_bt(box), pts, pts, bins, r_max, r_min=0.25, exclude_ii=True, nthreads=8)
+    npt.assert_array_equal(rdf.bin_counts, counts)
+
+
 def test_rdf_lj_golden():
     # tests/integration/test_reader_integrations.py:27-40 of the reference
     traj = open_gsd(os.path.join(GOLDEN, "lj.gsd"))
