@@ -107,6 +107,7 @@ def algorithmic_bytes(name, n, b_ball, b_knn, k=12, l=6, bins=200):
         "knn_aabb_kernel": 12 * n + 12 * n + 4 * n + 8 * k * n,
         "knn_write_kernel": 8 * k * n + 4 * (n + 1) + 28 * b_knn,
         "rdf_self_kernel": 16 * n + 4 * bins,
+        "rdf_cluster_kernel": 16 * n + 4 * bins,
         "qlm_kernel": 24 * b_ball + 8 * nm * n + 4 * n,
         "qlm_ave_kernel": 8 * b_ball + 8 * nm * b_ball + 8 * nm * n + 4 * n,
         "wl_kernel": 8 * nm * n + 4 * n + 4 * n,
@@ -117,6 +118,19 @@ def algorithmic_bytes(name, n, b_ball, b_knn, k=12, l=6, bins=200):
         if name.startswith(key) or key in name:
             return v
     return None
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    same counters on a copy of known size) from the committed PMC summary, or
+    None when no summary covers the kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_frame_1M.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get("kernels", {}).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
 
 
 def cpu_baseline(box, pts, budget_s=15.0):
@@ -155,12 +169,153 @@ def cpu_baseline(box, pts, budget_s=15.0):
     }
 
 
+class FrameLoop:
+    """The per-frame plain GPU loop over a trajectory (what a freud user writes:
+    one query / RDF / Steinhardt call sequence per frame), through the C-ABI."""
+
+    def __init__(self, frames, bins=200, l=6):
+        from freud_amd import _lib
+        from freud_amd._lib import DeviceArray
+
+        self.L = _lib
+        self.lib = _lib.lib()
+        self.frames = []
+        for box, pts in frames:
+            n = len(pts)
+            self.frames.append((_lib.make_box(box), DeviceArray.from_numpy(pts), n))
+        nmax = max(n for _, _, n in self.frames)
+        self.nl = _lib.fa_nlist()
+        self.counts = DeviceArray.zeros(len(frames) * bins, np.uint32)
+        self.bins = bins
+        self.d_qlm = DeviceArray((nmax * (2 * l + 1) * 2,), np.float32)
+        self.d_ql = DeviceArray((nmax,), np.float32)
+        self.ls = np.array([l], np.uint32)
+        self.order = np.zeros(1, np.float32)
+        self.ball = _lib.fa_query_args(_lib.MODE_BALL, 0xFFFFFFFF, 1.5, 0.0, -1.0, -1.0, 1)
+
+    def run(self):
+        L, lib, check = self.L, self.lib, self.L.check
+        self.counts.zero()
+        for f, (cbox, d_pts, n) in enumerate(self.frames):
+            h = ctypes.c_void_p()
+            check(lib.fa_nq_create(ctypes.byref(cbox), d_pts.ptr, n, L.NQ_AABB, 0.0, ctypes.byref(h)))
+            try:
+                check(lib.fa_query(h, d_pts.ptr, n, ctypes.byref(self.ball), 0, ctypes.byref(self.nl)))
+                check(lib.fa_rdf_accumulate_query(h, d_pts.ptr, n, 5.0, 0.0, 1, self.bins, 0.0, 5.0,
+                                                  self.counts.ptr + 4 * f * self.bins))
+                check(lib.fa_steinhardt(ctypes.byref(cbox), d_pts.ptr, n, ctypes.byref(self.nl), self.ls.ctypes.data,
+                                        1, 0, self.d_qlm.ptr, self.d_ql.ptr, None, None, None,
+                                        self.order.ctypes.data))
+            finally:
+                lib.fa_nq_destroy(h)
+        self.L.synchronize()
+
+
+def run_batched(args):
+    """256 frames, N log-uniform in [10k, 200k], liquid or jittered FCC/BCC,
+    cubic or triclinic: ball query r=1.5 -> NeighborLists, RDF 200 bins r=5,
+    Steinhardt q6 per frame.  One step = the whole trajectory, run as one
+    batch replayed from a HIP graph; the per-frame loop is timed beside it."""
+    from freud_amd import _lib, data
+    from freud_amd.batch import TrajectoryBatch
+
+    frames, kinds = data.mixed_frames(256, 10_000, 200_000, seed=4)
+    total = sum(len(p) for _, p in frames)
+    print(f"[bench] batched: {len(frames)} frames, {total} points", file=sys.stderr, flush=True)
+    batch = TrajectoryBatch(frames, ball_r_max=1.5, rdf_bins=200, rdf_r_max=5.0, l=6)
+    for w in range(max(args.warmup, 2)):
+        batch.compute(use_graph=True)
+        print(f"[bench] batch warmup {w} done, config {batch.config}", file=sys.stderr, flush=True)
+    _lib.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.compute(use_graph=True)
+    t1 = time.perf_counter()
+    ms_batch = (t1 - t0) * 1e3 / args.steps
+    bonds = int(batch.num_bonds.astype(np.int64).sum())
+    counts = batch.rdf_counts
+    rdf_pairs = int(counts.astype(np.int64).sum())
+    # eager (no graph) batch for the per-kernel breakdown
+    _lib.profile_reset()
+    _lib.profile_enable(True)
+    t2 = time.perf_counter()
+    batch.compute(use_graph=False)
+    t3 = time.perf_counter()
+    _lib.profile_enable(False)
+    stats = _lib.profile_stats()
+    ms_eager = (t3 - t2) * 1e3
+    print(f"[bench] batch {ms_batch:.2f} ms/trajectory (graph), {ms_eager:.2f} ms eager; bonds {bonds}, rdf pairs "
+          f"{rdf_pairs}", file=sys.stderr, flush=True)
+
+    loop = FrameLoop(frames)
+    loop.run()
+    steps_loop = max(1, min(args.steps, 3))
+    t4 = time.perf_counter()
+    for _ in range(steps_loop):
+        loop.run()
+    t5 = time.perf_counter()
+    ms_loop = (t5 - t4) * 1e3 / steps_loop
+    loop_counts = loop.counts.numpy().reshape(len(frames), 200)
+    same = bool(np.array_equal(loop_counts, counts))
+    print(f"[bench] per-frame loop {ms_loop:.2f} ms/trajectory; batch/loop RDF counts identical: {same}",
+          file=sys.stderr, flush=True)
+
+    kernels = {}
+    for name, (cnt, tot) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        kernels[name] = {"launches_per_step": cnt, "avg_ms": round(tot / cnt, 4), "share": round(tot / ms_eager, 4)}
+    dom = max(stats.items(), key=lambda kv: kv[1][1])[0]
+    cnt, tot = stats[dom]
+    flops = 11.0 * rdf_pairs
+    achieved = flops / (tot / cnt * 1e-3) / 1e12
+    out = {
+        "metric": "pairs/s (query+RDF) & particles/s (SolidLiquid), plain vs batched/learned",
+        "value": (bonds + rdf_pairs) / (ms_batch / 1e3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 2),
+        "ms_per_step": ms_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded liquid / jittered FCC / BCC frames)",
+        "config": {
+            "workload": "batched trajectory: 256 frames, N log-uniform in [10k, 200k], liquid (density 0.8-1.0) or "
+                        "jittered FCC/BCC (sigma 0.03-0.1a), cubic or triclinic (tilts in [-0.3, 0.3]); ball query "
+                        "r_max=1.5 + RDF 200 bins r_max=5 + Steinhardt q6, one batch per step, HIP-graph replay",
+            "n_frames": len(frames),
+            "n_points": total,
+            "ball_bonds": bonds,
+            "rdf_pairs": rdf_pairs,
+            "particles_per_s": total / (ms_batch / 1e3),
+            "batch_config": batch.config,
+            "ms_eager_batch": ms_eager,
+            "ms_per_frame_loop": ms_loop,
+            "speedup_vs_per_frame_loop": ms_loop / ms_batch,
+            "loop_counts_identical": same,
+        },
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": dom, "avg_ms": round(tot / cnt, 4)},
+        "kernels": kernels,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("frame", "batched"), default="frame",
+                    help="frame: one 1M-point frame (the headline line); batched: a 256-frame trajectory of uneven "
+                         "frames, batched + graph-replayed against the per-frame loop")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -170,6 +325,17 @@ def main():
     from freud_amd._lib import DeviceArray
 
     print(f"[bench] device: {_lib.device_name()}", file=sys.stderr, flush=True)
+    if args.workload == "batched":
+        return run_batched(args)
+    if args.copy_mib > 0:
+        nbytes = args.copy_mib << 20
+        src = DeviceArray((nbytes // 4,), np.uint32)
+        dst = DeviceArray((nbytes // 4,), np.uint32)
+        src.zero()
+        _lib.check(_lib.lib().fa_memcpy_d2d(dst.ptr, src.ptr, nbytes))
+        _lib.synchronize()
+        del src, dst
+        print(f"[bench] copied {args.copy_mib} MiB device to device", file=sys.stderr, flush=True)
     box, pts = make_workload()
     n = len(pts)
     d_pts = DeviceArray.from_numpy(pts)
@@ -215,13 +381,14 @@ def main():
         flops = 11.0 * b_rdf
         achieved = flops / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / FP32_VALU_PEAK_TFLOPS, 5), "traffic": measured_traffic(dom), "kernel": dom,
                 "avg_ms": round(avg_ms, 4)}
     else:
         ab = algorithmic_bytes(dom, n, b_ball, b_knn) or 0
         achieved = ab / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "avg_ms": round(avg_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom), "kernel": dom,
+                "avg_ms": round(avg_ms, 4)}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -7,7 +7,7 @@ runs in hand-written HIP kernels (``freud_amd/csrc``) behind the C-ABI declared
 in ``include/freud_amd.h``.
 """
 
-from freud_amd import box, cluster, data, density, locality, order, parallel
+from freud_amd import batch, box, cluster, data, density, locality, order, parallel
 from freud_amd.box import Box
 from freud_amd.locality import AABBQuery, LinkCell, NeighborList
 
@@ -16,6 +16,7 @@ __version__ = "0.1.0"
 __all__ = [
     "AABBQuery",
     "Box",
+    "batch",
     "LinkCell",
     "NeighborList",
     "box",

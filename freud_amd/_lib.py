@@ -56,6 +56,15 @@ class fa_query_args(ctypes.Structure):
     ]
 
 
+class fa_batch_plan(ctypes.Structure):
+    _fields_ = [
+        ("ball_r_max", ctypes.c_float),
+        ("rdf_bins", ctypes.c_uint32),
+        ("rdf_r_max", ctypes.c_float),
+        ("l", ctypes.c_uint32),
+    ]
+
+
 class fa_nlist(ctypes.Structure):
     _fields_ = [
         ("neighbors", ctypes.c_void_p),
@@ -87,6 +96,24 @@ _SIGS = {
     "fa_synchronize": (ctypes.c_int, []),
     "fa_timer_start": (ctypes.c_int, []),
     "fa_timer_stop": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float)]),
+    "fa_batch_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.POINTER(fa_batch_plan), ctypes.POINTER(ctypes.c_void_p)]),
+    "fa_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fa_batch_points": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "fa_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "fa_batch_num_bonds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fa_batch_nlist": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(fa_nlist)]),
+    "fa_batch_rdf_counts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "fa_batch_ql": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.c_void_p]),
+    "fa_batch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
+    "fa_tune_enable": (ctypes.c_int, [ctypes.c_int]),
+    "fa_tune_count": (ctypes.c_int, []),
+    "fa_tune_get": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]),
+    "fa_tune_set": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fa_tune_reset": (ctypes.c_int, []),
     "fa_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fa_profile_reset": (ctypes.c_int, []),
     "fa_profile_count": (ctypes.c_int, []),
@@ -199,6 +226,31 @@ def profile_stats():
         check(lib().fa_profile_get(i, buf, 256, ctypes.byref(cnt), ctypes.byref(ms)))
         out[buf.value.decode()] = (int(cnt.value), float(ms.value))
     return out
+
+
+def tune_enable(on=True):
+    """Switch the autotuner on or off (on by default)."""
+    check(lib().fa_tune_enable(int(bool(on))))
+
+
+def tune_table():
+    """The autotuner's table as {(analysis, key): variant}."""
+    out = {}
+    a, k, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    for i in range(lib().fa_tune_count()):
+        check(lib().fa_tune_get(i, ctypes.byref(a), ctypes.byref(k), ctypes.byref(v)))
+        out[(a.value, k.value)] = v.value
+    return out
+
+
+def tune_load(table):
+    """Preset the autotuner's table (the dict tune_table returns)."""
+    for (a, k), v in table.items():
+        check(lib().fa_tune_set(int(a), int(k), int(v)))
+
+
+def tune_reset():
+    check(lib().fa_tune_reset())
 
 
 def synchronize():

@@ -9,8 +9,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 
 #include "../../include/freud_amd.h"
@@ -54,10 +56,38 @@ void prof_end();
         FA_PROF_END();                             \
     } while (0)
 
+// Autotuner table (fa_tune.hip).  Analyses that the tuner covers:
+enum TunedAnalysis
+{
+    TUNE_RDF_SELF = 1, // variant = cells r_max / R wide, R = 2 + variant
+    TUNE_BALL = 2,     // variant = cells r_max / R wide, R = 1 + variant
+};
+bool tuning_enabled();
+int tuned_variant(int analysis, int key); // -1 when not tuned yet
+void record_variant(int analysis, int key, int variant);
+// Elapsed time of the work `launch` enqueues on the library stream (synchronises).
+int time_launch(const std::function<int()>& launch, float* ms);
+// Problem class of a query: expected points in an r^3 volume, in half-octaves.
+int tune_key(double points_per_volume, double r);
+
 // Device scratch that persists between calls (grown on demand, never shrunk).
 int scratch(size_t slot, size_t bytes, void** ptr);
 int device_alloc(void** p, size_t bytes);
 void device_free(void* p);
+
+// ------------------------------------------------------------------ launch shape
+// MI355X dispatches the workgroups of a grid round-robin over its 8 XCDs, each
+// with its own L2.  xcd_tile maps a workgroup id to a tile so that every XCD
+// works through one contiguous range of tiles: spatially sorted data then stays
+// within one L2 instead of being fetched by all eight.  A bijection on
+// [0, ntiles) for any ntiles.
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_tile(uint32_t bid, uint32_t ntiles)
+{
+    uint32_t const x = bid % kXcds, k = bid / kXcds;
+    uint32_t const per = ntiles / kXcds, rem = ntiles % kXcds;
+    return x * per + min(x, rem) + k;
+}
 
 // ------------------------------------------------------------------ box
 // Box with the 27 periodic image vectors precomputed on the host with the
@@ -377,6 +407,86 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
 // Exclusive scan of n uint32 counts into n+1 offsets (offsets[n] = total);
 // returns the 64-bit total through *total (host) when total != nullptr.
 int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total);
+// The same scan for code that may be captured into a HIP graph: no memset, no
+// host synchronisation; `tmp` holds scan_tmp_bytes(n) bytes.
+size_t scan_tmp_bytes(uint32_t n);
+int exclusive_scan_u32_async(const uint32_t* d_in, uint32_t* d_out, uint32_t n, void* tmp, size_t tmp_bytes);
+// Sets n uint32 words to zero with a kernel (graph-safe, unlike hipMemsetAsync).
+int zero_u32(uint32_t* d, uint64_t n);
+
+// ------------------------------------------------------------------ batches
+// A batch concatenates the points of many frames (any box shape, any size).
+// Each frame is cut into 256-point tiles that never straddle frames, so every
+// workgroup of a batched kernel works on one frame and reads that frame's
+// description with uniform (scalar) loads.
+constexpr uint32_t kTile = 256;
+
+struct BatchFrame
+{
+    DevBox box;
+    int dim[3];         // the frame's cell grid
+    uint32_t cell_base; // first cell of the frame in the batch's cell arrays
+    uint32_t row_base;  // first x-row (dim[1] * dim[2] rows per frame)
+    uint32_t pt_base;   // first point of the frame in the batch
+    uint32_t n;         // points in the frame
+    uint32_t tile_base; // first tile of the frame
+    int any_outside;    // set by the grid build when a point lies outside the box
+};
+
+// One cell grid over every frame of a batch (device arrays).
+struct BatchGrid
+{
+    BatchFrame* frames = nullptr; // n_frames
+    float4* cpos = nullptr;       // cell-ordered points, (x, y, z, bits(frame-local index))
+    int* wrapc = nullptr;
+    uint32_t* cstart = nullptr; // total_cells + 1
+    uint32_t* cell_of_point = nullptr;
+    uint32_t* fill = nullptr; // total_cells + 1 (counts, then scatter cursors)
+    uint32_t n_frames = 0, total_points = 0, total_cells = 0, total_rows = 0;
+};
+
+__device__ inline DevGrid frame_grid(const BatchFrame& f, const BatchGrid& G)
+{
+    DevGrid g;
+    g.cpos = G.cpos;
+    g.wrapc = G.wrapc;
+    g.cstart = G.cstart + f.cell_base;
+    g.dim[0] = f.dim[0];
+    g.dim[1] = f.dim[1];
+    g.dim[2] = f.dim[2];
+    g.any_outside = f.any_outside;
+    return g;
+}
+
+// Batched launches (all enqueue on the library stream without synchronising,
+// so a batch run can be captured into a HIP graph).
+int batch_grid_build(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, void* tmp,
+                     size_t tmp_bytes);
+int batch_ball_count(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, float r_max, int R,
+                     uint32_t* d_counts);
+// Writes nothing when *d_total (the 64-bit bond total) exceeds capacity.
+int batch_ball_fill(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, float r_max,
+                    int R, const uint32_t* d_offsets, const unsigned long long* d_total, uint64_t capacity,
+                    uint32_t* nb, float* dist, float* wts, float* vec, void* gkeys);
+struct BatchRdfScratch
+{
+    uint32_t* nclus;  // total_rows + 1
+    uint32_t* cl_off; // total_rows + 1
+    uint4* table;     // max_clusters
+    uint32_t* queue;  // 1 work-queue counter
+    const float* T;   // bins + 1 r_sq thresholds (thresholds_for)
+    void* scan_tmp;
+    size_t scan_bytes;
+    uint64_t max_clusters;
+};
+int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const BatchRdfScratch& S, uint32_t* d_counts);
+// The bins + 1 r_sq thresholds of RegularAxis(bins, 0, r_max) binning of
+// sqrt(r_sq), computed on the host and copied (synchronously) to d_T.
+int thresholds_for(uint32_t bins, float r_max, float* d_T);
+int batch_steinhardt(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts,
+                     const uint32_t* nb, const float* dist, const uint32_t* d_offsets,
+                     const unsigned long long* d_total, uint64_t capacity, uint32_t l, float* d_qlm, float* d_ql,
+                     float* d_partial, float* d_order);
 int sort_u64(uint64_t* d_keys, uint32_t n);
 
 } // namespace fa

@@ -40,9 +40,75 @@ __global__ void grid_scatter_kernel(fa::DevBox b, const float* __restrict__ pts,
     wrapc[slot] = fa::pack_wrap(w);
 }
 
+// ---------------------------------------------------------------- batches
+// Per tile (one frame per workgroup): the global cell of each point and the
+// cell counts; flags the frame when a point lies outside its box.
+__global__ __launch_bounds__(256) void bgrid_assign_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                           const float* __restrict__ pts)
+{
+    uint32_t const t = blockIdx.x;
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const local = (t - F.tile_base) * fa::kTile + threadIdx.x;
+    if (local >= F.n)
+        return;
+    uint32_t const i = F.pt_base + local;
+    int c[3], w[3];
+    fa::cell_of(F.box, F.dim, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], c, w);
+    uint32_t const cell = F.cell_base + uint32_t((c[2] * F.dim[1] + c[1]) * F.dim[0] + c[0]);
+    G.cell_of_point[i] = cell;
+    if (w[0] != 0 || w[1] != 0 || w[2] != 0)
+        atomicOr(&G.frames[tile_frame[t]].any_outside, 1);
+    atomicAdd(&G.fill[cell], 1u);
+}
+
+__global__ __launch_bounds__(256) void bgrid_scatter_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                            const float* __restrict__ pts)
+{
+    uint32_t const t = blockIdx.x;
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const local = (t - F.tile_base) * fa::kTile + threadIdx.x;
+    if (local >= F.n)
+        return;
+    uint32_t const i = F.pt_base + local;
+    uint32_t const cell = G.cell_of_point[i];
+    uint32_t const slot = G.cstart[cell] + atomicAdd(&G.fill[cell], 1u);
+    float const x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    G.cpos[slot] = make_float4(x, y, z, __uint_as_float(local));
+    int c[3], w[3];
+    fa::cell_of(F.box, F.dim, x, y, z, c, w);
+    G.wrapc[slot] = fa::pack_wrap(w);
+}
+
+__global__ void bgrid_reset_kernel(fa::BatchGrid G)
+{
+    uint32_t const f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < G.n_frames)
+        G.frames[f].any_outside = 0;
+}
+
 } // namespace
 
 namespace fa {
+
+int batch_grid_build(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, void* tmp,
+                     size_t tmp_bytes)
+{
+    hipStream_t s = stream();
+    FA_KLAUNCH(bgrid_reset_kernel, dim3((G.n_frames + 255) / 256), dim3(256), 0, s, G);
+    if (int rc = zero_u32(G.fill, G.total_cells))
+        return rc;
+    if (ntiles)
+        FA_KLAUNCH(bgrid_assign_kernel, dim3(ntiles), dim3(kTile), 0, s, G, d_tile_frame, d_pts);
+    if (int rc = exclusive_scan_u32_async(G.fill, G.cstart, G.total_cells, tmp, tmp_bytes))
+        return rc;
+    if (int rc = zero_u32(G.fill, G.total_cells))
+        return rc;
+    if (ntiles)
+        FA_KLAUNCH(bgrid_scatter_kernel, dim3(ntiles), dim3(kTile), 0, s, G, d_tile_frame, d_pts);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
 
 Grid::~Grid()
 {

@@ -134,65 +134,57 @@ template<int L> __device__ inline void accumulate_ylm(float ct, float st, float 
 // q_l(i) = sqrt(4 pi / (2l+1) sum_m |q_lm|^2); also the block partial sums of
 // q_lm(i) / N for the system-wide order.
 template<int L>
-__global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
-                                                     const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
-                                                     const float* __restrict__ wts, const uint32_t* __restrict__ offsets,
-                                                     int weighted, int want_sys, float2* __restrict__ qlm,
-                                                     float* __restrict__ ql, int ql_stride, int ql_col,
-                                                     float2* __restrict__ sys_partial)
+__device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i,
+                                  const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
+                                  const float* __restrict__ wts, const uint32_t* __restrict__ offsets, int weighted,
+                                  float* q)
 {
     constexpr int NM = 2 * L + 1;
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     float2 acc[L + 1];
 #pragma unroll
     for (int m = 0; m <= L; ++m)
         acc[m] = make_float2(0.0f, 0.0f);
-    float q[2 * NM];
-    if (i < n)
+    float total_w = 0.0f;
+    uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+    for (uint32_t e = e0; e < e1; ++e)
     {
-        float total_w = 0.0f;
-        uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
-        for (uint32_t e = e0; e < e1; ++e)
-        {
-            uint32_t const j = nbr[2 * e + 1];
-            float const w = weighted ? wts[e] : 1.0f;
-            float ct, st, cp, sp;
-            bond_angles(b, pts, i, j, dist[e], ct, st, cp, sp);
-            accumulate_ylm<L>(ct, st, cp, sp, w, acc);
-            total_w += w;
-        }
-        float norm = 0.0f;
-#pragma unroll
-        for (int m = 0; m <= L; ++m)
-        {
-            float const re = acc[m].x / total_w, im = acc[m].y / total_w;
-            q[2 * m] = re;
-            q[2 * m + 1] = im;
-            if (m > 0)
-            {
-                float const sgn = (m & 1) ? -1.0f : 1.0f;
-                q[2 * (L + m)] = sgn * re;
-                q[2 * (L + m) + 1] = -sgn * im;
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < NM; ++t)
-            norm += q[2 * t] * q[2 * t] + q[2 * t + 1] * q[2 * t + 1];
-        float2* out = qlm + uint64_t(i) * NM;
-#pragma unroll
-        for (int t = 0; t < NM; ++t)
-            out[t] = make_float2(q[2 * t], q[2 * t + 1]);
-        float const nf = float(4.0 * M_PI / NM);
-        ql[uint64_t(i) * ql_stride + ql_col] = sqrtf(norm * nf);
+        uint32_t const j = nbr[2 * e + 1];
+        float const w = weighted ? wts[e] : 1.0f;
+        float ct, st, cp, sp;
+        bond_angles(b, pts, i, j, dist[e], ct, st, cp, sp);
+        accumulate_ylm<L>(ct, st, cp, sp, w, acc);
+        total_w += w;
     }
-    if (!want_sys)
-        return;
-    // block partial sum of q_lm / N (system q_lm, Steinhardt.cc:213-216)
+    float norm = 0.0f;
+#pragma unroll
+    for (int m = 0; m <= L; ++m)
+    {
+        float const re = acc[m].x / total_w, im = acc[m].y / total_w;
+        q[2 * m] = re;
+        q[2 * m + 1] = im;
+        if (m > 0)
+        {
+            float const sgn = (m & 1) ? -1.0f : 1.0f;
+            q[2 * (L + m)] = sgn * re;
+            q[2 * (L + m) + 1] = -sgn * im;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NM; ++t)
+        norm += q[2 * t] * q[2 * t] + q[2 * t + 1] * q[2 * t + 1];
+    float const nf = float(4.0 * M_PI / NM);
+    return sqrtf(norm * nf);
+}
+
+// Block partial sum of q_lm(i) * invn over the block's valid points (system
+// q_lm, Steinhardt.cc:213-216), written to out[0..NM).
+template<int L> __device__ inline void block_qlm_partial(bool valid, const float* q, float invn, float2* out)
+{
+    constexpr int NM = 2 * L + 1;
     __shared__ float2 s_red[kBlock];
-    float const invn = 1.0f / float(n);
     for (int t = 0; t < NM; ++t)
     {
-        s_red[threadIdx.x] = (i < n) ? make_float2(q[2 * t] * invn, q[2 * t + 1] * invn) : make_float2(0.0f, 0.0f);
+        s_red[threadIdx.x] = valid ? make_float2(q[2 * t] * invn, q[2 * t + 1] * invn) : make_float2(0.0f, 0.0f);
         __syncthreads();
         for (int s = kBlock / 2; s > 0; s >>= 1)
         {
@@ -204,9 +196,107 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
             __syncthreads();
         }
         if (threadIdx.x == 0)
-            sys_partial[uint64_t(blockIdx.x) * NM + t] = s_red[0];
+            out[t] = s_red[0];
         __syncthreads();
     }
+}
+
+template<int L>
+__global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
+                                                     const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
+                                                     const float* __restrict__ wts, const uint32_t* __restrict__ offsets,
+                                                     int weighted, int want_sys, float2* __restrict__ qlm,
+                                                     float* __restrict__ ql, int ql_stride, int ql_col,
+                                                     float2* __restrict__ sys_partial)
+{
+    constexpr int NM = 2 * L + 1;
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    float q[2 * NM];
+    if (i < n)
+    {
+        float const v = qlm_point<L>(b, pts, i, nbr, dist, wts, offsets, weighted, q);
+        float2* out = qlm + uint64_t(i) * NM;
+#pragma unroll
+        for (int t = 0; t < NM; ++t)
+            out[t] = make_float2(q[2 * t], q[2 * t + 1]);
+        ql[uint64_t(i) * ql_stride + ql_col] = v;
+    }
+    if (want_sys)
+        block_qlm_partial<L>(i < n, q, 1.0f / float(n), sys_partial + uint64_t(blockIdx.x) * NM);
+}
+
+// Batched q_l: one workgroup per 256-point tile of one frame; bonds hold
+// frame-local indices, offsets are batch-indexed; per-tile partial sums of
+// q_lm / N_frame.
+template<int L>
+__global__ __launch_bounds__(kBlock) void bqlm_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                      uint32_t ntiles, const float* __restrict__ pts,
+                                                      const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
+                                                      const uint32_t* __restrict__ offsets,
+                                                      const unsigned long long* __restrict__ total_bonds,
+                                                      uint64_t capacity, float2* __restrict__ qlm,
+                                                      float* __restrict__ ql, float2* __restrict__ partial)
+{
+    constexpr int NM = 2 * L + 1;
+    // bonds beyond the allocation were not written (the host reruns)
+    if (*total_bonds > capacity)
+        return;
+    uint32_t const t = fa::xcd_tile(blockIdx.x, ntiles);
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const i = (t - F.tile_base) * fa::kTile + threadIdx.x;
+    bool const valid = i < F.n;
+    float q[2 * NM];
+    if (valid)
+    {
+        float const v = qlm_point<L>(F.box, pts + 3 * size_t(F.pt_base), i, nbr, dist, nullptr, offsets + F.pt_base, 0,
+                                     q);
+        float2* out = qlm + uint64_t(F.pt_base + i) * NM;
+#pragma unroll
+        for (int m = 0; m < NM; ++m)
+            out[m] = make_float2(q[2 * m], q[2 * m + 1]);
+        ql[F.pt_base + i] = v;
+    }
+    block_qlm_partial<L>(valid, q, 1.0f / float(F.n), partial + uint64_t(t) * NM);
+}
+
+// Per frame: system q_lm from its tiles' partials, then the frame's order
+// sqrt(4 pi / (2l+1) sum_m |Q_lm|^2) (Steinhardt.cc:291-359 with the
+// reference's float operations).
+__global__ __launch_bounds__(kBlock) void bframe_order_kernel(fa::BatchGrid G, int nm, const float2* __restrict__ part,
+                                                              float* __restrict__ order)
+{
+    __shared__ float2 s_red[kBlock];
+    __shared__ float s_norm;
+    const fa::BatchFrame& F = G.frames[blockIdx.x];
+    uint32_t const ntl = (F.n + fa::kTile - 1) / fa::kTile;
+    if (threadIdx.x == 0)
+        s_norm = 0.0f;
+    for (int t = 0; t < nm; ++t)
+    {
+        float2 acc = make_float2(0.0f, 0.0f);
+        for (uint32_t bl = threadIdx.x; bl < ntl; bl += blockDim.x)
+        {
+            float2 const v = part[uint64_t(F.tile_base + bl) * nm + t];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        s_red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1)
+        {
+            if (threadIdx.x < s)
+            {
+                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
+                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            s_norm += s_red[0].x * s_red[0].x + s_red[0].y * s_red[0].y;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        order[blockIdx.x] = sqrtf(s_norm * float(4.0 * M_PI / nm));
 }
 
 // Steinhardt.cc:224-289: qbar_lm(i) = (sum_j q_lm(j) + q_lm(i)) / (n_b + 1)
@@ -218,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
                                                          float2* __restrict__ sys_partial)
 {
     constexpr int NM = 2 * L + 1;
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     float2 acc[NM];
 #pragma unroll
     for (int t = 0; t < NM; ++t)
@@ -395,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, 
                                                              float q_threshold, float* __restrict__ ql_ij,
                                                              uint32_t* __restrict__ num_conn)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     int const nm = 2 * l + 1;
@@ -766,6 +856,46 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
     }
     return FA_OK;
 }
+template<int L>
+int launch_batch_qlm(uint32_t l, const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts,
+                     const uint32_t* nb, const float* dist, const uint32_t* d_offsets,
+                     const unsigned long long* d_total, uint64_t capacity, float* d_qlm, float* d_ql, float* d_partial)
+{
+    if (uint32_t(L) == l)
+    {
+        FA_KLAUNCH(bqlm_kernel<L>, dim3(ntiles), dim3(kBlock), 0, stream(), G, d_tile_frame, ntiles, d_pts, nb, dist,
+                   d_offsets, d_total, capacity, reinterpret_cast<float2*>(d_qlm), d_ql,
+                   reinterpret_cast<float2*>(d_partial));
+        FA_HIP(hipGetLastError());
+        return FA_OK;
+    }
+    if constexpr (L < kLMax)
+        return launch_batch_qlm<L + 1>(l, G, d_tile_frame, ntiles, d_pts, nb, dist, d_offsets, d_total, capacity,
+                                       d_qlm, d_ql, d_partial);
+    return fail(FA_ERR_INVALID, "freud_amd supports l <= 20");
+}
+
+// Batched Steinhardt q_l over the batch's ball-query bonds: d_qlm (total x
+// (2l+1) complex), d_ql (total), d_partial (ntiles x (2l+1) complex scratch),
+// d_order (n_frames, device).
+int batch_steinhardt(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts,
+                     const uint32_t* nb, const float* dist, const uint32_t* d_offsets,
+                     const unsigned long long* d_total, uint64_t capacity, uint32_t l, float* d_qlm, float* d_ql,
+                     float* d_partial, float* d_order)
+{
+    if (int rc = ensure_ylm_coef())
+        return rc;
+    if (ntiles == 0)
+        return FA_OK;
+    if (int rc = launch_batch_qlm<0>(l, G, d_tile_frame, ntiles, d_pts, nb, dist, d_offsets, d_total, capacity,
+                                     d_qlm, d_ql, d_partial))
+        return rc;
+    FA_KLAUNCH(bframe_order_kernel, dim3(G.n_frames), dim3(kBlock), 0, stream(), G, int(2 * l + 1),
+               reinterpret_cast<const float2*>(d_partial), d_order);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
 } // namespace fa
 
 extern "C" {

@@ -1,5 +1,6 @@
 // fa_prim.hip -- scan and sort primitives (rocPRIM device algorithms) used to
 // turn per-query counts into NeighborList offsets and to order cluster labels.
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -45,6 +46,47 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
         FA_HIP(hipMemcpyAsync(total, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
         FA_HIP(hipStreamSynchronize(s));
     }
+    return FA_OK;
+}
+
+namespace {
+__global__ void zero_u32_kernel(uint32_t* __restrict__ d, uint64_t n)
+{
+    for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += uint64_t(gridDim.x) * blockDim.x)
+        d[t] = 0u;
+}
+} // namespace
+
+int zero_u32(uint32_t* d, uint64_t n)
+{
+    if (n == 0)
+        return FA_OK;
+    uint64_t const blocks = std::min<uint64_t>((n + 255) / 256, 4096);
+    FA_KLAUNCH(zero_u32_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream(), d, n);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+size_t scan_tmp_bytes(uint32_t n)
+{
+    size_t bytes = 0;
+    if (n == 0)
+        return 256;
+    uint32_t* dummy = nullptr;
+    if (rocprim::inclusive_scan(nullptr, bytes, dummy, dummy, size_t(n), rocprim::plus<uint32_t>(), stream()) !=
+        hipSuccess)
+        return 0;
+    return bytes + 256;
+}
+
+int exclusive_scan_u32_async(const uint32_t* d_in, uint32_t* d_out, uint32_t n, void* tmp, size_t tmp_bytes)
+{
+    if (int rc = zero_u32(d_out, 1))
+        return rc;
+    if (n == 0)
+        return FA_OK;
+    size_t bytes = tmp_bytes;
+    FA_HIP(rocprim::inclusive_scan(tmp, bytes, d_in, d_out + 1, size_t(n), rocprim::plus<uint32_t>(), stream()));
     return FA_OK;
 }
 

@@ -76,14 +76,13 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uin
     return total;
 }
 
+// Count of query position p (p < pend).  In SELF mode positions index the
+// grid's cell-ordered points and the point index comes from the grid.
 template<int FORMULA, bool SELF>
-__global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                            uint32_t nq, float r_max_sq, float r_min_sq,
-                                                            int exclude_ii, int R, uint32_t* __restrict__ counts)
+__device__ inline void ball_count_body(const DevBox& b, const DevGrid& g, const float* __restrict__ qpts, uint32_t p,
+                                       float r_max_sq, float r_min_sq, int exclude_ii, int R,
+                                       uint32_t* __restrict__ counts)
 {
-    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nq)
-        return;
     float qx, qy, qz;
     uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
     uint32_t c = 0;
@@ -95,6 +94,16 @@ __global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g,
                                             ++c;
                                     });
     counts[i] = c;
+}
+
+template<int FORMULA, bool SELF>
+__global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                            uint32_t nq, float r_max_sq, float r_min_sq,
+                                                            int exclude_ii, int R, uint32_t* __restrict__ counts)
+{
+    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p < nq)
+        ball_count_body<FORMULA, SELF>(b, g, qpts, p, r_max_sq, r_min_sq, exclude_ii, R, counts);
 }
 
 template<typename K> __device__ inline void insertion_sort(K* a, uint32_t n)
@@ -160,14 +169,16 @@ __device__ inline void store_block_bonds(const DevBox& b, const float* __restric
     }
 }
 
+// One block's share of the fill pass: query positions [p0, p0 + nloc).  Point
+// indices i are relative to `offsets`, `qpts` and `pts` (a frame of a batch
+// passes pointers advanced to its first point); bond indices are absolute.
 template<int FORMULA, bool BYDIST, bool SELF>
-__global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                           uint32_t nq, const float* __restrict__ pts, float r_max_sq,
-                                                           float r_min_sq, int exclude_ii, int R,
-                                                           const uint32_t* __restrict__ offsets,
-                                                           uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
-                                                           float* __restrict__ w_out, float* __restrict__ v_out,
-                                                           void* __restrict__ gkeys_raw)
+__device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const float* __restrict__ qpts, uint32_t p0,
+                                      uint32_t nloc, const float* __restrict__ pts, float r_max_sq, float r_min_sq,
+                                      int exclude_ii, int R, const uint32_t* __restrict__ offsets,
+                                      uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                      float* __restrict__ w_out, float* __restrict__ v_out,
+                                      void* __restrict__ gkeys_raw)
 {
     using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
     constexpr uint32_t CAP = kKeyBytes / sizeof(K);
@@ -176,8 +187,6 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
     uint32_t const tid = threadIdx.x;
-    uint32_t const p0 = blockIdx.x * kBlock;
-    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
     float qx = 0.0f, qy = 0.0f, qz = 0.0f;
     uint32_t i = 0, cnt = 0;
     if (tid < nloc)
@@ -225,6 +234,20 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
     if (in_lds)
         store_block_bonds<FORMULA, BYDIST>(b, qpts, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
                                            w_out, v_out);
+}
+
+template<int FORMULA, bool BYDIST, bool SELF>
+__global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, const float* __restrict__ pts, float r_max_sq,
+                                                           float r_min_sq, int exclude_ii, int R,
+                                                           const uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                                           float* __restrict__ w_out, float* __restrict__ v_out,
+                                                           void* __restrict__ gkeys_raw)
+{
+    uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
+    ball_fill_body<FORMULA, BYDIST, SELF>(b, g, qpts, p0, min(uint32_t(kBlock), nq - p0), pts, r_max_sq, r_min_sq,
+                                          exclude_ii, R, offsets, nb_out, d_out, w_out, v_out, gkeys_raw);
 }
 
 // Segments with the reference's convention (0 for query points without bonds).
@@ -284,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
                                                           uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
                                                           uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag)
 {
-    uint32_t const p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= nq)
         return;
     float qx, qy, qz;
@@ -481,7 +504,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
     uint32_t const tid = threadIdx.x;
-    uint32_t const p0 = blockIdx.x * kBlock;
+    uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
     uint32_t i = 0, n = 0;
     if (tid < nloc)
@@ -552,9 +575,73 @@ __global__ void all_pairs_kernel(DevBox b, const float* __restrict__ pts, uint32
     v_out[3 * idx + 2] = rz;
 }
 
+// ------------------------------------------------------------------ batches
+// Ball query over every frame of a batch (query points = the frame's points,
+// AABB formula, exclude_ii): one workgroup per 256-point tile of one frame,
+// positions in the frame's cell order.  Counts and offsets are indexed by
+// batch point index; bonds hold frame-local (i, j).
+__global__ __launch_bounds__(kBlock) void bball_count_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                             uint32_t ntiles, float r_max_sq, int R,
+                                                             uint32_t* __restrict__ counts)
+{
+    uint32_t const t = fa::xcd_tile(blockIdx.x, ntiles);
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const local = (t - F.tile_base) * fa::kTile + threadIdx.x;
+    if (local >= F.n)
+        return;
+    DevGrid const g = fa::frame_grid(F, G);
+    ball_count_body<AABB_FORMULA, true>(F.box, g, nullptr, F.pt_base + local, r_max_sq, 0.0f, 1, R,
+                                        counts + F.pt_base);
+}
+
+__global__ __launch_bounds__(kBlock) void bball_fill_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                            uint32_t ntiles, const float* __restrict__ pts,
+                                                            float r_max_sq, int R,
+                                                            const uint32_t* __restrict__ offsets,
+                                                            const unsigned long long* __restrict__ total_bonds,
+                                                            uint64_t capacity, uint32_t* __restrict__ nb_out,
+                                                            float* __restrict__ d_out, float* __restrict__ w_out,
+                                                            float* __restrict__ v_out, void* __restrict__ gkeys)
+{
+    // a run whose bonds exceed the allocation writes nothing (the host sees
+    // the 64-bit total and runs again with room for it)
+    if (*total_bonds > capacity)
+        return;
+    uint32_t const t = fa::xcd_tile(blockIdx.x, ntiles);
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const local0 = (t - F.tile_base) * fa::kTile;
+    DevGrid const g = fa::frame_grid(F, G);
+    const float* fp = pts + 3 * size_t(F.pt_base);
+    ball_fill_body<AABB_FORMULA, false, true>(F.box, g, fp, F.pt_base + local0, min(fa::kTile, F.n - local0), fp,
+                                              r_max_sq, 0.0f, 1, R, offsets + F.pt_base, nb_out, d_out, w_out, v_out,
+                                              gkeys);
+}
+
 } // namespace
 
 namespace fa {
+
+int batch_ball_count(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, float r_max, int R,
+                     uint32_t* d_counts)
+{
+    if (ntiles)
+        FA_KLAUNCH(bball_count_kernel, dim3(ntiles), dim3(kBlock), 0, stream(), G, d_tile_frame, ntiles,
+                   r_max * r_max, R, d_counts);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int batch_ball_fill(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, float r_max,
+                    int R, const uint32_t* d_offsets, const unsigned long long* d_total, uint64_t capacity,
+                    uint32_t* nb, float* dist, float* wts, float* vec, void* gkeys)
+{
+    if (ntiles)
+        FA_KLAUNCH(bball_fill_kernel, dim3(ntiles), dim3(kBlock), 0, stream(), G, d_tile_frame, ntiles, d_pts,
+                   r_max * r_max, R, d_offsets, d_total, capacity, nb, dist, wts, vec, gkeys);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
 
 int nlist_reserve(fa_nlist* nl, uint64_t nb, uint32_t nq)
 {

@@ -117,12 +117,18 @@ __device__ inline uint32_t bin_of_rsq(float x, const float* __restrict__ T, uint
     float const d = __builtin_amdgcn_sqrtf(x);
     float v = (d - vmin) * inv_width;
     int k = int(fminf(fmaxf(v, 0.0f), float(nbins - 1)));
-#pragma unroll
-    for (int it = 0; it < 2; ++it)
+    // the estimate is off by at most one bin; the loops are exact for any k
+    float const t0 = T[k], t1 = T[k + 1];
+    if (x < t0)
     {
-        if (k > 0 && x < T[k])
+        --k;
+        while (k > 0 && x < T[k])
             --k;
-        if (k < int(nbins) - 1 && x >= T[k + 1])
+    }
+    else if (x >= t1)
+    {
+        ++k;
+        while (k < int(nbins) - 1 && x >= T[k + 1])
             ++k;
     }
     return uint32_t(k);
@@ -150,119 +156,193 @@ template<bool LDS_HIST> __device__ inline void hist_epilogue(const uint32_t* his
             atomicAdd(&counts[t], hist[t]);
 }
 
-// ------------------------------------------------------------------ self, cluster pairs
-// Clusters: runs of up to 8 consecutive cell-ordered points of one cell.
-__global__ void cluster_count_kernel(const uint32_t* __restrict__ cstart, uint32_t ncell, uint32_t* __restrict__ nclus)
+// ------------------------------------------------------------------ self, row clusters
+// A row is the run of cells (cx = 0..dim0-1) at fixed (cy, cz); its points are
+// contiguous in cell order.  Clusters are runs of up to 8 consecutive points of
+// one row, so every wave of the pair kernel is full except at row ends.
+// Cluster entry: (row, start, count, cx_lo | cx_hi << 16), cx_lo/cx_hi the
+// cells of its first and last point.
+__global__ void row_cluster_count_kernel(const uint32_t* __restrict__ cstart, uint32_t dim0, uint32_t nrows,
+                                         uint32_t* __restrict__ nclus)
 {
-    uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell)
-        nclus[c] = (cstart[c + 1] - cstart[c] + 7u) / 8u;
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows)
+        nclus[r] = (cstart[(r + 1) * dim0] - cstart[r * dim0] + 7u) / 8u;
 }
 
-__global__ void cluster_table_kernel(const uint32_t* __restrict__ cstart, uint32_t ncell,
-                                     const uint32_t* __restrict__ cl_off, uint2* __restrict__ table)
+// One thread per cluster: the row by binary search over the row offsets, the
+// cells of the first and last point by binary search over the row's cstart.
+__global__ void row_cluster_table_kernel(const uint32_t* __restrict__ cstart, uint32_t dim0, uint32_t nrows,
+                                         const uint32_t* __restrict__ cl_off, uint32_t max_clusters,
+                                         uint4* __restrict__ table)
 {
     uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell)
+    if (c >= max_clusters || c >= cl_off[nrows])
         return;
-    uint32_t const beg = cstart[c], n = cstart[c + 1] - beg;
-    uint32_t const o = cl_off[c];
-    for (uint32_t k = 0; k * 8 < n; ++k)
-        table[o + k] = make_uint2(c, ((beg + 8 * k) << 4) | min(8u, n - 8 * k));
+    uint32_t lo = 0, hi = nrows; // last row with cl_off[row] <= c
+    while (hi - lo > 1)
+    {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (cl_off[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    uint32_t const r = lo;
+    const uint32_t* cs = cstart + size_t(r) * dim0;
+    uint32_t const p = cs[0] + 8u * (c - cl_off[r]);
+    uint32_t const last = min(p + 8u, cs[dim0]) - 1u;
+    auto cell_of_pos = [&](uint32_t q) { // last cell with cs[cell] <= q
+        uint32_t l = 0, h = dim0;
+        while (h - l > 1)
+        {
+            uint32_t const m = (l + h) >> 1;
+            if (cs[m] <= q)
+                l = m;
+            else
+                h = m;
+        }
+        return l;
+    };
+    table[c] = make_uint4(r, p, last - p + 1u, cell_of_pos(p) | (cell_of_pos(last) << 16));
 }
 
-// One wave per cluster I of cell C; lane = (a, b) = (lane / 8, lane % 8):
-// query point a of I against point b of each 8-point chunk of every stencil
-// cell (cells at least r_max / R wide).  Non-wrapping cell pairs are taken
-// once (lexicographically positive offset, or a < b inside C) with weight 2.
-template<int R, bool LDS_HIST>
-__global__ __launch_bounds__(kBlock) void rdf_cluster_kernel(DevBox b, DevGrid g, const uint2* __restrict__ table,
-                                                             const uint32_t* __restrict__ ncl_ptr, float q_rmax_sq,
-                                                             float q_rmin_sq, const float* __restrict__ T,
-                                                             uint32_t nbins, float vmin, float inv_width,
-                                                             uint32_t* __restrict__ counts)
+// One wave per cluster; lane = (a, b) = (lane / 8, lane % 8): query point a of
+// the cluster against point b of each 8-point chunk of the stencil rows.  The
+// stencil spans cells [cx_lo - R, cx_hi + R] x [cy - R, cy + R] x [cz - R, cz + R]
+// (cells at least r_max / R wide), walked as contiguous pieces of a row that
+// share one periodic shift.  A pair related without a periodic shift is found
+// from both of its points with bitwise-equal r_sq (x - y == -(y - x)), so it
+// is counted once, from the point earlier in cell order, with weight 2; pairs
+// across the box edge are counted from both sides with weight 1, as the
+// reference's (q + image) arithmetic rounds differently for the two directions.
+//
+// Latency hiding: lane t first resolves stencil row t (its up to 3 periodic
+// pieces and their cell-range bounds), so the cstart reads of all rows are in
+// flight together; the wave then walks the rows, broadcasting each row's
+// bounds with readlane, and loads the next chunk before binning the current.
+//
+// The pair work of one cluster entry e = (frame-local row, start, count,
+// cx_lo | cx_hi << 16) for one wave, binned into h (LDS).
+template<int R>
+__device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e, float q_rmax_sq, float q_rmin_sq,
+                                     const float* sT, uint32_t nbins, float vmin, float inv_width, uint32_t* h)
 {
-    extern __shared__ float smem[];
-    float* sT = smem;
-    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + nbins + 1);
-    uint32_t* h = LDS_HIST ? hist : counts;
-    hist_prologue<LDS_HIST>(hist, sT, T, nbins);
-
+    constexpr int kRowsY = 2 * R + 1;
     uint32_t const lane = threadIdx.x & 63u;
-    uint32_t const I = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-    if (I < *ncl_ptr)
+    int const dim0 = g.dim[0], dim1 = g.dim[1], dim2 = g.dim[2];
+    uint32_t const A0 = e.y, na = e.z;
+    int const cx_lo = int(e.w & 0xffffu), cx_hi = int(e.w >> 16);
+    int const cy0 = int(e.x % uint32_t(dim1));
+    int const cz0 = int(e.x / uint32_t(dim1));
+    uint32_t const a = lane >> 3, bl = lane & 7u;
+    bool const a_ok = a < na;
+    uint32_t const pos_a = A0 + (a_ok ? a : 0u);
+    float4 pa = g.cpos[pos_a];
+    if (b.is2D)
+        pa.z = 0.0f;
+    int const Rz = b.is2D ? 0 : R;
+    int const nrows = kRowsY * (2 * Rz + 1);
+    for (int r0 = 0; r0 < nrows; r0 += 64)
     {
-        int const dim0 = g.dim[0], dim1 = g.dim[1], dim2 = g.dim[2];
-        uint2 const e = table[I];
-        uint32_t const C = e.x;
-        uint32_t const ia0 = e.y >> 4, na = e.y & 15u;
-        int const cx0 = int(C % uint32_t(dim0));
-        int const cy0 = int((C / uint32_t(dim0)) % uint32_t(dim1));
-        int const cz0 = int(C / (uint32_t(dim0) * uint32_t(dim1)));
-        uint32_t const a = lane >> 3, bl = lane & 7u;
-        bool const a_ok = a < na;
-        float4 pa = g.cpos[ia0 + (a_ok ? a : 0u)];
-        if (b.is2D)
-            pa.z = 0.0f;
-        uint32_t const pos_a = ia0 + a;
-        int const Rz = b.is2D ? 0 : R;
-        for (int dz = -Rz; dz <= Rz; ++dz)
+        // lane t: stencil row r0 + t -> pieces (beg, end, image code); code < 0: no piece
+        uint32_t pb[3] = {0u, 0u, 0u}, pe[3] = {0u, 0u, 0u};
+        int pc[3] = {-1, -1, -1};
+        int const rt = r0 + int(lane);
+        if (rt < nrows)
         {
-            int const uz = cz0 + dz;
-            if (uz < -dim2 || uz >= 2 * dim2)
-                continue;
-            int const sz = uz < 0 ? -1 : (uz >= dim2 ? 1 : 0);
-            int const cz = uz - sz * dim2;
-            for (int dy = -R; dy <= R; ++dy)
+            int const dz = rt / kRowsY - Rz, dy = rt % kRowsY - R;
+            int const uz = cz0 + dz, uy = cy0 + dy;
+            int const sz = fa::floor_div(uz, dim2), sy = fa::floor_div(uy, dim1);
+            if (sz >= -1 && sz <= 1 && sy >= -1 && sy <= 1)
             {
-                int const uy = cy0 + dy;
-                if (uy < -dim1 || uy >= 2 * dim1)
-                    continue;
-                int const sy = uy < 0 ? -1 : (uy >= dim1 ? 1 : 0);
-                int const cy = uy - sy * dim1;
-                for (int dx = -R; dx <= R; ++dx)
+                uint32_t const rowbase = uint32_t((uz - sz * dim2) * dim1 + (uy - sy * dim1)) * uint32_t(dim0);
+                // shifts beyond one box length are not images the reference visits,
+                // so the clipped range has at most 3 pieces (shift -1, 0, 1)
+                int ux = max(cx_lo - R, -dim0);
+                int const uxend = min(cx_hi + R, 2 * dim0 - 1);
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
                 {
-                    int const ux = cx0 + dx;
-                    if (ux < -dim0 || ux >= 2 * dim0)
-                        continue;
-                    int const sx = ux < 0 ? -1 : (ux >= dim0 ? 1 : 0);
-                    bool const wraps = (sx | sy | sz) != 0;
-                    bool const self = !wraps && dx == 0 && dy == 0 && dz == 0;
-                    if (!wraps && !self && !(dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0)))))
-                        continue;
-                    uint32_t const weight = wraps ? 1u : 2u;
-                    int const cx = ux - sx * dim0;
-                    uint32_t const B = uint32_t((cz * dim1 + cy) * dim0 + cx);
-                    uint32_t const b_beg = g.cstart[B];
-                    uint32_t const b_end = g.cstart[B + 1];
-                    int const code = (-sx + 1) * 9 + (-sy + 1) * 3 + (-sz + 1);
-                    // AABB formula: r = p_j - (q_i + image)
-                    float const qx = pa.x + b.img[code][0];
-                    float const qy = pa.y + b.img[code][1];
-                    float const qz = pa.z + b.img[code][2];
-                    for (uint32_t jb = self ? ia0 : b_beg; jb < b_end; jb += 8)
+                    if (ux <= uxend)
                     {
-                        uint32_t const pos_b = jb + bl;
-                        bool valid = a_ok && pos_b < b_end;
-                        if (self)
-                            valid = valid && pos_a < pos_b;
-                        float4 pj = g.cpos[valid ? pos_b : b_beg];
-                        float const pjz = b.is2D ? 0.0f : pj.z;
-                        float const rx = pj.x - qx;
-                        float const ry = pj.y - qy;
-                        float const rz = pjz - qz;
-                        float const x = (rx * rx + ry * ry) + rz * rz;
-                        if (valid && x < q_rmax_sq && x >= q_rmin_sq)
+                        int const sx = fa::floor_div(ux, dim0);
+                        int const piece_end = min(uxend, (sx + 1) * dim0 - 1);
+                        if (sx >= -1 && sx <= 1)
                         {
-                            uint32_t const bin = bin_of_rsq(x, sT, nbins, vmin, inv_width);
-                            if (bin < nbins)
-                                atomicAdd(&h[bin], weight);
+                            int const cx = ux - sx * dim0;
+                            pb[k] = g.cstart[rowbase + cx];
+                            pe[k] = g.cstart[rowbase + cx + (piece_end - ux) + 1];
+                            // AABB formula: r = p_j - (q_i + image), image = -shift
+                            pc[k] = (-sx + 1) * 9 + (-sy + 1) * 3 + (-sz + 1);
                         }
+                        ux = piece_end + 1;
                     }
                 }
             }
         }
+        int const nr = min(64, nrows - r0);
+        for (int r = 0; r < nr; ++r)
+        {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                int const code = __builtin_amdgcn_readlane(pc[k], r);
+                if (code < 0)
+                    continue;
+                uint32_t beg = uint32_t(__builtin_amdgcn_readlane(int(pb[k]), r));
+                uint32_t const end = uint32_t(__builtin_amdgcn_readlane(int(pe[k]), r));
+                bool const wraps = code != 13;
+                if (!wraps)
+                    beg = max(beg, A0);
+                if (beg >= end)
+                    continue;
+                uint32_t const weight = wraps ? 1u : 2u;
+                float const qx = pa.x + b.img[code][0];
+                float const qy = pa.y + b.img[code][1];
+                float const qz = pa.z + b.img[code][2];
+                float4 cur = g.cpos[min(beg + bl, end - 1u)];
+                for (uint32_t jb = beg; jb < end; jb += 8)
+                {
+                    float4 nxt = cur;
+                    if (jb + 8u < end)
+                        nxt = g.cpos[min(jb + 8u + bl, end - 1u)];
+                    uint32_t const pos_b = jb + bl;
+                    bool const valid = a_ok && pos_b < end && (wraps || pos_b > pos_a);
+                    float const pjz = b.is2D ? 0.0f : cur.z;
+                    float const rx = cur.x - qx;
+                    float const ry = cur.y - qy;
+                    float const rz = pjz - qz;
+                    float const x = (rx * rx + ry * ry) + rz * rz;
+                    if (valid && x < q_rmax_sq && x >= q_rmin_sq)
+                    {
+                        uint32_t const bin = bin_of_rsq(x, sT, nbins, vmin, inv_width);
+                        if (bin < nbins)
+                            atomicAdd(&h[bin], weight);
+                    }
+                    cur = nxt;
+                }
+            }
+        }
     }
+}
+
+template<int R, bool LDS_HIST>
+__global__ __launch_bounds__(kBlock) void rdf_rows_kernel(DevBox b, DevGrid g, const uint4* __restrict__ table,
+                                                          const uint32_t* __restrict__ ncl_ptr, float q_rmax_sq,
+                                                          float q_rmin_sq, const float* __restrict__ T,
+                                                          uint32_t nbins, float vmin, float inv_width,
+                                                          uint32_t* __restrict__ counts)
+{
+    extern __shared__ float smem[];
+    float* sT = smem;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + nbins + 1);
+    hist_prologue<LDS_HIST>(hist, sT, T, nbins);
+    constexpr uint32_t kWaves = kBlock / 64;
+    uint32_t const I = fa::xcd_tile(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
+    if (I < *ncl_ptr)
+        rdf_rows_wave<R>(b, g, table[I], q_rmax_sq, q_rmin_sq, sT, nbins, vmin, inv_width,
+                         LDS_HIST ? hist : counts);
     hist_epilogue<LDS_HIST>(hist, nbins, counts);
 }
 
@@ -376,6 +456,234 @@ __global__ void add_self_pairs_kernel(uint32_t* counts, uint32_t bin, uint32_t n
         counts[bin] += n;
 }
 
+struct RowsArgs
+{
+    fa_nq* nq;
+    float qmax_sq, qmin_sq;
+    const float* T;
+    uint32_t bins;
+    float r_min, inv_width, q_r_max;
+};
+constexpr int kRowVariants = 3; // R = 2, 3, 4
+
+// Self RDF through the row-cluster kernel with cells at least r_max / R wide.
+// Sets *handled = false (and does nothing) when the kernel does not apply:
+// points outside the box, or rows too long for the cluster encoding.
+int run_rows(const RowsArgs& a, int R, uint32_t* d_counts, bool* handled)
+{
+    *handled = false;
+    fa_nq* nq = a.nq;
+    hipStream_t s = fa::stream();
+    int dims[3];
+    fa::grid_dims_for(nq->box, nq->n, a.q_r_max, R, dims);
+    if (dims[0] > 65535)
+        return FA_OK;
+    if (int rc = fa::ensure_grid(nq, dims))
+        return rc;
+    if (nq->cur->any_outside)
+        return FA_OK;
+    *handled = true;
+    DevGrid const g = nq->cur->dev();
+    DevBox const& db = nq->cur->dbox;
+    uint32_t const nrows = uint32_t(dims[1]) * uint32_t(dims[2]);
+    uint64_t const max_clusters = uint64_t(nq->n) / 8 + nrows;
+    void* tmp = nullptr;
+    size_t const rb = sizeof(uint32_t) * (size_t(nrows) + 1);
+    if (int rc = fa::scratch(13, 2 * rb + sizeof(uint4) * max_clusters + 256, &tmp))
+        return rc;
+    uint32_t* nclus = static_cast<uint32_t*>(tmp);
+    uint32_t* cl_off = nclus + (nrows + 1);
+    uint4* table = reinterpret_cast<uint4*>(static_cast<char*>(tmp) + ((2 * rb + 15) & ~size_t(15)));
+    unsigned const rblocks = (nrows + kBlock - 1) / kBlock;
+    FA_KLAUNCH(row_cluster_count_kernel, dim3(rblocks), dim3(kBlock), 0, s, g.cstart, uint32_t(dims[0]), nrows,
+               nclus);
+    if (int rc = fa::exclusive_scan_u32(nclus, cl_off, nrows, nullptr))
+        return rc;
+    unsigned const tblocks = unsigned((max_clusters + kBlock - 1) / kBlock);
+    FA_KLAUNCH(row_cluster_table_kernel, dim3(tblocks), dim3(kBlock), 0, s, g.cstart, uint32_t(dims[0]), nrows,
+               cl_off, uint32_t(max_clusters), table);
+    unsigned const wblocks = unsigned((max_clusters + kBlock / 64 - 1) / (kBlock / 64));
+    bool const lds = a.bins <= kMaxLdsBins;
+    size_t const shmem = sizeof(float) * (a.bins + 1) + (lds ? sizeof(uint32_t) * a.bins : 0);
+#define FA_ROWS(RR)                                                                                                 \
+    case RR:                                                                                                        \
+        if (lds)                                                                                                    \
+            FA_KLAUNCH((rdf_rows_kernel<RR, true>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,            \
+                       cl_off + nrows, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,        \
+                       a.inv_width, d_counts);                                                                      \
+        else                                                                                                        \
+            FA_KLAUNCH((rdf_rows_kernel<RR, false>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,           \
+                       cl_off + nrows, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,        \
+                       a.inv_width, d_counts);                                                                      \
+        break;
+    switch (R)
+    {
+        FA_ROWS(2)
+        FA_ROWS(3)
+        FA_ROWS(4)
+    default:
+        return fa::fail(FA_ERR_RUNTIME, "unsupported RDF cell ratio");
+    }
+#undef FA_ROWS
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+// ------------------------------------------------------------------ batches
+// The frame of a batch row (frames' rows are consecutive; binary search).
+__device__ inline uint32_t frame_of_row(const fa::BatchFrame* __restrict__ frames, uint32_t nf, uint32_t row)
+{
+    uint32_t lo = 0, hi = nf;
+    while (hi - lo > 1)
+    {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (frames[mid].row_base <= row)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void brow_count_kernel(fa::BatchGrid G, uint32_t* __restrict__ nclus)
+{
+    uint32_t const r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= G.total_rows)
+        return;
+    const fa::BatchFrame& F = G.frames[frame_of_row(G.frames, G.n_frames, r)];
+    uint32_t const dim0 = uint32_t(F.dim[0]);
+    const uint32_t* cs = G.cstart + F.cell_base + size_t(r - F.row_base) * dim0;
+    nclus[r] = (cs[dim0] - cs[0] + 7u) / 8u;
+}
+
+// Entries (frame-local row, start, count, cx_lo | cx_hi << 16) plus the frame
+// of each entry in `cl_frame`.
+__global__ void brow_table_kernel(fa::BatchGrid G, const uint32_t* __restrict__ cl_off, uint32_t max_clusters,
+                                  uint4* __restrict__ table, uint32_t* __restrict__ cl_frame)
+{
+    uint32_t const c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= max_clusters || c >= cl_off[G.total_rows])
+        return;
+    uint32_t lo = 0, hi = G.total_rows;
+    while (hi - lo > 1)
+    {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (cl_off[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    uint32_t const r = lo;
+    uint32_t const f = frame_of_row(G.frames, G.n_frames, r);
+    const fa::BatchFrame& F = G.frames[f];
+    uint32_t const dim0 = uint32_t(F.dim[0]);
+    const uint32_t* cs = G.cstart + F.cell_base + size_t(r - F.row_base) * dim0;
+    uint32_t const p = cs[0] + 8u * (c - cl_off[r]);
+    uint32_t const last = min(p + 8u, cs[dim0]) - 1u;
+    auto cell_of_pos = [&](uint32_t q) {
+        uint32_t l = 0, h = dim0;
+        while (h - l > 1)
+        {
+            uint32_t const m = (l + h) >> 1;
+            if (cs[m] <= q)
+                l = m;
+            else
+                h = m;
+        }
+        return l;
+    };
+    table[c] = make_uint4(r - F.row_base, p, last - p + 1u, cell_of_pos(p) | (cell_of_pos(last) << 16));
+    cl_frame[c] = f;
+}
+
+// Flush a wave's private histogram into frame f's counts and clear it.
+__device__ inline void flush_wave_hist(uint32_t* wh, uint32_t nbins, uint32_t* __restrict__ counts)
+{
+    uint32_t const lane = threadIdx.x & 63u;
+    for (uint32_t k = lane; k < nbins; k += 64)
+    {
+        uint32_t const v = wh[k];
+        if (v)
+        {
+            atomicAdd(&counts[k], v);
+            wh[k] = 0u;
+        }
+    }
+}
+
+// Persistent batched RDF: every wave takes chunks of kChunk cluster entries
+// from a device work queue until the batch is exhausted, so frames of any size
+// share one launch and the GPU stays full to the end.  Each wave bins into its
+// own LDS histogram and flushes it to its frame's counts (bins per frame) when
+// the frame changes and at exit.  Frames with points outside their box take
+// the generic per-point search for the cluster's points (weight 1 per ordered
+// pair).
+constexpr uint32_t kChunk = 4;
+
+template<int R>
+__global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, const uint4* __restrict__ table,
+                                                           const uint32_t* __restrict__ cl_frame,
+                                                           const uint32_t* __restrict__ ncl_ptr,
+                                                           uint32_t* __restrict__ queue, float q_rmax_sq,
+                                                           const float* __restrict__ T, uint32_t nbins, float vmin,
+                                                           float inv_width, uint32_t* __restrict__ counts)
+{
+    extern __shared__ float smem[];
+    float* sT = smem;
+    uint32_t const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    uint32_t* wh = reinterpret_cast<uint32_t*>(smem + nbins + 1) + wave * nbins;
+    for (uint32_t t = threadIdx.x; t <= nbins; t += blockDim.x)
+        sT[t] = T[t];
+    for (uint32_t t = lane; t < nbins; t += 64)
+        wh[t] = 0u;
+    __syncthreads();
+    uint32_t const ncl = *ncl_ptr;
+    uint32_t cur = 0xffffffffu;
+    for (;;)
+    {
+        uint32_t base = 0;
+        if (lane == 0)
+            base = atomicAdd(queue, kChunk);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base >= ncl)
+            break;
+        uint32_t const stop = min(base + kChunk, ncl);
+        for (uint32_t I = base; I < stop; ++I)
+        {
+            uint32_t const f = cl_frame[I];
+            if (f != cur)
+            {
+                if (cur != 0xffffffffu)
+                    flush_wave_hist(wh, nbins, counts + size_t(cur) * nbins);
+                cur = f;
+            }
+            const fa::BatchFrame& F = G.frames[f];
+            DevGrid const g = fa::frame_grid(F, G);
+            uint4 const e = table[I];
+            if (!F.any_outside)
+            {
+                rdf_rows_wave<R>(F.box, g, e, q_rmax_sq, 0.0f, sT, nbins, vmin, inv_width, wh);
+            }
+            else if (lane < e.z)
+            {
+                float4 const q = g.cpos[e.y + lane];
+                uint32_t const qi = __float_as_uint(q.w);
+                fa::for_each_candidate<AABB_FORMULA>(F.box, g, q.x, q.y, q.z, R, false,
+                                                     [&](uint32_t j, float, float, float, float r_sq, int) {
+                                                         if (j == qi || !(r_sq < q_rmax_sq))
+                                                             return;
+                                                         uint32_t const bin =
+                                                             bin_of_rsq(r_sq, sT, nbins, vmin, inv_width);
+                                                         if (bin < nbins)
+                                                             atomicAdd(&wh[bin], 1u);
+                                                     });
+            }
+        }
+    }
+    if (cur != 0xffffffffu)
+        flush_wave_hist(wh, nbins, counts + size_t(cur) * nbins);
+}
+
 int grid_blocks(uint64_t work)
 {
     uint64_t const b = (work + kBlock - 1) / kBlock;
@@ -383,6 +691,71 @@ int grid_blocks(uint64_t work)
 }
 
 } // namespace
+
+namespace fa {
+
+// Batched self RDF with cells at least r_max / R wide (R in 2..4): counts has
+// n_frames x bins entries.  Bins beyond 2048 are not supported here (each wave
+// keeps a private LDS histogram).
+int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const BatchRdfScratch& S, uint32_t* d_counts)
+{
+    hipStream_t s = stream();
+    if (bins == 0 || bins > 2048)
+        return fail(FA_ERR_INVALID, "batched RDF supports 1..2048 bins");
+    const float* T = S.T;
+    float const inv_width = 1.0f / ((r_max - 0.0f) / float(bins));
+    unsigned const rblocks = (G.total_rows + kBlock - 1) / kBlock;
+    if (G.total_rows)
+        FA_KLAUNCH(brow_count_kernel, dim3(rblocks), dim3(kBlock), 0, s, G, S.nclus);
+    if (int rc = exclusive_scan_u32_async(S.nclus, S.cl_off, G.total_rows, S.scan_tmp, S.scan_bytes))
+        return rc;
+    unsigned const tblocks = unsigned((S.max_clusters + kBlock - 1) / kBlock);
+    uint32_t* cl_frame = reinterpret_cast<uint32_t*>(S.table + S.max_clusters);
+    if (S.max_clusters)
+        FA_KLAUNCH(brow_table_kernel, dim3(tblocks), dim3(kBlock), 0, s, G, S.cl_off, uint32_t(S.max_clusters),
+                   S.table, cl_frame);
+    if (int rc = zero_u32(S.queue, 1))
+        return rc;
+    // persistent grid: enough workgroups to fill every CU a few times over
+    static int n_cu = 0;
+    if (!n_cu)
+    {
+        int dev = 0;
+        FA_HIP(hipGetDevice(&dev));
+        FA_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    unsigned const blocks = unsigned(std::max(1, n_cu) * 8);
+    size_t const shmem = sizeof(float) * (bins + 1) + sizeof(uint32_t) * bins * (kBlock / 64);
+    float const qmax_sq = r_max * r_max;
+#define FA_BROWS(RR)                                                                                                \
+    case RR:                                                                                                        \
+        FA_KLAUNCH(brdf_rows_kernel<RR>, dim3(blocks), dim3(kBlock), shmem, s, G, S.table, cl_frame,               \
+                   S.cl_off + G.total_rows, S.queue, qmax_sq, T, bins, 0.0f, inv_width, d_counts);                  \
+        break;
+    switch (R)
+    {
+        FA_BROWS(2)
+        FA_BROWS(3)
+        FA_BROWS(4)
+    default:
+        return fail(FA_ERR_INVALID, "unsupported RDF cell ratio");
+    }
+#undef FA_BROWS
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int thresholds_for(uint32_t bins, float r_max, float* d_T)
+{
+    float const inv_width = 1.0f / ((r_max - 0.0f) / float(bins));
+    std::vector<float> T(bins + 1);
+    for (uint32_t k = 0; k <= bins; ++k)
+        T[k] = first_rsq_with_bin_at_least(int(k), 0.0f, r_max, inv_width, bins);
+    FA_HIP(hipMemcpy(d_T, T.data(), sizeof(float) * (bins + 1), hipMemcpyHostToDevice));
+    return FA_OK;
+}
+
+} // namespace fa
 
 extern "C" {
 
@@ -410,38 +783,42 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
 
     if (self && nq->kind == FA_NQ_AABB)
     {
-        // cluster-pair kernel over cells at least r_max/2 wide
-        int const R = 2;
-        int dims[3];
-        fa::grid_dims_for(nq->box, nq->n, q_r_max, R, dims);
-        if (int rc = fa::ensure_grid(nq, dims))
-            return rc;
-        if (!nq->cur->any_outside)
+        // row-cluster kernel; the cell size (r_max / R) is picked by the autotuner
+        RowsArgs ra{nq, qmax_sq, qmin_sq, T, bins, r_min, inv_width, q_r_max};
+        double const density = double(nq->n) / double(fa::box_volume(nq->box));
+        int const key = fa::tune_key(density, q_r_max);
+        int variant = fa::tuned_variant(fa::TUNE_RDF_SELF, key);
+        bool handled = false;
+        if (variant < 0 && fa::tuning_enabled())
         {
-            DevGrid const g = nq->cur->dev();
-            DevBox const& db = nq->cur->dbox;
-            uint32_t const ncell = uint32_t(dims[0]) * dims[1] * dims[2];
-            uint64_t const max_clusters = uint64_t(nq->n) / 8 + ncell;
-            void* tmp = nullptr;
-            size_t const cb = sizeof(uint32_t) * (size_t(ncell) + 1);
-            if (int rc = fa::scratch(13, 2 * cb + sizeof(uint2) * max_clusters + 256, &tmp))
+            uint32_t* trial = nullptr;
+            if (int rc = fa::scratch(14, sizeof(uint32_t) * bins + 256, (void**) &trial))
                 return rc;
-            uint32_t* nclus = static_cast<uint32_t*>(tmp);
-            uint32_t* cl_off = nclus + (ncell + 1);
-            uint2* table = reinterpret_cast<uint2*>(static_cast<char*>(tmp) + ((2 * cb + 15) & ~size_t(15)));
-            unsigned const cblocks = (ncell + kBlock - 1) / kBlock;
-            FA_KLAUNCH(cluster_count_kernel, dim3(cblocks), dim3(kBlock), 0, s, g.cstart, ncell, nclus);
-            if (int rc = fa::exclusive_scan_u32(nclus, cl_off, ncell, nullptr))
-                return rc;
-            FA_KLAUNCH(cluster_table_kernel, dim3(cblocks), dim3(kBlock), 0, s, g.cstart, ncell, cl_off, table);
-            unsigned const wblocks = unsigned((max_clusters + kBlock / 64 - 1) / (kBlock / 64));
-            if (lds)
-                FA_KLAUNCH((rdf_cluster_kernel<2, true>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,
-                           cl_off + ncell, qmax_sq, qmin_sq, T, bins, r_min, inv_width, d_counts);
-            else
-                FA_KLAUNCH((rdf_cluster_kernel<2, false>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,
-                           cl_off + ncell, qmax_sq, qmin_sq, T, bins, r_min, inv_width, d_counts);
-            FA_HIP(hipGetLastError());
+            float best = 0.0f;
+            for (int v = 0; v < kRowVariants; ++v)
+            {
+                float ms = 0.0f;
+                bool ok = false;
+                // first run builds the grid; the second is timed
+                if (int rc = run_rows(ra, 2 + v, trial, &ok))
+                    return rc;
+                if (!ok)
+                    break;
+                if (int rc = fa::time_launch([&] { return run_rows(ra, 2 + v, trial, &ok); }, &ms))
+                    return rc;
+                if (variant < 0 || ms < best)
+                {
+                    best = ms;
+                    variant = v;
+                }
+            }
+            if (variant >= 0)
+                fa::record_variant(fa::TUNE_RDF_SELF, key, variant);
+        }
+        if (int rc = run_rows(ra, 2 + std::max(variant, 0), d_counts, &handled))
+            return rc;
+        if (handled)
+        {
             if (!exclude_ii && qmin_sq <= 0.0f)
             {
                 // (i, i) pairs at distance 0 (NeighborQuery.h:317-328 keeps them when exclude_ii is off)

@@ -160,3 +160,46 @@ def two_phase(n_solid_cells, rho_solid=1.05, rho_liquid=0.95, sigma=0.04, seed=5
     pts = box.wrap(pts.astype(np.float32))
     labels = np.concatenate([np.ones(n_s, np.int8), np.zeros(n_s, np.int8)])
     return box, np.ascontiguousarray(pts, dtype=np.float32), labels
+
+
+_BASIS = {
+    "fcc": np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]),
+    "bcc": np.array([[0, 0, 0], [0.5, 0.5, 0.5]]),
+}
+
+
+def mixed_frames(n_frames=256, n_min=10_000, n_max=200_000, tilt=0.3, seed=4):
+    """A trajectory of frames of uneven size: N per frame log-uniform in
+    [n_min, n_max]; each frame is a uniform liquid (density 0.8-1.0) or a
+    jittered FCC/BCC crystal (density 1.0, sigma 0.03-0.1 a), in a cubic box or
+    a triclinic box with tilts uniform in [-tilt, tilt].  Returns a list of
+    (box, float32 points) and a list of frame kinds."""
+    rng = np.random.default_rng(seed)
+    frames, kinds = [], []
+    for _ in range(n_frames):
+        n_target = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        kind = ("liquid", "fcc", "bcc")[int(rng.integers(3))]
+        tilts = rng.uniform(-tilt, tilt, 3) if rng.random() < 0.5 else np.zeros(3)
+        if kind == "liquid":
+            rho = rng.uniform(0.8, 1.0)
+            n = n_target
+            L = (n / rho) ** (1.0 / 3.0)
+            frac = rng.random((n, 3))
+            noise = None
+        else:
+            basis = _BASIS[kind]
+            m = len(basis)
+            cells = max(2, int(round((n_target / m) ** (1.0 / 3.0))))
+            a = (m / 1.0) ** (1.0 / 3.0)
+            L = cells * a
+            grid = np.stack(np.meshgrid(*[np.arange(cells)] * 3, indexing="ij"), -1).reshape(-1, 3)
+            frac = ((grid[:, None, :] + basis[None]) / cells).reshape(-1, 3)
+            noise = rng.normal(0.0, rng.uniform(0.03, 0.1) * a, size=frac.shape)
+        box = freud_amd.box.Box(L, L, L, *tilts)
+        pts = box.make_absolute(frac.astype(np.float32)).astype(np.float64)
+        if noise is not None:
+            pts = pts + noise
+        pts = box.wrap(pts.astype(np.float32))
+        frames.append((box, np.ascontiguousarray(pts, dtype=np.float32)))
+        kinds.append(kind)
+    return frames, kinds

@@ -115,6 +115,59 @@ int fa_profile_reset(void);
 int fa_profile_count(void);
 int fa_profile_get(int idx, char* name, int len, uint64_t* launches, double* total_ms);
 
+/* Autotuner (no counterpart in the reference, which has one CPU code path).
+ * Analyses with several bit-identical launch variants (cell size relative to
+ * r_max) time each variant on their first call for an unseen problem class
+ * and record the fastest.  fa_tune_get/fa_tune_set read and preset the table
+ * (analysis, key, variant) so a caller can persist it; variant < 0 in
+ * fa_tune_set removes an entry.  Tuning is on by default. */
+int fa_tune_enable(int on);
+int fa_tune_count(void);
+int fa_tune_get(int idx, int* analysis, int* key, int* variant);
+int fa_tune_set(int analysis, int key, int variant);
+int fa_tune_reset(void);
+
+/* ---------------------------------------------------------------- batched trajectories
+ * No counterpart in the reference, which analyses one frame per call (a
+ * Python loop of compute(..., reset=False) over a trajectory; per frame
+ * RDF.cc:101-110, NeighborQuery.h:434-481, Steinhardt.cc:85-222).  A batch
+ * holds many frames of any box shape (cubic, triclinic, 2D) and size; their
+ * points are concatenated, frame f at rows point_offsets[f] ..
+ * point_offsets[f+1]-1 of the batch's point array (fa_batch_points).  One run
+ * computes, for every frame, the planned analyses with the single-frame
+ * semantics: an AABBQuery ball query (query points = points, exclude_ii) ->
+ * the frame's NeighborList, the RDF bin counts (0 .. rdf_r_max), and
+ * Steinhardt q_l over the ball bonds with the frame's system order.  Runs
+ * after the first replay a captured HIP graph when use_graph is set. */
+typedef struct fa_batch fa_batch;
+typedef struct fa_batch_plan
+{
+    float ball_r_max;  /* > 0: ball query radius; 0: no ball query (and no q_l) */
+    uint32_t rdf_bins; /* 0: no RDF; at most 2048 */
+    float rdf_r_max;
+    uint32_t l; /* Steinhardt l over the ball bonds; 0xffffffff: none */
+} fa_batch_plan;
+int fa_batch_create(uint32_t n_frames, const fa_box* boxes, const uint64_t* point_offsets, const fa_batch_plan* plan,
+                    fa_batch** out);
+int fa_batch_destroy(fa_batch* b);
+/* The batch's device point array (total x 3 float32) for the caller to fill. */
+int fa_batch_points(fa_batch* b, float** d_points);
+/* Runs every planned analysis on the current points and synchronises. */
+int fa_batch_run(fa_batch* b, int use_graph);
+/* Bonds per frame (host array of n_frames). */
+int fa_batch_num_bonds(fa_batch* b, uint64_t* per_frame);
+/* Copies frame f's NeighborList (frame-local indices) into *out. */
+int fa_batch_nlist(fa_batch* b, uint32_t frame, fa_nlist* out);
+/* Device pointer to the n_frames x rdf_bins counts of the last run. */
+int fa_batch_rdf_counts(fa_batch* b, uint32_t** d_counts);
+/* Device pointers to q_l (total) and q_lm (total x (2l+1) complex, m order
+ * 0..l, -1..-l) of the last run; order (host, n_frames) = each frame's
+ * system q_l.  Any output pointer may be NULL. */
+int fa_batch_ql(fa_batch* b, float** d_ql, float** d_qlm, float* order);
+/* The launch shape the autotuner chose (cells r/R wide), the bond capacity and
+ * whether a graph is captured. */
+int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, uint64_t* capacity, int* graph_ready);
+
 /* ---------------------------------------------------------------- box
  * freud/box/Box.h:212-329: op 0 makeFractional, 1 makeAbsolute, 2 wrap over n
  * device vec3s (in place allowed). */

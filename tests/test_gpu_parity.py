@@ -167,9 +167,26 @@ def test_rdf_self_cluster_cases(oracle, L, r_max, n, layout):
     if layout == "2d":
         pts[:, 2] = 0
     bins = 37
-    rdf = freud_amd.density.RDF(bins=bins, r_max=r_max, r_min=0.25).compute((box, pts))
     counts = oracle.rdf_accumulate(_bt(box), pts, pts, bins, r_max, r_min=0.25, exclude_ii=True, nthreads=8)
-    npt.assert_array_equal(rdf.bin_counts, counts)
+    for variant in _rdf_variants(box, len(pts), r_max):
+        rdf = freud_amd.density.RDF(bins=bins, r_max=r_max, r_min=0.25).compute((box, pts))
+        npt.assert_array_equal(rdf.bin_counts, counts, err_msg=f"cell variant {variant}")
+
+
+def _rdf_variants(box, n, r_max):
+    """Presets each self-RDF launch variant (cells r_max / (2 + v) wide) for
+    this problem class in turn; the autotuner keys a class by the expected
+    number of points in an r_max^3 volume, in half-octaves."""
+    from freud_amd import _lib
+
+    key = int(np.floor(2.0 * np.log2(n / box.volume * r_max ** 3) + 0.5))
+    before = _lib.tune_table()
+    try:
+        for v in range(3):
+            _lib.tune_load({(1, key): v})
+            yield v
+    finally:
+        _lib.tune_load({(1, key): before.get((1, key), -1)})
 
 
 def test_rdf_lj_golden():
