@@ -304,15 +304,251 @@ def run_batched(args):
     print(json.dumps(out))
 
 
+def run_stress(args):
+    """Triclinic N=500,000 at density 0.85 (tilts U[-0.3, 0.3], seed 2): ball
+    query r_max=2 -> NeighborList, RDF 150 bins r_max=2, Steinhardt q_l for
+    l = 4, 6, 8, 10, 12; plus a 2D hexagonal lattice of 250,000 points (jitter
+    0.05): kNN k=6 -> Hexatic k=6, and a ball query r_max=2."""
+    from freud_amd import _lib, data
+    from freud_amd._lib import DeviceArray
+
+    L, lib, check = _lib, _lib.lib(), _lib.check
+    tbox, tpts = data.triclinic_uniform(500_000, 0.85, seed=2, tilt=0.3)
+    hbox, hpts = data.hex2d_jittered(250_000, jitter=0.05, seed=3)
+    nt, nh = len(tpts), len(hpts)
+    d_t, d_h = DeviceArray.from_numpy(tpts), DeviceArray.from_numpy(hpts)
+    cb_t, cb_h = L.make_box(tbox), L.make_box(hbox)
+    ls = np.array([4, 6, 8, 10, 12], np.uint32)
+    nm = int(np.sum(2 * ls + 1))
+    d_qlm = DeviceArray((nt * nm * 2,), np.float32)
+    d_ql = DeviceArray((nt * len(ls),), np.float32)
+    d_psi = DeviceArray((nh * 2,), np.float32)
+    counts = DeviceArray.zeros(150, np.uint32)
+    order = np.zeros(len(ls), np.float32)
+    nl_t, nl_h, nl_k = L.fa_nlist(), L.fa_nlist(), L.fa_nlist()
+    ball2 = L.fa_query_args(L.MODE_BALL, 0xFFFFFFFF, 2.0, 0.0, -1.0, -1.0, 1)
+    knn6 = L.fa_query_args(L.MODE_NEAREST, 6, -1.0, 0.0, -1.0, -1.0, 1)
+
+    def step():
+        h = ctypes.c_void_p()
+        check(lib.fa_nq_create(ctypes.byref(cb_t), d_t.ptr, nt, L.NQ_AABB, 0.0, ctypes.byref(h)))
+        try:
+            check(lib.fa_query(h, d_t.ptr, nt, ctypes.byref(ball2), 0, ctypes.byref(nl_t)))
+            counts.zero()
+            check(lib.fa_rdf_accumulate_query(h, d_t.ptr, nt, 2.0, 0.0, 1, 150, 0.0, 2.0, counts.ptr))
+            check(lib.fa_steinhardt(ctypes.byref(cb_t), d_t.ptr, nt, ctypes.byref(nl_t), ls.ctypes.data, len(ls), 0,
+                                    d_qlm.ptr, d_ql.ptr, None, None, None, order.ctypes.data))
+        finally:
+            lib.fa_nq_destroy(h)
+        check(lib.fa_nq_create(ctypes.byref(cb_h), d_h.ptr, nh, L.NQ_AABB, 0.0, ctypes.byref(h)))
+        try:
+            check(lib.fa_query(h, d_h.ptr, nh, ctypes.byref(knn6), 0, ctypes.byref(nl_k)))
+            check(lib.fa_hexatic(ctypes.byref(cb_h), d_h.ptr, nh, ctypes.byref(nl_k), 6, 0, d_psi.ptr))
+            check(lib.fa_query(h, d_h.ptr, nh, ctypes.byref(ball2), 0, ctypes.byref(nl_h)))
+        finally:
+            lib.fa_nq_destroy(h)
+
+    for w in range(args.warmup):
+        step()
+        L.synchronize()
+    L.profile_reset()
+    L.profile_enable(True)
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    L.synchronize()
+    t1 = time.perf_counter()
+    L.profile_enable(False)
+    stats = L.profile_stats()
+    ms = (t1 - t0) * 1e3 / args.steps
+    bt, bh, bk = int(nl_t.num_bonds), int(nl_h.num_bonds), int(nl_k.num_bonds)
+    rdf_pairs = int(counts.numpy().astype(np.int64).sum())
+    pairs = bt + bh + rdf_pairs
+    print(f"[bench] stress {ms:.3f} ms/step: triclinic bonds {bt}, rdf pairs {rdf_pairs}, 2D ball bonds {bh}, "
+          f"2D kNN bonds {bk}", file=sys.stderr, flush=True)
+    kernels = {}
+    for name, (cnt, tot) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        kernels[name] = {"launches_per_step": cnt / args.steps, "avg_ms": round(tot / cnt, 4),
+                         "share": round(tot / (ms * args.steps), 4)}
+    dom = max(stats.items(), key=lambda kv: kv[1][1])[0]
+    cnt, tot = stats[dom]
+    avg_ms = tot / cnt
+    # dominant kernel against HBM with its compulsory bytes (q_lm for five l)
+    if dom.startswith("qlm_kernel"):
+        # one l per launch: the mean of 2l+1 over l = 4..12 is 17 complex values per point
+        ab = 24 * bt + 8 * 17 * nt + 4 * nt
+    else:
+        ab = algorithmic_bytes(dom, nt, bt, 0) or 0
+    achieved = ab / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "pairs/s (query+RDF) & particles/s (SolidLiquid), plain vs batched/learned",
+        "value": pairs / (ms / 1e3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform triclinic + jittered hexagonal lattice)",
+        "config": {
+            "workload": "stress: triclinic N=500000 density 0.85 tilts U[-0.3,0.3] (ball r_max=2, RDF 150 bins "
+                        "r_max=2, Steinhardt l=4,6,8,10,12) + 2D hexagonal N=250000 jitter 0.05 (kNN k=6 + Hexatic "
+                        "k=6, ball r_max=2)",
+            "n_points": nt + nh,
+            "triclinic_bonds": bt,
+            "rdf_pairs": rdf_pairs,
+            "hex_ball_bonds": bh,
+            "hex_knn_bonds": bk,
+            "particles_per_s": (nt + nh) / (ms / 1e3),
+        },
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "avg_ms": round(avg_ms, 4)},
+        "kernels": kernels,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def run_learned(args):
+    """Two-phase box of 2,097,152 particles (FCC 64^3 x 4 at density 1.05,
+    sigma 0.04 a, beside as many uniform-liquid particles at density 0.95):
+    SolidLiquid l=6, q_threshold 0.7, solid_threshold 6 on the r_max=1.5
+    NeighborList, exactly and through the learned classifier (trained on 32
+    generated two-phase frames + 2 validation frames).  One step = the
+    solid-like labels of the frame."""
+    from freud_amd import _lib, data
+    from freud_amd.learned import LearnedSolidLiquid
+    from freud_amd.locality import AABBQuery
+    from freud_amd.order import SolidLiquid
+
+    t0 = time.perf_counter()
+    frames = []
+    for s in range(34):
+        b, p, _ = data.two_phase(20, sigma=0.03 + 0.005 * (s % 5), seed=1000 + s)
+        frames.append((b, p))
+    model = LearnedSolidLiquid(q_threshold=0.7, solid_threshold=6, margin=0.5, seed=11)
+    losses = model.fit(frames, epochs=6, batch=8192, lr=3e-3, validation=2, target_agreement=0.997)
+    _lib.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"[bench] trained on {model._train_size} particles in {t_train:.1f} s; loss {losses[0]:.4f} -> "
+          f"{losses[-1]:.4f}; margin {model.margin:.3f}", file=sys.stderr, flush=True)
+
+    box, pts, _ = data.two_phase(64, rho_solid=1.05, rho_liquid=0.95, sigma=0.04, seed=5)
+    n = len(pts)
+    aq = AABBQuery(box, pts)
+    nl = aq.query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
+    exact_sl = SolidLiquid(6, 0.7, 6)
+
+    def run_exact():
+        exact_sl.compute(aq, neighbors=nl)
+
+    def run_learned():
+        model.compute(aq, neighbors=nl)
+
+    for _ in range(max(args.warmup, 1)):
+        run_exact()
+        run_learned()
+    _lib.synchronize()
+    exact = exact_sl.num_connections >= 6
+    res = {}
+    for name, fn in (("exact", run_exact), ("learned", run_learned)):
+        _lib.profile_reset()
+        _lib.profile_enable(True)
+        _lib.synchronize()
+        a = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        _lib.synchronize()
+        b = time.perf_counter()
+        _lib.profile_enable(False)
+        stats = _lib.profile_stats()
+        res[name] = ((b - a) * 1e3 / args.steps, {k: round(v[1] / args.steps, 4) for k, v in stats.items()})
+    agree = float(np.mean(model.solid == exact))
+    net_only = float(np.mean((model.logits > 0) == exact))
+    at_half = float(np.mean(np.where(np.abs(model.logits) < 0.5, exact, model.logits > 0) == exact))
+    ms_exact, k_exact = res["exact"]
+    ms_learned, k_learned = res["learned"]
+    labels_exact = sum(v for k, v in k_exact.items() if k.startswith(("qlm_kernel", "solid_bonds")))
+    labels_learned = sum(v for k, v in k_learned.items() if k.startswith(("sl_features", "mlp_infer", "sl_decide")))
+    print(f"[bench] exact SolidLiquid {ms_exact:.3f} ms (labels stage {labels_exact:.3f} ms); learned "
+          f"{ms_learned:.3f} ms (kernels {labels_learned:.3f} ms); agreement {agree:.5f}, fallback "
+          f"{model.num_fallback / n:.4f}", file=sys.stderr, flush=True)
+    mlp_ms = k_learned.get("mlp_infer_kernel", 0.0)
+    flops = 2.0 * n * (11 * 64 + 64 * 64 + 64)
+    # dominant learned kernel: the fused feature pass, HBM-bound; compulsory
+    # bytes = points 12 N + offsets 4 N + (j, d) 8 B per bond + outputs
+    # (features 48 N, q_6m 104 N, q_6 4 N)
+    dom = max(k_learned.items(), key=lambda kv: kv[1])[0]
+    if dom.startswith("sl_features"):
+        ab = 12 * n + 4 * n + 8 * len(nl) + (48 + 104 + 4) * n
+        ach = ab / (k_learned[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "avg_ms": k_learned[dom]}
+    else:
+        roof = {"bound": "mfma", "achieved": None, "peak": 2500.0, "unit": "TFLOP/s", "frac": None, "traffic": None,
+                "kernel": dom, "avg_ms": k_learned[dom]}
+    out = {
+        "metric": "pairs/s (query+RDF) & particles/s (SolidLiquid), plain vs batched/learned",
+        "value": n / (ms_learned / 1e3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": ms_learned,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 (network), f32 (features, exact path)",
+        "data": "synthetic (seeded two-phase FCC / liquid boxes)",
+        "config": {
+            "workload": "learned SolidLiquid: N=2097152 two-phase (FCC 64^3x4 at density 1.05, sigma 0.04a | "
+                        "uniform liquid at 0.95); SolidLiquid l=6, q_threshold 0.7, solid_threshold 6, r_max 1.5; "
+                        "MLP 11->64->64->1 BF16 MFMA, trained on 32 frames (+2 validation frames)",
+            "n_points": n,
+            "n_bonds": len(nl),
+            "train_particles": int(model._train_size),
+            "train_seconds": round(t_train, 2),
+            "train_loss_first_last": [float(losses[0]), float(losses[-1])],
+            "margin": model.margin,
+            "agreement": agree,
+            "agreement_network_only": net_only,
+            "agreement_at_margin_0.5": at_half,
+            "fallback_fraction": model.num_fallback / n,
+            "ms_exact_solidliquid": ms_exact,
+            "ms_exact_labels_kernels": labels_exact,
+            "ms_learned_kernels": labels_learned,
+            "speedup_vs_exact_solidliquid": ms_exact / ms_learned,
+            "speedup_labels_stage": labels_exact / labels_learned if labels_learned else None,
+            "particles_per_s_exact": n / (ms_exact / 1e3),
+            "kernels_exact_ms": k_exact,
+            "kernels_learned_ms": k_learned,
+        },
+        "roofline": roof,
+        "mlp_roofline": {"bound": "mfma", "achieved": round(flops / (mlp_ms * 1e-3) / 1e12, 3) if mlp_ms else None,
+                         "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(flops / (mlp_ms * 1e-3) / 1e12 / 2500.0, 5) if mlp_ms else None,
+                         "kernel": "mlp_infer_kernel", "avg_ms": mlp_ms},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("frame", "batched"), default="frame",
+    ap.add_argument("--workload", choices=("frame", "batched", "stress", "learned"), default="frame",
                     help="frame: one 1M-point frame (the headline line); batched: a 256-frame trajectory of uneven "
-                         "frames, batched + graph-replayed against the per-frame loop")
+                         "frames, batched + graph-replayed against the per-frame loop; stress: 500k triclinic "
+                         "(l = 4..12) + 250k 2D hexagonal (Hexatic); learned: 2M-particle two-phase SolidLiquid, "
+                         "exact vs learned")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -327,6 +563,10 @@ def main():
     print(f"[bench] device: {_lib.device_name()}", file=sys.stderr, flush=True)
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "stress":
+        return run_stress(args)
+    if args.workload == "learned":
+        return run_learned(args)
     if args.copy_mib > 0:
         nbytes = args.copy_mib << 20
         src = DeviceArray((nbytes // 4,), np.uint32)

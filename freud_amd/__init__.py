@@ -7,7 +7,7 @@ runs in hand-written HIP kernels (``freud_amd/csrc``) behind the C-ABI declared
 in ``include/freud_amd.h``.
 """
 
-from freud_amd import batch, box, cluster, data, density, locality, order, parallel
+from freud_amd import batch, box, cluster, data, density, learned, locality, order, parallel
 from freud_amd.box import Box
 from freud_amd.locality import AABBQuery, LinkCell, NeighborList
 
@@ -23,6 +23,7 @@ __all__ = [
     "cluster",
     "data",
     "density",
+    "learned",
     "locality",
     "order",
     "parallel",

@@ -108,6 +108,23 @@ _SIGS = {
                                    ctypes.c_void_p]),
     "fa_batch_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                        ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
+    "fa_sl_features": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32,
+                                      ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fa_sl_decide": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_float,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
+    "fa_mlp_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "fa_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "fa_mlp_num_params": (ctypes.c_int, []),
+    "fa_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fa_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "fa_mlp_set_norm": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fa_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "fa_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                    ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64,
+                                    ctypes.c_void_p]),
     "fa_tune_enable": (ctypes.c_int, [ctypes.c_int]),
     "fa_tune_count": (ctypes.c_int, []),
     "fa_tune_get": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),

@@ -477,17 +477,186 @@ __global__ __launch_bounds__(kBlock) void wl_kernel_t(uint32_t n, const float2* 
     wl[uint64_t(i) * stride + col] = result;
 }
 
-// SolidLiquid.cc:51-78 (bond dot products) and :90-96 (solid-bond counts)
-__global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, const uint32_t* __restrict__ nbr,
-                                                             const uint32_t* __restrict__ offsets,
-                                                             const float2* __restrict__ qlm,
-                                                             const float* __restrict__ ql, int normalize_q,
-                                                             float q_threshold, float* __restrict__ ql_ij,
-                                                             uint32_t* __restrict__ num_conn)
+// ------------------------------------------------------------------ learned SolidLiquid features
+// Rotation-invariant features of a particle for the learned SolidLiquid
+// classifier, from one pass over its bonds: q_l and normalised w_l for
+// l = 4, 6, 8, 10, 12 (the unweighted Steinhardt definitions,
+// Steinhardt.cc:120-222 and Wigner3j.cc:17-56 with wl_normalize) and the bond
+// count.  One associated-Legendre recurrence up to l = 12 serves all five l.
+// The particle's q_6m and q_6 are also written, with exactly the arithmetic of
+// the plain Steinhardt kernel, so that the exact SolidLiquid fallback computes
+// the reference's bond dot products from them.
+//
+// w_l here sums each unordered triple (m1 <= m2 <= m3, m1 + m2 + m3 = 0) once,
+// weighted by its number of distinct orderings: every ordering contributes the
+// same product and the same 3j symbol, so this equals the full sum up to float
+// rounding (about 80 terms instead of 469 at l = 12).
+constexpr int kFeat = 12; // q4..q12, w4..w12, n_b, padding
+
+__host__ __device__ constexpr int feat_base(int l)
 {
+    return ((l - 4) / 2) * ((l - 4) / 2) + 4 * ((l - 4) / 2);
+}
+
+template<int L> __device__ inline float2 qm_signed(const float2* q, int m)
+{
+    // q_{l,-m} = (-1)^m conj(q_{l,m})
+    if (m >= 0)
+        return q[m];
+    float2 const v = q[-m];
+    return (m & 1) ? make_float2(-v.x, v.y) : make_float2(v.x, -v.y);
+}
+
+template<int L> __device__ inline float wl_sorted(const float2* q, const float* __restrict__ coef)
+{
+    float result = 0.0f;
+    int counter = 0;
+#pragma unroll
+    for (int m1 = -L; m1 <= 0; ++m1)
+    {
+#pragma unroll
+        for (int m2 = (m1 > -L - m1 ? m1 : -L - m1); 2 * m2 <= -m1; ++m2)
+        {
+            int const m3 = -m1 - m2;
+            float2 const a = qm_signed<L>(q, m1);
+            float2 const bq = qm_signed<L>(q, m2);
+            float2 const c = qm_signed<L>(q, m3);
+            float const w = coef[counter++];
+            float const abr = a.x * bq.x - a.y * bq.y;
+            float const abi = a.x * bq.y + a.y * bq.x;
+            result += w * (abr * c.x - abi * c.y);
+        }
+    }
+    return result;
+}
+
+template<int L> __device__ inline void feat_l(const float2* acc, float inv_nb, const float* coef, float* qout, float* wout)
+{
+    float2 q[L + 1];
+    float s = 0.0f;
+#pragma unroll
+    for (int m = 0; m <= L; ++m)
+    {
+        q[m] = make_float2(acc[m].x * inv_nb, acc[m].y * inv_nb);
+        float const mag = q[m].x * q[m].x + q[m].y * q[m].y;
+        s += m ? 2.0f * mag : mag;
+    }
+    *qout = sqrtf(s * float(4.0 * M_PI / (2 * L + 1)));
+    float const w = wl_sorted<L>(q, coef);
+    *wout = w / (s * sqrtf(s));
+}
+
+__global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
+                                                             const uint32_t* __restrict__ nbr,
+                                                             const float* __restrict__ dist,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const float* __restrict__ coef4,
+                                                             const float* __restrict__ coef6,
+                                                             const float* __restrict__ coef8,
+                                                             const float* __restrict__ coef10,
+                                                             const float* __restrict__ coef12, float* __restrict__ feat,
+                                                             float2* __restrict__ qlm6, float* __restrict__ ql6)
+{
+    constexpr int LM = 12;
     uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
+    float2 acc[feat_base(14)];
+#pragma unroll
+    for (int t = 0; t < feat_base(14); ++t)
+        acc[t] = make_float2(0.0f, 0.0f);
+    uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+    for (uint32_t e = e0; e < e1; ++e)
+    {
+        float ct, st, cp, sp;
+        bond_angles(b, pts, i, nbr[2 * e + 1], dist[e], ct, st, cp, sp);
+        float pmm = 0.28209479177387814f;
+        float er = 1.0f, ei = 0.0f;
+#pragma unroll
+        for (int m = 0; m <= LM; ++m)
+        {
+            if (m > 0)
+            {
+                pmm = c_ylm.amm[m] * st * pmm;
+                float const nr = er * cp - ei * sp;
+                float const ni = er * sp + ei * cp;
+                er = nr;
+                ei = ni;
+            }
+            float p0 = pmm, p1 = 0.0f;
+#pragma unroll
+            for (int l = m; l <= LM; ++l)
+            {
+                float p;
+                if (l == m)
+                    p = pmm;
+                else if (l == m + 1)
+                {
+                    p1 = c_ylm.bm1[m] * ct * pmm;
+                    p = p1;
+                }
+                else
+                {
+                    p = c_ylm.a[l][m] * (ct * p1 - c_ylm.b[l][m] * p0);
+                    p0 = p1;
+                    p1 = p;
+                }
+                if (l >= 4 && (l & 1) == 0)
+                {
+                    float const wp = 1.0f * p;
+                    acc[feat_base(l) + m].x += wp * er;
+                    acc[feat_base(l) + m].y += wp * ei;
+                }
+            }
+        }
+    }
+    uint32_t const nb = e1 - e0;
+    float const total_w = float(nb);
+    float const inv_nb = 1.0f / total_w;
+    float* f = feat + uint64_t(i) * kFeat;
+    feat_l<4>(acc + feat_base(4), inv_nb, coef4, f + 0, f + 5);
+    feat_l<6>(acc + feat_base(6), inv_nb, coef6, f + 1, f + 6);
+    feat_l<8>(acc + feat_base(8), inv_nb, coef8, f + 2, f + 7);
+    feat_l<10>(acc + feat_base(10), inv_nb, coef10, f + 3, f + 8);
+    feat_l<12>(acc + feat_base(12), inv_nb, coef12, f + 4, f + 9);
+    f[10] = total_w;
+    f[11] = 0.0f;
+    // q_6m and q_6 as the plain Steinhardt kernel computes them
+    constexpr int L6 = 6, NM = 13;
+    const float2* a6 = acc + feat_base(6);
+    float q[2 * NM];
+    float norm = 0.0f;
+#pragma unroll
+    for (int m = 0; m <= L6; ++m)
+    {
+        float const re = a6[m].x / total_w, im = a6[m].y / total_w;
+        q[2 * m] = re;
+        q[2 * m + 1] = im;
+        if (m > 0)
+        {
+            float const sgn = (m & 1) ? -1.0f : 1.0f;
+            q[2 * (L6 + m)] = sgn * re;
+            q[2 * (L6 + m) + 1] = -sgn * im;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NM; ++t)
+        norm += q[2 * t] * q[2 * t] + q[2 * t + 1] * q[2 * t + 1];
+    float2* out = qlm6 + uint64_t(i) * NM;
+#pragma unroll
+    for (int t = 0; t < NM; ++t)
+        out[t] = make_float2(q[2 * t], q[2 * t + 1]);
+    ql6[i] = sqrtf(norm * float(4.0 * M_PI / NM));
+}
+
+// SolidLiquid.cc:51-78 (bond dot products) and :90-96 (solid-bond counts)
+// Solid-like bond count of particle i; writes each bond's q_ij when ql_ij is
+// not null.
+__device__ inline uint32_t solid_bond_count(uint32_t i, int l, const uint32_t* __restrict__ nbr,
+                                            const uint32_t* __restrict__ offsets, const float2* __restrict__ qlm,
+                                            const float* __restrict__ ql, int normalize_q, float q_threshold,
+                                            float* __restrict__ ql_ij)
+{
     int const nm = 2 * l + 1;
     float const nf = float(4.0 * M_PI / nm);
     const float2* qi = qlm + uint64_t(i) * nm;
@@ -505,10 +674,57 @@ __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, 
         }
         if (normalize_q)
             re *= nf / (ql[i] * ql[j]);
-        ql_ij[e] = re;
+        if (ql_ij)
+            ql_ij[e] = re;
         cnt += (re > q_threshold) ? 1u : 0u;
     }
-    num_conn[i] = cnt;
+    return cnt;
+}
+
+__global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, const uint32_t* __restrict__ nbr,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const float2* __restrict__ qlm,
+                                                             const float* __restrict__ ql, int normalize_q,
+                                                             float q_threshold, float* __restrict__ ql_ij,
+                                                             uint32_t* __restrict__ num_conn)
+{
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    num_conn[i] = solid_bond_count(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij);
+}
+
+// Learned SolidLiquid labels: a particle whose classifier logit is at least
+// `margin` away from 0 takes the classifier's label; the others (and NaN
+// logits, e.g. particles without bonds) count their solid-like bonds exactly,
+// with the arithmetic of solid_bonds_kernel.  fallback[i] marks the latter.
+__global__ __launch_bounds__(kBlock) void sl_decide_kernel(uint32_t n, const float* __restrict__ logits, float margin,
+                                                           const uint32_t* __restrict__ nbr,
+                                                           const uint32_t* __restrict__ offsets,
+                                                           const float2* __restrict__ qlm6,
+                                                           const float* __restrict__ ql6, int normalize_q,
+                                                           float q_threshold, uint32_t solid_threshold,
+                                                           uint8_t* __restrict__ solid, uint8_t* __restrict__ fallback,
+                                                           uint32_t* __restrict__ n_fallback)
+{
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    bool fb = false;
+    if (i < n)
+    {
+        float const y = logits[i];
+        fb = !(fabsf(y) >= margin);
+        uint8_t lab;
+        if (fb)
+            lab = solid_bond_count(i, 6, nbr, offsets, qlm6, ql6, normalize_q, q_threshold, nullptr) >= solid_threshold;
+        else
+            lab = y > 0.0f;
+        solid[i] = lab;
+        fallback[i] = fb;
+    }
+    // one atomic per wave
+    uint64_t const ball = __ballot(fb);
+    if ((threadIdx.x & 63u) == 0 && ball)
+        atomicAdd(n_fallback, uint32_t(__popcll(ball)));
 }
 
 // SolidLiquid.cc:100-110: solid-like bonds between particles with at least
@@ -856,6 +1072,29 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
     }
     return FA_OK;
 }
+// Coefficients of wl_sorted<l>: W3j(l; m1, m2, m3) times the number of
+// distinct orderings of (m1, m2, m3), in the kernel's loop order.
+int sorted_wigner_table(int l, float** d)
+{
+    static float* dev[kLMax + 1] = {};
+    if (!dev[l])
+    {
+        std::vector<float> v;
+        for (int m1 = -l; m1 <= 0; ++m1)
+            for (int m2 = std::max(m1, -l - m1); 2 * m2 <= -m1; ++m2)
+            {
+                int const m3 = -m1 - m2;
+                int const mult = (m1 == m2 && m2 == m3) ? 1 : ((m1 == m2 || m2 == m3) ? 3 : 6);
+                v.push_back(float(double(mult) * fa_wigner3j_lll(l, m1, m2, m3)));
+            }
+        if (int rc = device_alloc((void**) &dev[l], sizeof(float) * v.size()))
+            return rc;
+        FA_HIP(hipMemcpy(dev[l], v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice));
+    }
+    *d = dev[l];
+    return FA_OK;
+}
+
 template<int L>
 int launch_batch_qlm(uint32_t l, const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts,
                      const uint32_t* nb, const float* dist, const uint32_t* d_offsets,
@@ -928,6 +1167,56 @@ int fa_steinhardt(const fa_box* box, const float* d_points, uint32_t n, const fa
     if (int rc = fa_init())
         return rc;
     return fa::steinhardt_run(box, d_points, n, nl, ls, n_l, flags, d_qlm, d_ql, d_qlave, d_qlmave, d_wl, order);
+}
+
+int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, float* d_features,
+                   float* d_qlm6, float* d_ql6)
+{
+    if (int rc = fa_init())
+        return rc;
+    if (int rc = ensure_ylm_coef())
+        return rc;
+    if (nl->num_query_points != n)
+        return fa::fail(FA_ERR_RUNTIME, "NeighborList found inconsistent array sizes.");
+    float* c[5];
+    for (int k = 0; k < 5; ++k)
+        if (int rc = fa::sorted_wigner_table(4 + 2 * k, &c[k]))
+            return rc;
+    if (n == 0)
+        return FA_OK;
+    fa::DevBox const db = fa::make_devbox(*box);
+    unsigned const blocks = (n + kBlock - 1) / kBlock;
+    FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), 0, fa::stream(), db, d_points, n, nl->neighbors,
+               nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
+               reinterpret_cast<float2*>(d_qlm6), d_ql6);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float margin, const float* d_qlm6,
+                 const float* d_ql6, int normalize_q, float q_threshold, uint32_t solid_threshold, uint8_t* d_solid,
+                 uint8_t* d_fallback, uint32_t* n_fallback)
+{
+    if (int rc = fa_init())
+        return rc;
+    if (nl->num_query_points != n)
+        return fa::fail(FA_ERR_RUNTIME, "NeighborList found inconsistent array sizes.");
+    uint32_t* cnt = nullptr;
+    if (int rc = fa::scratch(15, 256, (void**) &cnt))
+        return rc;
+    if (int rc = fa::zero_u32(cnt, 1))
+        return rc;
+    if (n)
+        FA_KLAUNCH(sl_decide_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), n, d_logits,
+                   margin, nl->neighbors, nl->offsets, reinterpret_cast<const float2*>(d_qlm6), d_ql6, normalize_q,
+                   q_threshold, solid_threshold, d_solid, d_fallback, cnt);
+    FA_HIP(hipGetLastError());
+    if (n_fallback)
+    {
+        FA_HIP(hipMemcpyAsync(n_fallback, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, fa::stream()));
+        FA_HIP(hipStreamSynchronize(fa::stream()));
+    }
+    return FA_OK;
 }
 
 int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* d_cluster_idx,

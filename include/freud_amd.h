@@ -260,6 +260,46 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
 
 /* Hexatic::compute (freud/order/HexaticTranslational.cc:22-74): psi_k per
  * particle, complex64 interleaved. */
+/* Learned SolidLiquid features (no counterpart in the reference; they feed the
+ * classifier below).  Per particle, over the bonds of nl (unweighted):
+ * d_features (n x 12 float32) = q_l for l = 4, 6, 8, 10, 12, normalised w_l
+ * for the same l (Steinhardt.cc:120-222, 291-359 definitions), the bond count,
+ * and a zero pad; d_qlm6 (n x 13 complex64, m order 0..6, -1..-6) and d_ql6
+ * (n) equal fa_steinhardt's q_6m and q_6 bit for bit. */
+int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, float* d_features,
+                   float* d_qlm6, float* d_ql6);
+/* Learned SolidLiquid labels (no counterpart in the reference): particles
+ * whose classifier logit lies at least `margin` from 0 take the classifier's
+ * label (logit > 0: solid); the others, and NaN logits, count their solid-like
+ * bonds exactly as SolidLiquid.cc:51-96 does (q_ij from d_qlm6 / d_ql6 of
+ * fa_sl_features, strict q_threshold, at least solid_threshold bonds).
+ * d_solid and d_fallback are n bytes; *n_fallback (host, may be NULL) is the
+ * number of exact-path particles. */
+int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float margin, const float* d_qlm6,
+                 const float* d_ql6, int normalize_q, float q_threshold, uint32_t solid_threshold, uint8_t* d_solid,
+                 uint8_t* d_fallback, uint32_t* n_fallback);
+
+/* The classifier network (no counterpart in the reference): 11 -> 64 -> 64 ->
+ * 1, ReLU, BF16 MFMA with FP32 accumulation.  Parameters are one float32
+ * vector of fa_mlp_num_params() values: W1 (64 x 16, columns 11..15 unused),
+ * b1 (64), W2 (64 x 64), b2 (64), w3 (64), b3 (1).  Inputs are the first 11
+ * columns of fa_sl_features rows, standardised with (x - mu) / sigma. */
+typedef struct fa_mlp fa_mlp;
+int fa_mlp_create(uint64_t seed, fa_mlp** out);
+int fa_mlp_destroy(fa_mlp* m);
+int fa_mlp_num_params(void);
+int fa_mlp_set_params(fa_mlp* m, const float* params);
+int fa_mlp_get_params(fa_mlp* m, float* params);
+int fa_mlp_set_norm(fa_mlp* m, const float* mu, const float* sigma);
+/* logits (n) of feature rows d_features (n x 12) */
+int fa_mlp_forward(fa_mlp* m, const float* d_features, uint32_t n, float* d_logits);
+/* Gradient (host, fa_mlp_num_params() floats) and loss of the mean
+ * BCE-with-logits loss over the batch rows d_idx[0..batch) with 0/1 labels. */
+int fa_mlp_gradient(fa_mlp* m, const float* d_features, const uint8_t* d_labels, const uint32_t* d_idx,
+                    uint32_t batch, float* grads, float* loss);
+/* Adam training over shuffled mini-batches; epoch_loss (host, epochs) may be NULL. */
+int fa_mlp_train(fa_mlp* m, const float* d_features, const uint8_t* d_labels, uint32_t n, uint32_t epochs,
+                 uint32_t batch, float lr, uint64_t seed, float* epoch_loss);
 int fa_hexatic(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, uint32_t k, int weighted,
                float* d_psi);
 

@@ -414,3 +414,23 @@ def test_large_frame_properties():
     rot = np.ascontiguousarray(pts[:, [1, 0, 2]] * np.array([-1, 1, 1], np.float32))
     st2 = freud_amd.order.Steinhardt(6).compute((box, rot), neighbors=dict(r_max=1.5))
     npt.assert_allclose(np.sort(st2.particle_order), np.sort(q6), atol=1e-5)
+
+
+def test_stress_config_shapes(oracle):
+    """Full-size triclinic (500k, density 0.85, ball r_max=2, RDF 150 bins,
+    Steinhardt l = 4..12) and 2D hexagonal (250k, Hexatic k=6) frames."""
+    box, pts = freud_amd.data.triclinic_uniform(500_000, 0.85, seed=2, tilt=0.3)
+    aq = freud_amd.AABBQuery(box, pts)
+    nl = aq.query(pts, dict(r_max=2.0, exclude_ii=True)).toNeighborList()
+    ref = oracle.query(_bt(box), pts, pts, dict(r_max=2.0, exclude_ii=True), nthreads=16)
+    _assert_nlist_equal(nl, ref)
+    rdf = freud_amd.density.RDF(150, 2.0).compute(aq)
+    npt.assert_array_equal(rdf.bin_counts, oracle.rdf_accumulate(_bt(box), pts, pts, 150, 2.0, nthreads=16))
+    ls = [4, 6, 8, 10, 12]
+    st = freud_amd.order.Steinhardt(ls).compute(aq, neighbors=nl)
+    r = oracle.steinhardt(_bt(box), pts, ref, ls, nthreads=16)
+    npt.assert_allclose(st.particle_order, r["particle_order"], atol=1e-5)
+    hbox, hpts = freud_amd.data.hex2d_jittered(250_000, jitter=0.05, seed=3)
+    hx = freud_amd.order.Hexatic(6).compute((hbox, hpts))
+    href = oracle.query(_bt(hbox), hpts, hpts, dict(num_neighbors=6, exclude_ii=True), nthreads=16)
+    npt.assert_allclose(hx.particle_order, oracle.hexatic(_bt(hbox), hpts, href, k=6), atol=1e-5)
