@@ -319,6 +319,18 @@ class DeviceArray:
             self.ptr = None
 
 
+def reuse(owner, name, shape, dtype):
+    """owner.<name> as a DeviceArray of this shape and dtype, reusing the
+    previous compute's allocation when it matches (results are copied to the
+    host when read, so reuse never changes what an earlier read returned)."""
+    shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+    a = getattr(owner, name, None)
+    if a is None or a.shape != shape or a.dtype != np.dtype(dtype):
+        a = DeviceArray(shape, dtype)
+        setattr(owner, name, a)
+    return a
+
+
 def to_device_f32(arr):
     return DeviceArray.from_numpy(np.ascontiguousarray(arr, dtype=np.float32))
 

@@ -5,7 +5,8 @@ import ctypes
 
 import numpy as np
 
-from freud_amd._lib import DeviceArray, check, lib
+from freud_amd import _lib
+from freud_amd._lib import check, lib
 from freud_amd.locality import _PairCompute
 from freud_amd.util import _Compute
 
@@ -21,8 +22,8 @@ class Cluster(_PairCompute):
         nq, nlist, qargs, d_q, nqp = self._preprocess_arguments(system, neighbors=neighbors)
         nlist = self._neighbor_list(nq, nlist, qargs, d_q, nqp)
         n = nq._n
-        self._d_cidx = DeviceArray((n,), np.uint32)
-        self._d_sizes = DeviceArray((n,), np.uint32)
+        self._d_cidx = _lib.reuse(self, "_d_cidx", (n,), np.uint32)
+        self._d_sizes = _lib.reuse(self, "_d_sizes", (n,), np.uint32)
         ncl = ctypes.c_uint32()
         check(lib().fa_cluster(n, ctypes.byref(nlist._c), None, self._d_cidx.ptr, self._d_sizes.ptr,
                                ctypes.byref(ncl)))

@@ -404,6 +404,20 @@ struct Grid
 // wide along each plane normal (dims clamped to >= 1, z = 1 in 2D).
 int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, const int dims[3]);
 
+// A processing order of the points that follows space (cells of about four
+// points, counting-sorted), for kernels that gather neighbour data: threads
+// that run together then touch nearby memory.  It is only built when a probe
+// finds the caller's order scattered (random order); a coherent caller order
+// is kept (*order = nullptr, identity), as it also keeps the per-point outputs
+// and bond segments streaming.  Any order gives the same per-point results, so
+// the decision is cached by (pointer, n, box) and a stale one costs speed
+// only.  Identity below kSpatialOrderMin points.
+constexpr uint32_t kSpatialOrderMin = 65536;
+int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const uint32_t** order);
+// The last order spatial_order produced if it holds n points (a valid
+// permutation of 0..n-1 whatever the points), else nullptr.
+const uint32_t* cached_spatial_order(uint32_t n);
+
 // Exclusive scan of n uint32 counts into n+1 offsets (offsets[n] = total);
 // returns the 64-bit total through *total (host) when total != nullptr.
 int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total);

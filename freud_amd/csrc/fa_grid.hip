@@ -2,6 +2,9 @@
 // freud's LinkCell / CellQuery cell lists, freud/locality/LinkCell.cc:316-367,
 // freud/locality/CellQuery.cc): counting sort of the points into cells, stored
 // as cell-ordered float4 (x, y, z, bits(j)) for coalesced stencil reads.
+#include <cmath>
+#include <cstring>
+
 #include "fa_common.h"
 
 namespace {
@@ -86,9 +89,134 @@ __global__ void bgrid_reset_kernel(fa::BatchGrid G)
         G.frames[f].any_outside = 0;
 }
 
+// Counting-sort placement of point indices: order[slot] = i.
+__global__ void order_scatter_kernel(uint32_t n, const uint32_t* __restrict__ cell_of_point,
+                                     const uint32_t* __restrict__ cstart, uint32_t* __restrict__ fill,
+                                     uint32_t* __restrict__ order)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t const cell = cell_of_point[i];
+    order[cstart[cell] + atomicAdd(&fill[cell], 1u)] = i;
+}
+
+// How scattered the caller's order is: of kProbe evenly spaced index pairs
+// (i, i + 1), the number whose points lie more than `far` apart (minimum
+// image).  Points stored in a spatially coherent order (lattice order,
+// simulation cell sorting) have consecutive points close together.
+constexpr uint32_t kProbe = 4096;
+__global__ void order_probe_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, float far_sq,
+                                   uint32_t* __restrict__ count)
+{
+    uint32_t const t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= kProbe)
+        return;
+    uint32_t const i = uint32_t((uint64_t(t) * (n - 1)) / kProbe);
+    float dx, dy, dz;
+    fa::wrap(b, pts[3 * i + 3] - pts[3 * i], pts[3 * i + 4] - pts[3 * i + 1], pts[3 * i + 5] - pts[3 * i + 2], dx, dy,
+             dz);
+    if ((dx * dx + dy * dy) + dz * dz > far_sq)
+        atomicAdd(count, 1u);
+}
+
+struct OrderCache
+{
+    const float* pts = nullptr;
+    uint32_t n = 0;
+    fa_box box{};
+    bool use = false; // false: the caller's order is already coherent
+    uint32_t* order = nullptr;
+    uint32_t cap = 0;
+};
+OrderCache g_order;
+
 } // namespace
 
 namespace fa {
+
+int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const uint32_t** out)
+{
+    *out = nullptr;
+    if (n < kSpatialOrderMin)
+        return FA_OK;
+    OrderCache& c = g_order;
+    if (c.pts == d_points && c.n == n && std::memcmp(&c.box, &box, sizeof(box)) == 0)
+    {
+        *out = c.use ? c.order : nullptr;
+        return FA_OK;
+    }
+    hipStream_t s = stream();
+    double const v = double(box_volume(box));
+    double const spacing = box.is2D ? std::sqrt(v / n) : std::cbrt(v / n);
+    {
+        // keep the caller's order when at most a quarter of consecutive points
+        // lie more than 4 mean spacings apart
+        uint32_t* cnt = nullptr;
+        if (int rc = scratch(18, 256, (void**) &cnt))
+            return rc;
+        FA_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), s));
+        float const far = float(4.0 * spacing);
+        FA_KLAUNCH(order_probe_kernel, dim3(kProbe / 256), dim3(256), 0, s, make_devbox(box), d_points, n, far * far,
+                   cnt);
+        uint32_t h = 0;
+        FA_HIP(hipMemcpyAsync(&h, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+        if (h <= kProbe / 4)
+        {
+            c.pts = d_points;
+            c.n = n;
+            c.box = box;
+            c.use = false;
+            return FA_OK;
+        }
+    }
+    // cells holding about four points each
+    double const r = box.is2D ? std::sqrt(4.0 * v / n) : std::cbrt(4.0 * v / n);
+    int dims[3];
+    grid_dims_for(box, n, float(r), 1, dims);
+    uint32_t const ncell = uint32_t(dims[0]) * dims[1] * dims[2];
+    if (c.cap < n)
+    {
+        device_free(c.order);
+        c.order = nullptr;
+        c.cap = 0;
+        c.pts = nullptr;
+        if (int rc = device_alloc((void**) &c.order, sizeof(uint32_t) * n))
+            return rc;
+        c.cap = n;
+    }
+    void* tmp = nullptr;
+    size_t const cb = sizeof(uint32_t) * (size_t(ncell) + 1);
+    if (int rc = scratch(17, sizeof(uint32_t) * size_t(n) + 3 * cb + 256, &tmp))
+        return rc;
+    uint32_t* cell_of_point = static_cast<uint32_t*>(tmp);
+    uint32_t* counts = cell_of_point + n;
+    uint32_t* cstart = counts + ncell + 1;
+    uint32_t* fill = cstart + ncell + 1;
+    int* d_any = reinterpret_cast<int*>(fill + ncell);
+    FA_HIP(hipMemsetAsync(counts, 0, cb, s));
+    FA_HIP(hipMemsetAsync(fill, 0, sizeof(uint32_t) * ncell + sizeof(int), s));
+    DevBox const db = make_devbox(box);
+    unsigned const blocks = (n + 255) / 256;
+    FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, db, d_points, n, dims[0], dims[1], dims[2],
+               cell_of_point, counts, d_any);
+    if (int rc = exclusive_scan_u32(counts, cstart, ncell, nullptr))
+        return rc;
+    FA_KLAUNCH(order_scatter_kernel, dim3(blocks), dim3(256), 0, s, n, cell_of_point, cstart, fill, c.order);
+    FA_HIP(hipGetLastError());
+    c.pts = d_points;
+    c.n = n;
+    c.box = box;
+    c.use = true;
+    *out = c.order;
+    return FA_OK;
+}
+
+const uint32_t* cached_spatial_order(uint32_t n)
+{
+    return (g_order.pts && g_order.use && g_order.n == n && n >= kSpatialOrderMin) ? g_order.order : nullptr;
+}
 
 int batch_grid_build(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, void* tmp,
                      size_t tmp_bytes)

@@ -207,12 +207,15 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
                                                      const float* __restrict__ wts, const uint32_t* __restrict__ offsets,
                                                      int weighted, int want_sys, float2* __restrict__ qlm,
                                                      float* __restrict__ ql, int ql_stride, int ql_col,
-                                                     float2* __restrict__ sys_partial)
+                                                     float2* __restrict__ sys_partial,
+                                                     const uint32_t* __restrict__ order)
 {
     constexpr int NM = 2 * L + 1;
-    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    bool const valid = t0 < n;
+    uint32_t const i = (order && valid) ? order[t0] : t0;
     float q[2 * NM];
-    if (i < n)
+    if (valid)
     {
         float const v = qlm_point<L>(b, pts, i, nbr, dist, wts, offsets, weighted, q);
         float2* out = qlm + uint64_t(i) * NM;
@@ -222,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
         ql[uint64_t(i) * ql_stride + ql_col] = v;
     }
     if (want_sys)
-        block_qlm_partial<L>(i < n, q, 1.0f / float(n), sys_partial + uint64_t(blockIdx.x) * NM);
+        block_qlm_partial<L>(valid, q, 1.0f / float(n), sys_partial + uint64_t(blockIdx.x) * NM);
 }
 
 // Batched q_l: one workgroup per 256-point tile of one frame; bonds hold
@@ -305,15 +308,18 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
                                                          const uint32_t* __restrict__ offsets,
                                                          const float2* __restrict__ qlm, float2* __restrict__ qlmave,
                                                          float* __restrict__ qlave, int ql_stride, int ql_col,
-                                                         float2* __restrict__ sys_partial)
+                                                         float2* __restrict__ sys_partial,
+                                                         const uint32_t* __restrict__ order)
 {
     constexpr int NM = 2 * L + 1;
-    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    bool const valid = t0 < n;
+    uint32_t const i = (order && valid) ? order[t0] : t0;
     float2 acc[NM];
 #pragma unroll
     for (int t = 0; t < NM; ++t)
         acc[t] = make_float2(0.0f, 0.0f);
-    if (i < n)
+    if (valid)
     {
         uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
         for (uint32_t e = e0; e < e1; ++e)
@@ -347,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
     float const invn = 1.0f / float(n);
     for (int t = 0; t < NM; ++t)
     {
-        s_red[threadIdx.x] = (i < n) ? make_float2(acc[t].x * invn, acc[t].y * invn) : make_float2(0.0f, 0.0f);
+        s_red[threadIdx.x] = valid ? make_float2(acc[t].x * invn, acc[t].y * invn) : make_float2(0.0f, 0.0f);
         __syncthreads();
         for (int s = kBlock / 2; s > 0; s >>= 1)
         {
@@ -555,12 +561,14 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
                                                              const float* __restrict__ coef8,
                                                              const float* __restrict__ coef10,
                                                              const float* __restrict__ coef12, float* __restrict__ feat,
-                                                             float2* __restrict__ qlm6, float* __restrict__ ql6)
+                                                             float2* __restrict__ qlm6, float* __restrict__ ql6,
+                                                             const uint32_t* __restrict__ order)
 {
     constexpr int LM = 12;
-    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
+    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t0 >= n)
         return;
+    uint32_t const i = order ? order[t0] : t0;
     float2 acc[feat_base(14)];
 #pragma unroll
     for (int t = 0; t < feat_base(14); ++t)
@@ -651,26 +659,48 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
 
 // SolidLiquid.cc:51-78 (bond dot products) and :90-96 (solid-bond counts)
 // Solid-like bond count of particle i; writes each bond's q_ij when ql_ij is
-// not null.
+// not null.  NM = 2l + 1 known at compile time keeps q_lm(i) in registers and
+// issues the 2l + 1 loads of a neighbour's q_lm together; NM = 0: runtime l.
+template<int NM>
 __device__ inline uint32_t solid_bond_count(uint32_t i, int l, const uint32_t* __restrict__ nbr,
                                             const uint32_t* __restrict__ offsets, const float2* __restrict__ qlm,
                                             const float* __restrict__ ql, int normalize_q, float q_threshold,
                                             float* __restrict__ ql_ij)
 {
-    int const nm = 2 * l + 1;
+    int const nm = NM ? NM : 2 * l + 1;
     float const nf = float(4.0 * M_PI / nm);
     const float2* qi = qlm + uint64_t(i) * nm;
     uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
     uint32_t cnt = 0;
+    float2 a[NM ? NM : 1];
+    if (NM)
+    {
+#pragma unroll
+        for (int t = 0; t < NM; ++t)
+            a[t] = qi[t];
+    }
     for (uint32_t e = e0; e < e1; ++e)
     {
         uint32_t const j = nbr[2 * e + 1];
         const float2* qj = qlm + uint64_t(j) * nm;
         float re = 0.0f;
-        for (int t = 0; t < nm; ++t)
+        if (NM)
         {
-            float2 const a = qi[t], c = qj[t];
-            re += a.x * c.x + a.y * c.y;
+            float2 c[NM ? NM : 1];
+#pragma unroll
+            for (int t = 0; t < NM; ++t)
+                c[t] = qj[t];
+#pragma unroll
+            for (int t = 0; t < NM; ++t)
+                re += a[t].x * c[t].x + a[t].y * c[t].y;
+        }
+        else
+        {
+            for (int t = 0; t < nm; ++t)
+            {
+                float2 const av = qi[t], cv = qj[t];
+                re += av.x * cv.x + av.y * cv.y;
+            }
         }
         if (normalize_q)
             re *= nf / (ql[i] * ql[j]);
@@ -686,12 +716,15 @@ __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, 
                                                              const float2* __restrict__ qlm,
                                                              const float* __restrict__ ql, int normalize_q,
                                                              float q_threshold, float* __restrict__ ql_ij,
-                                                             uint32_t* __restrict__ num_conn)
+                                                             uint32_t* __restrict__ num_conn,
+                                                             const uint32_t* __restrict__ order)
 {
-    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
+    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t0 >= n)
         return;
-    num_conn[i] = solid_bond_count(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij);
+    uint32_t const i = order ? order[t0] : t0;
+    num_conn[i] = l == 6 ? solid_bond_count<13>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij)
+                         : solid_bond_count<0>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij);
 }
 
 // Learned SolidLiquid labels: a particle whose classifier logit is at least
@@ -705,17 +738,20 @@ __global__ __launch_bounds__(kBlock) void sl_decide_kernel(uint32_t n, const flo
                                                            const float* __restrict__ ql6, int normalize_q,
                                                            float q_threshold, uint32_t solid_threshold,
                                                            uint8_t* __restrict__ solid, uint8_t* __restrict__ fallback,
-                                                           uint32_t* __restrict__ n_fallback)
+                                                           uint32_t* __restrict__ n_fallback,
+                                                           const uint32_t* __restrict__ order)
 {
-    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool fb = false;
-    if (i < n)
+    if (t0 < n)
     {
+        uint32_t const i = order ? order[t0] : t0;
         float const y = logits[i];
         fb = !(fabsf(y) >= margin);
         uint8_t lab;
         if (fb)
-            lab = solid_bond_count(i, 6, nbr, offsets, qlm6, ql6, normalize_q, q_threshold, nullptr) >= solid_threshold;
+            lab = solid_bond_count<13>(i, 6, nbr, offsets, qlm6, ql6, normalize_q, q_threshold, nullptr) >=
+                  solid_threshold;
         else
             lab = y > 0.0f;
         solid[i] = lab;
@@ -797,21 +833,51 @@ __global__ void uf_union_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, c
 
 // Roots into a separate array with a read-only walk: path-halving writes here
 // could overwrite another thread's finished entry with a non-root ancestor.
-__global__ void uf_flatten_kernel(uint32_t n, const uint32_t* __restrict__ parent, uint32_t* __restrict__ root,
-                                  uint32_t* size)
+// One round of pointer jumping, parent <- grandparent.  Parallel hooking can
+// leave long chains (bonds (i, i+1) processed together hook i+1 under i for
+// every i); ceil(log2 n) + 1 rounds bring every point next to its root, so the
+// flatten below walks at most a step or two.  Parents only move to ancestors,
+// so the in-place updates are safe.
+__global__ void uf_jump_kernel(uint32_t n, uint32_t* parent)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    uint32_t x = i;
-    uint32_t p = parent[x];
-    while (p != x)
+    uint32_t const p = parent[i];
+    uint32_t const gp = parent[p];
+    if (gp != p)
+        parent[i] = gp;
+}
+
+// Root of every point, compressing paths in place as it goes: a node's parent
+// only ever moves to one of its ancestors, and roots never change, so the
+// concurrent halving of shared paths is safe and later threads walk the
+// shortened paths (a long hooking chain would otherwise cost O(depth) per
+// point).
+// Cluster sizes are counted with one atomic per distinct root in a wave: the
+// members of a large cluster would otherwise all add into one counter.
+__global__ void uf_flatten_kernel(uint32_t n, uint32_t* parent, uint32_t* __restrict__ root, uint32_t* size)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool pending = i < n;
+    uint32_t r = 0;
+    if (pending)
     {
-        x = p;
-        p = parent[x];
+        r = uf_find(parent, i);
+        root[i] = r;
     }
-    root[i] = x;
-    atomicAdd(&size[x], 1u);
+    uint32_t const lane = threadIdx.x & 63u;
+    while (__ballot(pending))
+    {
+        uint64_t const act = __ballot(pending);
+        uint32_t const leader = uint32_t(__ffsll(static_cast<unsigned long long>(act)) - 1);
+        uint32_t const lr = __shfl(r, int(leader));
+        uint64_t const same = __ballot(pending && r == lr);
+        if (lane == leader)
+            atomicAdd(&size[lr], uint32_t(__popcll(same)));
+        if (pending && r == lr)
+            pending = false;
+    }
 }
 
 // key = (n - size) << 32 | root sorts clusters by size descending, then by
@@ -851,11 +917,12 @@ __global__ __launch_bounds__(kBlock) void hexatic_kernel(fa::DevBox b, const flo
                                                          const uint32_t* __restrict__ nbr,
                                                          const float* __restrict__ wts,
                                                          const uint32_t* __restrict__ offsets, float k, int weighted,
-                                                         float2* __restrict__ psi)
+                                                         float2* __restrict__ psi, const uint32_t* __restrict__ order)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t0 >= n)
         return;
+    uint32_t const i = order ? order[t0] : t0;
     float re = 0.0f, im = 0.0f, total = 0.0f;
     for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e)
     {
@@ -915,47 +982,47 @@ float reduce_wigner3j_host(const std::complex<float>* src, int l)
 
 template<int L>
 void launch_qlm_l(const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted, int want_sys,
-                  float2* qlm, float* ql, int stride, int col, float2* part)
+                  float2* qlm, float* ql, int stride, int col, float2* part, const uint32_t* order)
 {
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     FA_KLAUNCH(qlm_kernel<L>, dim3(blocks), dim3(kBlock), 0, fa::stream(), db, pts, n, nl->neighbors,
-                       nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part);
+               nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part, order);
 }
 
 template<int L>
 void launch_ave_l(uint32_t n, const fa_nlist* nl, const float2* qlm, float2* qlmave, float* qlave, int stride, int col,
-                  float2* part)
+                  float2* part, const uint32_t* order)
 {
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     FA_KLAUNCH(qlm_ave_kernel<L>, dim3(blocks), dim3(kBlock), 0, fa::stream(), n, nl->neighbors, nl->offsets,
-                       qlm, qlmave, qlave, stride, col, part);
+               qlm, qlmave, qlave, stride, col, part, order);
 }
 
 template<int L>
 int dispatch_l(int l, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted,
-               int want_sys, float2* qlm, float* ql, int stride, int col, float2* part)
+               int want_sys, float2* qlm, float* ql, int stride, int col, float2* part, const uint32_t* order)
 {
     if (l == L)
     {
-        launch_qlm_l<L>(db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part);
+        launch_qlm_l<L>(db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part, order);
         return 1;
     }
     if constexpr (L < kLMax)
-        return dispatch_l<L + 1>(l, db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part);
+        return dispatch_l<L + 1>(l, db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part, order);
     return 0;
 }
 
 template<int L>
 int dispatch_ave(int l, uint32_t n, const fa_nlist* nl, const float2* qlm, float2* qlmave, float* qlave, int stride,
-                 int col, float2* part)
+                 int col, float2* part, const uint32_t* order)
 {
     if (l == L)
     {
-        launch_ave_l<L>(n, nl, qlm, qlmave, qlave, stride, col, part);
+        launch_ave_l<L>(n, nl, qlm, qlmave, qlave, stride, col, part, order);
         return 1;
     }
     if constexpr (L < kLMax)
-        return dispatch_ave<L + 1>(l, n, nl, qlm, qlmave, qlave, stride, col, part);
+        return dispatch_ave<L + 1>(l, n, nl, qlm, qlmave, qlave, stride, col, part, order);
     return 0;
 }
 
@@ -976,6 +1043,9 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
     hipStream_t s = stream();
     DevBox const db = make_devbox(*box);
     unsigned const blocks = (n + kBlock - 1) / kBlock;
+    const uint32_t* sorder = nullptr;
+    if (int rc = spatial_order(*box, d_points, n, &sorder))
+        return rc;
     size_t moff = 0;
     for (int k = 0; k < n_l; ++k)
     {
@@ -994,14 +1064,14 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         float2* qlm_l = reinterpret_cast<float2*>(d_qlm) + size_t(n) * moff;
         if (n)
         {
-            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part);
+            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part, sorder);
             FA_HIP(hipGetLastError());
         }
         float2* src = qlm_l;
         if (average && n)
         {
             float2* qlmave_l = reinterpret_cast<float2*>(d_qlmave) + size_t(n) * moff;
-            dispatch_ave<0>(l, n, nl, qlm_l, qlmave_l, d_qlave, n_l, k, part_ave);
+            dispatch_ave<0>(l, n, nl, qlm_l, qlmave_l, d_qlave, n_l, k, part_ave, sorder);
             FA_HIP(hipGetLastError());
             src = qlmave_l;
         }
@@ -1186,9 +1256,12 @@ int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const f
         return FA_OK;
     fa::DevBox const db = fa::make_devbox(*box);
     unsigned const blocks = (n + kBlock - 1) / kBlock;
+    const uint32_t* sorder = nullptr;
+    if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
+        return rc;
     FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), 0, fa::stream(), db, d_points, n, nl->neighbors,
                nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
-               reinterpret_cast<float2*>(d_qlm6), d_ql6);
+               reinterpret_cast<float2*>(d_qlm6), d_ql6, sorder);
     FA_HIP(hipGetLastError());
     return FA_OK;
 }
@@ -1209,7 +1282,7 @@ int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float ma
     if (n)
         FA_KLAUNCH(sl_decide_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), n, d_logits,
                    margin, nl->neighbors, nl->offsets, reinterpret_cast<const float2*>(d_qlm6), d_ql6, normalize_q,
-                   q_threshold, solid_threshold, d_solid, d_fallback, cnt);
+                   q_threshold, solid_threshold, d_solid, d_fallback, cnt, fa::cached_spatial_order(n));
     FA_HIP(hipGetLastError());
     if (n_fallback)
     {
@@ -1245,6 +1318,8 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
     if (nl->num_bonds)
         FA_KLAUNCH(uf_union_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            nl->num_bonds, nl->neighbors, d_keep, parent);
+    for (uint32_t span = 1; span < n; span *= 2)
+        FA_KLAUNCH(uf_jump_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent);
     FA_KLAUNCH(uf_flatten_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, root, size);
     FA_HIP(hipMemsetAsync(counter, 0, sizeof(uint32_t), s));
     FA_KLAUNCH(uf_keys_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size, counter, keys);
@@ -1281,10 +1356,13 @@ int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const f
         return rc;
     hipStream_t s = fa::stream();
     unsigned const blocks = (n + kBlock - 1) / kBlock;
+    const uint32_t* sorder = nullptr;
+    if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
+        return rc;
     if (n)
         FA_KLAUNCH(solid_bonds_kernel, dim3(blocks), dim3(kBlock), 0, s, n, int(l), nl->neighbors, nl->offsets,
-                           reinterpret_cast<const float2*>(d_qlm), d_ql, normalize_q, q_threshold, d_ql_ij,
-                           d_num_connections);
+                   reinterpret_cast<const float2*>(d_qlm), d_ql, normalize_q, q_threshold, d_ql_ij,
+                   d_num_connections, sorder);
     FA_HIP(hipGetLastError());
     void* keep = nullptr;
     if (int rc = fa::scratch(8, std::max<uint64_t>(nl->num_bonds, 1) + 64, &keep))
@@ -1308,8 +1386,11 @@ int fa_hexatic(const fa_box* box, const float* d_points, uint32_t n, const fa_nl
     if (n == 0)
         return FA_OK;
     fa::DevBox const db = fa::make_devbox(*box);
+    const uint32_t* sorder = nullptr;
+    if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
+        return rc;
     FA_KLAUNCH(hexatic_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), db, d_points, n,
-                       nl->neighbors, nl->weights, nl->offsets, float(k), weighted, reinterpret_cast<float2*>(d_psi));
+               nl->neighbors, nl->weights, nl->offsets, float(k), weighted, reinterpret_cast<float2*>(d_psi), sorder);
     FA_HIP(hipGetLastError());
     return FA_OK;
 }

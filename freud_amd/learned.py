@@ -75,12 +75,11 @@ class LearnedSolidLiquid(_PairCompute):
         check(lib().fa_mlp_set_norm(self._h, self._mu.ctypes.data, self._sigma.ctypes.data))
 
     # ------------------------------------------------------------ pieces
-    @staticmethod
-    def _features(nq, nlist):
+    def _features(self, nq, nlist):
         n = nq._n
-        feat = DeviceArray((n, 12), np.float32)
-        qlm6 = DeviceArray((n, 13, 2), np.float32)
-        ql6 = DeviceArray((n,), np.float32)
+        feat = _lib.reuse(self, "_d_feat", (n, 12), np.float32)
+        qlm6 = _lib.reuse(self, "_d_qlm6", (n, 13, 2), np.float32)
+        ql6 = _lib.reuse(self, "_d_ql6", (n,), np.float32)
         check(lib().fa_sl_features(ctypes.byref(nq._cbox), nq._d_points.ptr, n, ctypes.byref(nlist._c), feat.ptr,
                                    qlm6.ptr, ql6.ptr))
         return feat, qlm6, ql6
@@ -177,10 +176,10 @@ class LearnedSolidLiquid(_PairCompute):
         nlist = self._neighbor_list(nq, nlist, qargs, d_q, nqp)
         n = nq._n
         feat, qlm6, ql6 = self._features(nq, nlist)
-        logits = DeviceArray((n,), np.float32)
+        logits = _lib.reuse(self, "_d_logits", (n,), np.float32)
         check(lib().fa_mlp_forward(self._h, feat.ptr, n, logits.ptr))
-        solid = DeviceArray((max(n, 1),), np.uint8)
-        fb = DeviceArray((max(n, 1),), np.uint8)
+        solid = _lib.reuse(self, "_d_solid", (max(n, 1),), np.uint8)
+        fb = _lib.reuse(self, "_d_fb", (max(n, 1),), np.uint8)
         nfb = ctypes.c_uint32()
         check(lib().fa_sl_decide(n, ctypes.byref(nlist._c), logits.ptr, self._margin, qlm6.ptr, ql6.ptr,
                                  int(self._normalize_q), self._q_threshold, self._solid_threshold, solid.ptr, fb.ptr,

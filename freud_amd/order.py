@@ -9,7 +9,7 @@ from math import floor
 import numpy as np
 
 from freud_amd import _lib
-from freud_amd._lib import DeviceArray, check, lib
+from freud_amd._lib import check, lib
 from freud_amd.locality import _PairCompute
 from freud_amd.util import _Compute
 
@@ -64,11 +64,11 @@ class Steinhardt(_PairCompute):
         n = nq._n
         nl = len(self._ls)
         total_m = sum(2 * l + 1 for l in self._ls)
-        self._d_qlm = DeviceArray((n * total_m * 2,), np.float32)
-        self._d_ql = DeviceArray((n, nl), np.float32)
-        self._d_qlave = DeviceArray((n, nl), np.float32) if self._average else None
-        self._d_qlmave = DeviceArray((n * total_m * 2,), np.float32) if self._average else None
-        self._d_wl = DeviceArray((n, nl), np.float32) if self._wl else None
+        self._d_qlm = _lib.reuse(self, "_d_qlm", (n * total_m * 2,), np.float32)
+        self._d_ql = _lib.reuse(self, "_d_ql", (n, nl), np.float32)
+        self._d_qlave = _lib.reuse(self, "_d_qlave", (n, nl), np.float32) if self._average else None
+        self._d_qlmave = _lib.reuse(self, "_d_qlmave", (n * total_m * 2,), np.float32) if self._average else None
+        self._d_wl = _lib.reuse(self, "_d_wl", (n, nl), np.float32) if self._wl else None
         order = np.zeros(nl, np.float32)
         ls = np.asarray(self._ls, dtype=np.uint32)
         check(lib().fa_steinhardt(ctypes.byref(nq._cbox), nq._d_points.ptr, n, ctypes.byref(nlist._c),
@@ -157,12 +157,12 @@ class SolidLiquid(_PairCompute):
         nlist = self._neighbor_list(nq, nlist, qargs, d_q, nqp)
         n = nq._n
         nm = 2 * self._l + 1
-        self._d_qlm = DeviceArray((n, nm, 2), np.float32)
-        self._d_ql = DeviceArray((n,), np.float32)
-        self._d_qlij = DeviceArray((max(len(nlist), 1),), np.float32)
-        self._d_nc = DeviceArray((n,), np.uint32)
-        self._d_cidx = DeviceArray((n,), np.uint32)
-        self._d_sizes = DeviceArray((n,), np.uint32)
+        self._d_qlm = _lib.reuse(self, "_d_qlm", (n, nm, 2), np.float32)
+        self._d_ql = _lib.reuse(self, "_d_ql", (n,), np.float32)
+        self._d_qlij = _lib.reuse(self, "_d_qlij", (max(len(nlist), 1),), np.float32)
+        self._d_nc = _lib.reuse(self, "_d_nc", (n,), np.uint32)
+        self._d_cidx = _lib.reuse(self, "_d_cidx", (n,), np.uint32)
+        self._d_sizes = _lib.reuse(self, "_d_sizes", (n,), np.uint32)
         ncl = ctypes.c_uint32()
         check(lib().fa_solidliquid(ctypes.byref(nq._cbox), nq._d_points.ptr, n, ctypes.byref(nlist._c), self._l,
                                    self._q_threshold, self._solid_threshold, int(self._normalize_q),
@@ -241,7 +241,7 @@ class Hexatic(_PairCompute):
             msg = "A 3D box was provided to a class that only supports 2D systems."
             raise ValueError(msg)
         nlist = self._neighbor_list(nq, nlist, qargs, d_q, nqp)
-        self._d_psi = DeviceArray((nq._n, 2), np.float32)
+        self._d_psi = _lib.reuse(self, "_d_psi", (nq._n, 2), np.float32)
         check(lib().fa_hexatic(ctypes.byref(nq._cbox), nq._d_points.ptr, nq._n, ctypes.byref(nlist._c), self._k,
                                int(self._weighted), self._d_psi.ptr))
         return self

@@ -434,3 +434,27 @@ def test_stress_config_shapes(oracle):
     hx = freud_amd.order.Hexatic(6).compute((hbox, hpts))
     href = oracle.query(_bt(hbox), hpts, hpts, dict(num_neighbors=6, exclude_ii=True), nthreads=16)
     npt.assert_allclose(hx.particle_order, oracle.hexatic(_bt(hbox), hpts, href, k=6), atol=1e-5)
+
+
+def test_spatially_ordered_paths(oracle):
+    """Above 65536 points the gather kernels (q_lm, averaged q_lm, SolidLiquid
+    dots, learned features) run in a cached spatial order; per-particle results
+    must not change."""
+    box, pts, _ = freud_amd.data.two_phase(27, seed=8)  # 157,464 points, liquid half in random order
+    assert len(pts) >= 65536
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.5, exclude_ii=True), nthreads=16)
+    sl = freud_amd.order.SolidLiquid(6, 0.7, 6).compute((box, pts), neighbors=nl)
+    st = oracle.steinhardt(_bt(box), pts, ref_nl, [6], nthreads=16)
+    ref = oracle.solidliquid(ref_nl, st["qlm"][0], st["ql"][:, 0], 6, 0.7, 6)
+    npt.assert_allclose(sl.ql_ij, ref["ql_ij"], atol=1e-5)
+    near = np.abs(ref["ql_ij"] - 0.7) < 1e-5
+    ok = ~np.isin(np.arange(len(pts)), ref_nl["ij"][near, 0])
+    npt.assert_array_equal(sl.num_connections[ok], ref["num_connections"][ok])
+    if not near.any():
+        npt.assert_array_equal(sl.cluster_idx, ref["cluster_idx"])
+    for kw in (dict(average=True), dict(wl=True, wl_normalize=True)):
+        s2 = freud_amd.order.Steinhardt(6, **kw).compute((box, pts), neighbors=nl)
+        r2 = oracle.steinhardt(_bt(box), pts, ref_nl, [6], nthreads=16, **kw)
+        npt.assert_allclose(s2.particle_order, np.ravel(r2["particle_order"]), atol=1e-5)
+        npt.assert_allclose(s2.order, np.ravel(r2["order"])[0], atol=1e-5)
