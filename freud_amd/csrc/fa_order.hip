@@ -133,9 +133,50 @@ template<int L> __device__ inline void accumulate_ylm(float ct, float st, float 
 // Steinhardt.cc:120-222: q_lm(i) = sum_b w Y_lm / sum_b w and
 // q_l(i) = sqrt(4 pi / (2l+1) sum_m |q_lm|^2); also the block partial sums of
 // q_lm(i) / N for the system-wide order.
+// Where a kernel reads a bond's neighbour index and distance.  When a block's
+// points are consecutive (no processing order), their bonds form one
+// contiguous range of the NeighborList, which the block copies into LDS with
+// coalesced loads; each thread then walks its own bonds in LDS instead of
+// keeping a private strided stream through L2 (thousands of such streams per
+// CU thrash the cache).  Ranges longer than kStage bonds read global memory.
+constexpr uint32_t kStage = 4096;
+struct BondSrc
+{
+    const uint32_t* nbr;
+    const float* dist;
+    const uint32_t* sj;
+    const float* sd;
+    uint32_t base;
+    bool staged;
+    __device__ uint32_t j(uint32_t e) const { return staged ? sj[e - base] : nbr[2 * e + 1]; }
+    __device__ float d(uint32_t e) const { return staged ? sd[e - base] : dist[e]; }
+};
+
+// Block-uniform: stages the bonds of points [p0, p0 + nloc) (offsets-relative)
+// when they fit.  Every thread of the block must call it.
+__device__ inline BondSrc stage_bonds(uint32_t p0, uint32_t nloc, const uint32_t* __restrict__ offsets,
+                                      const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
+                                      uint32_t* s_j, float* s_d, bool allow)
+{
+    BondSrc src{nbr, dist, s_j, s_d, 0u, false};
+    if (!allow || nloc == 0)
+        return src;
+    uint32_t const b0 = offsets[p0], b1 = offsets[p0 + nloc];
+    if (b1 - b0 > kStage)
+        return src;
+    for (uint32_t t = threadIdx.x; t < b1 - b0; t += blockDim.x)
+    {
+        s_j[t] = reinterpret_cast<const uint2*>(nbr)[b0 + t].y;
+        s_d[t] = dist[b0 + t];
+    }
+    __syncthreads();
+    src.base = b0;
+    src.staged = true;
+    return src;
+}
+
 template<int L>
-__device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i,
-                                  const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
+__device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i, const BondSrc& src,
                                   const float* __restrict__ wts, const uint32_t* __restrict__ offsets, int weighted,
                                   float* q)
 {
@@ -148,10 +189,10 @@ __device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__
     uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
     for (uint32_t e = e0; e < e1; ++e)
     {
-        uint32_t const j = nbr[2 * e + 1];
+        uint32_t const j = src.j(e);
         float const w = weighted ? wts[e] : 1.0f;
         float ct, st, cp, sp;
-        bond_angles(b, pts, i, j, dist[e], ct, st, cp, sp);
+        bond_angles(b, pts, i, j, src.d(e), ct, st, cp, sp);
         accumulate_ylm<L>(ct, st, cp, sp, w, acc);
         total_w += w;
     }
@@ -211,13 +252,17 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
                                                      const uint32_t* __restrict__ order)
 {
     constexpr int NM = 2 * L + 1;
-    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_j[kStage];
+    __shared__ float s_d[kStage];
+    uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    uint32_t const t0 = tile0 + threadIdx.x;
     bool const valid = t0 < n;
     uint32_t const i = (order && valid) ? order[t0] : t0;
+    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, s_j, s_d, order == nullptr);
     float q[2 * NM];
     if (valid)
     {
-        float const v = qlm_point<L>(b, pts, i, nbr, dist, wts, offsets, weighted, q);
+        float const v = qlm_point<L>(b, pts, i, src, wts, offsets, weighted, q);
         float2* out = qlm + uint64_t(i) * NM;
 #pragma unroll
         for (int t = 0; t < NM; ++t)
@@ -244,15 +289,18 @@ __global__ __launch_bounds__(kBlock) void bqlm_kernel(fa::BatchGrid G, const uin
     // bonds beyond the allocation were not written (the host reruns)
     if (*total_bonds > capacity)
         return;
+    __shared__ uint32_t s_j[kStage];
+    __shared__ float s_d[kStage];
     uint32_t const t = fa::xcd_tile(blockIdx.x, ntiles);
     const fa::BatchFrame& F = G.frames[tile_frame[t]];
-    uint32_t const i = (t - F.tile_base) * fa::kTile + threadIdx.x;
+    uint32_t const i0 = (t - F.tile_base) * fa::kTile;
+    uint32_t const i = i0 + threadIdx.x;
     bool const valid = i < F.n;
+    BondSrc const src = stage_bonds(i0, min(fa::kTile, F.n - i0), offsets + F.pt_base, nbr, dist, s_j, s_d, true);
     float q[2 * NM];
     if (valid)
     {
-        float const v = qlm_point<L>(F.box, pts + 3 * size_t(F.pt_base), i, nbr, dist, nullptr, offsets + F.pt_base, 0,
-                                     q);
+        float const v = qlm_point<L>(F.box, pts + 3 * size_t(F.pt_base), i, src, nullptr, offsets + F.pt_base, 0, q);
         float2* out = qlm + uint64_t(F.pt_base + i) * NM;
 #pragma unroll
         for (int m = 0; m < NM; ++m)
@@ -565,7 +613,11 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
                                                              const uint32_t* __restrict__ order)
 {
     constexpr int LM = 12;
-    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_j[kStage];
+    __shared__ float s_d[kStage];
+    uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    uint32_t const t0 = tile0 + threadIdx.x;
+    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, s_j, s_d, order == nullptr);
     if (t0 >= n)
         return;
     uint32_t const i = order ? order[t0] : t0;
@@ -577,7 +629,7 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
     for (uint32_t e = e0; e < e1; ++e)
     {
         float ct, st, cp, sp;
-        bond_angles(b, pts, i, nbr[2 * e + 1], dist[e], ct, st, cp, sp);
+        bond_angles(b, pts, i, src.j(e), src.d(e), ct, st, cp, sp);
         float pmm = 0.28209479177387814f;
         float er = 1.0f, ei = 0.0f;
 #pragma unroll
