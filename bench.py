@@ -127,9 +127,16 @@ def measured_traffic(kernel):
     path = os.path.join(ROOT, "profiles", "hbm_traffic_frame_1M.json")
     try:
         with open(path) as fh:
-            rec = json.load(fh).get("kernels", {}).get(kernel)
+            table = json.load(fh).get("kernels", {})
     except (OSError, ValueError):
         return None
+    name = kernel.strip("()")
+    rec = table.get(name)
+    if rec is None:
+        # the library names a template family once (qlm_kernel<L>); rocprof names the instance
+        base = name.split("<")[0]
+        hits = [v for k, v in table.items() if k.split("<")[0] == base]
+        rec = hits[0] if len(hits) == 1 else None
     return None if rec is None else int(rec["fetch_bytes"] + rec["write_bytes"])
 
 

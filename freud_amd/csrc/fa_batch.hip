@@ -175,6 +175,12 @@ int make_grid(fa_batch* b, float r, int R, BatchGrid& g)
         F.box = fa::make_devbox(b->boxes[f]);
         uint32_t const n = uint32_t(b->pt_off[f + 1] - b->pt_off[f]);
         fa::grid_dims_for(b->boxes[f], n, r, R, F.dim);
+        {
+            float npd[3];
+            fa::nearest_plane_distance(b->boxes[f], npd);
+            for (int k = 0; k < 3; ++k)
+                F.cw[k] = (k == 2 && b->boxes[f].is2D) ? 0.0f : npd[k] / float(F.dim[k]);
+        }
         F.cell_base = uint32_t(cells);
         F.row_base = uint32_t(rows);
         F.pt_base = uint32_t(b->pt_off[f]);

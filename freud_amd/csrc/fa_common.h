@@ -61,6 +61,7 @@ enum TunedAnalysis
 {
     TUNE_RDF_SELF = 1, // variant = cells r_max / R wide, R = 2 + variant
     TUNE_BALL = 2,     // variant = cells r_max / R wide, R = 1 + variant
+    TUNE_KNN = 3,      // variant = cells rk, rk / 1.5, rk / 2 wide (rk: radius holding ~2k points)
 };
 bool tuning_enabled();
 int tuned_variant(int analysis, int key); // -1 when not tuned yet
@@ -176,6 +177,7 @@ struct DevGrid
     const uint32_t* cstart; // ncells + 1
     int dim[3];
     int any_outside;
+    float cw[3]; // cell extent along each plane normal (nearest plane distance / dim)
 };
 
 __device__ __host__ inline int floor_div(int a, int b)
@@ -385,6 +387,7 @@ struct Grid
     uint32_t* cstart = nullptr;
     uint32_t* cell_of_point = nullptr; // scratch
     int any_outside = 0;
+    float cw[3] = {0.0f, 0.0f, 0.0f};
     DevGrid dev() const
     {
         DevGrid g;
@@ -395,6 +398,9 @@ struct Grid
         g.dim[1] = dim[1];
         g.dim[2] = dim[2];
         g.any_outside = any_outside;
+        g.cw[0] = cw[0];
+        g.cw[1] = cw[1];
+        g.cw[2] = cw[2];
         return g;
     }
     ~Grid();
@@ -445,6 +451,7 @@ struct BatchFrame
     uint32_t n;         // points in the frame
     uint32_t tile_base; // first tile of the frame
     int any_outside;    // set by the grid build when a point lies outside the box
+    float cw[3];        // cell extent along each plane normal
 };
 
 // One cell grid over every frame of a batch (device arrays).
@@ -469,6 +476,9 @@ __device__ inline DevGrid frame_grid(const BatchFrame& f, const BatchGrid& G)
     g.dim[1] = f.dim[1];
     g.dim[2] = f.dim[2];
     g.any_outside = f.any_outside;
+    g.cw[0] = f.cw[0];
+    g.cw[1] = f.cw[1];
+    g.cw[2] = f.cw[2];
     return g;
 }
 

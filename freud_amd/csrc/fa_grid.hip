@@ -283,6 +283,12 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
     g.dim[1] = dims[1];
     g.dim[2] = dims[2];
     g.n = n;
+    {
+        float npd[3];
+        nearest_plane_distance(box, npd);
+        for (int k = 0; k < 3; ++k)
+            g.cw[k] = (k == 2 && box.is2D) ? 0.0f : npd[k] / float(dims[k]);
+    }
     // scratch: counts (ncell) + fill (ncell) + any_outside flag
     void* tmp = nullptr;
     size_t const cb = sizeof(uint32_t) * size_t(ncell);

@@ -972,39 +972,73 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
     if (nq->kind == FA_NQ_AABB)
     {
         rlim = knn_radius_limit(nq, a);
-        // cells sized so that the first ring typically holds ~2k points
+        // cells a fraction of the radius rk that holds ~2k points; the
+        // autotuner picks the fraction (rk, rk/1.5, rk/2): smaller cells mean
+        // fewer candidates but more stencil cells per query
         float const vol = fa::box_volume(nq->box);
         double const dens = double(nq->n) / std::max(double(vol), 1e-30);
-        double const dimn = nq->box.is2D ? 2.0 : 3.0;
-        double rk = nq->box.is2D ? std::sqrt(2.0 * k / (M_PI * dens)) : std::cbrt(3.0 * 2.0 * k / (4.0 * M_PI * dens));
-        (void) dimn;
-        float const wcell = float(std::max(rk, 1e-6));
-        int dims[3];
-        fa::grid_dims_for(nq->box, nq->n, wcell, 1, dims);
-        if (int rc = fa::ensure_grid(nq, dims))
-            return rc;
-        dbp = &nq->cur->dbox;
-        float wmin = std::numeric_limits<float>::infinity();
-        for (int d = 0; d < (nq->box.is2D ? 2 : 3); ++d)
-            wmin = std::min(wmin, npd[d] / float(dims[d]));
-        int const rmax_ring = std::max(dims[0], std::max(dims[1], dims[2])) * 2 + 2;
-        int rc;
-        if (k <= 8)
-            rc = launch_knn_aabb<8>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                    min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
-        else if (k <= 16)
-            rc = launch_knn_aabb<16>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
-        else if (k <= 32)
-            rc = launch_knn_aabb<32>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
-        else if (k <= 64)
-            rc = launch_knn_aabb<64>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin,
-                                     min_pd / 2.0f, rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n);
+        double const rk =
+            nq->box.is2D ? std::sqrt(2.0 * k / (M_PI * dens)) : std::cbrt(3.0 * 2.0 * k / (4.0 * M_PI * dens));
+        static constexpr double kFrac[3] = {1.0, 1.0 / 1.5, 0.5};
+        auto run_variant = [&](int v) -> int {
+            float const wcell = float(std::max(rk * kFrac[v], 1e-6));
+            int dims[3];
+            fa::grid_dims_for(nq->box, nq->n, wcell, 1, dims);
+            if (int rc = fa::ensure_grid(nq, dims))
+                return rc;
+            dbp = &nq->cur->dbox;
+            float wmin = std::numeric_limits<float>::infinity();
+            for (int d = 0; d < (nq->box.is2D ? 2 : 3); ++d)
+                wmin = std::min(wmin, npd[d] / float(dims[d]));
+            int const rmax_ring = std::max(dims[0], std::max(dims[1], dims[2])) * 2 + 2;
+            FA_HIP(hipMemsetAsync(nflag, 0, sizeof(uint32_t), s));
+#define FA_KNN_LAUNCH(KM)                                                                                     \
+    launch_knn_aabb<KM>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin, min_pd / 2.0f, \
+                        rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n)
+            if (k <= 8)
+                return FA_KNN_LAUNCH(8);
+            if (k <= 12)
+                return FA_KNN_LAUNCH(12);
+            if (k <= 16)
+                return FA_KNN_LAUNCH(16);
+            if (k <= 32)
+                return FA_KNN_LAUNCH(32);
+            return FA_KNN_LAUNCH(64);
+#undef FA_KNN_LAUNCH
+        };
+        int rc = FA_OK;
+        if (k <= 64)
+        {
+            int const key = fa::tune_key(dens, rk) * 128 + int(std::min<uint32_t>(k, 127u));
+            int variant = fa::tuned_variant(fa::TUNE_KNN, key);
+            if (variant < 0 && fa::tuning_enabled())
+            {
+                float best = 0.0f;
+                for (int v = 0; v < 3; ++v)
+                {
+                    float ms = 0.0f;
+                    if (int rc2 = run_variant(v)) // builds the grid
+                        return rc2;
+                    if (int rc2 = fa::time_launch([&] { return run_variant(v); }, &ms))
+                        return rc2;
+                    if (variant < 0 || ms < best)
+                    {
+                        best = ms;
+                        variant = v;
+                    }
+                }
+                fa::record_variant(fa::TUNE_KNN, key, variant);
+            }
+            rc = run_variant(std::max(variant, 0));
+        }
         else
         {
             // every query takes the exhaustive path
-            rc = FA_OK;
+            int dims[3];
+            fa::grid_dims_for(nq->box, nq->n, float(std::max(rk, 1e-6)), 1, dims);
+            if (int rc2 = fa::ensure_grid(nq, dims))
+                return rc2;
+            dbp = &nq->cur->dbox;
             std::vector<uint32_t> all(n_query);
             for (uint32_t t = 0; t < n_query; ++t)
                 all[t] = t;

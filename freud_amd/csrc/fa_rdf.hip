@@ -108,6 +108,29 @@ int thresholds(uint32_t nbins, float vmin, float vmax, float** d_out)
     return FA_OK;
 }
 
+// The reference's bin of an r_sq already known to lie in [T[0], T[nbins]).
+// The hardware square root is within an ulp of the correctly rounded one, so
+// the estimate is within one bin of the reference's unless bins are narrower
+// than that rounding (far beyond 65536 bins): one compare with each
+// neighbouring threshold then gives the exact bin.
+__device__ inline uint32_t bin_in_range(float x, const float* __restrict__ T, uint32_t nbins, float vmin,
+                                        float inv_width)
+{
+    float const d = __builtin_amdgcn_sqrtf(x);
+    float const v = (d - vmin) * inv_width;
+    int k = int(fminf(fmaxf(v, 0.0f), float(nbins - 1)));
+    float const t0 = T[k], t1 = T[k + 1];
+    k += int(x >= t1) - int(x < t0);
+    if (nbins > 65536u)
+    {
+        while (k > 0 && x < T[k])
+            --k;
+        while (k < int(nbins) - 1 && x >= T[k + 1])
+            ++k;
+    }
+    return uint32_t(k);
+}
+
 // The reference's bin of r_sq, given T in LDS; nbins when outside [T0, Tn).
 __device__ inline uint32_t bin_of_rsq(float x, const float* __restrict__ T, uint32_t nbins, float vmin,
                                       float inv_width)
@@ -231,6 +254,14 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
     constexpr int kRowsY = 2 * R + 1;
     uint32_t const lane = threadIdx.x & 63u;
     int const dim0 = g.dim[0], dim1 = g.dim[1], dim2 = g.dim[2];
+    // pairs counted: q_rmin_sq <= r_sq < q_rmax_sq (the query) and inside the histogram
+    float const lo = fmaxf(q_rmin_sq, sT[0]), hi = fminf(q_rmax_sq, sT[nbins]);
+    // row trimming: a candidate row (dy, dz) is at least (|dy|-1) cell extents away
+    // along y and (|dz|-1) along z; conservative margins cover the rounding of
+    // the cell assignment
+    float const r_lim = sqrtf(q_rmax_sq) * 1.0001f;
+    float const wx = g.cw[0] * 0.9999f, wy = g.cw[1] * 0.9999f, wz = g.cw[2] * 0.9999f;
+    bool const ortho = b.xy == 0.0f && b.xz == 0.0f && b.yz == 0.0f;
     uint32_t const A0 = e.y, na = e.z;
     int const cx_lo = int(e.w & 0xffffu), cx_hi = int(e.w >> 16);
     int const cy0 = int(e.x % uint32_t(dim1));
@@ -254,13 +285,25 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
             int const dz = rt / kRowsY - Rz, dy = rt % kRowsY - R;
             int const uz = cz0 + dz, uy = cy0 + dy;
             int const sz = fa::floor_div(uz, dim2), sy = fa::floor_div(uy, dim1);
-            if (sz >= -1 && sz <= 1 && sy >= -1 && sy <= 1)
+            float const gy = float(max(0, abs(dy) - 1)) * wy, gz = float(max(0, abs(dz) - 1)) * wz;
+            int xr = R;
+            bool reach = true;
+            if (ortho)
+            {
+                float const rem = r_lim * r_lim - (gy * gy + gz * gz);
+                reach = rem > 0.0f;
+                if (reach && wx > 0.0f)
+                    xr = min(R, int(sqrtf(rem) / wx) + 1);
+            }
+            else
+                reach = fmaxf(gy, gz) < r_lim;
+            if (reach && sz >= -1 && sz <= 1 && sy >= -1 && sy <= 1)
             {
                 uint32_t const rowbase = uint32_t((uz - sz * dim2) * dim1 + (uy - sy * dim1)) * uint32_t(dim0);
                 // shifts beyond one box length are not images the reference visits,
                 // so the clipped range has at most 3 pieces (shift -1, 0, 1)
-                int ux = max(cx_lo - R, -dim0);
-                int const uxend = min(cx_hi + R, 2 * dim0 - 1);
+                int ux = max(cx_lo - xr, -dim0);
+                int const uxend = min(cx_hi + xr, 2 * dim0 - 1);
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
                 {
@@ -314,12 +357,8 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
                     float const ry = cur.y - qy;
                     float const rz = pjz - qz;
                     float const x = (rx * rx + ry * ry) + rz * rz;
-                    if (valid && x < q_rmax_sq && x >= q_rmin_sq)
-                    {
-                        uint32_t const bin = bin_of_rsq(x, sT, nbins, vmin, inv_width);
-                        if (bin < nbins)
-                            atomicAdd(&h[bin], weight);
-                    }
+                    if (valid && x >= lo && x < hi)
+                        atomicAdd(&h[bin_in_range(x, sT, nbins, vmin, inv_width)], weight);
                     cur = nxt;
                 }
             }

@@ -458,3 +458,23 @@ def test_spatially_ordered_paths(oracle):
         r2 = oracle.steinhardt(_bt(box), pts, ref_nl, [6], nthreads=16, **kw)
         npt.assert_allclose(s2.particle_order, np.ravel(r2["particle_order"]), atol=1e-5)
         npt.assert_allclose(s2.order, np.ravel(r2["order"])[0], atol=1e-5)
+
+
+@pytest.mark.parametrize("name,box,pts", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
+def test_knn_cell_variants(oracle, name, box, pts):
+    """Every kNN cell size the autotuner can pick gives the same NeighborList."""
+    from freud_amd import _lib
+
+    for k in (6, 12):
+        ref = oracle.query(_bt(box), pts, pts, dict(num_neighbors=k, exclude_ii=True), nthreads=8)
+        dens = len(pts) / box.volume
+        rk = np.sqrt(2.0 * k / (np.pi * dens)) if box.is2D else np.cbrt(3.0 * 2.0 * k / (4.0 * np.pi * dens))
+        key = int(np.floor(2.0 * np.log2(dens * rk ** 3) + 0.5)) * 128 + k
+        before = _lib.tune_table()
+        try:
+            for v in range(3):
+                _lib.tune_load({(3, key): v})
+                nl = freud_amd.AABBQuery(box, pts).query(pts, dict(num_neighbors=k, exclude_ii=True)).toNeighborList()
+                _assert_nlist_equal(nl, ref)
+        finally:
+            _lib.tune_load({(3, key): before.get((3, key), -1)})
