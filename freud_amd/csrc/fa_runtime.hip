@@ -2,7 +2,9 @@
 // box-transform kernel of the freud_amd library.
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "fa_common.h"
@@ -122,19 +124,90 @@ static int ensure_init()
 
 hipStream_t stream() { return g_stream; }
 
+// Caching device allocator.  hipFree synchronises the whole device and
+// hipMalloc is slow, and a frame-by-frame analysis (a new NeighborQuery with its
+// cell grids per frame) would pay both every frame.  Freed blocks go to a
+// size-keyed cache instead and are handed out again.  All library work runs in
+// order on the library stream, so a block reused by a later call is only
+// touched after the earlier call's kernels finished.  Sizes are rounded up
+// (256 B granules up to 1 MiB, 2 MiB granules above) and a cached block is
+// reused for requests between half and all of its size.  At most
+// kCacheLimit bytes stay cached; an allocation failure first returns the
+// cache to HIP and retries.
+namespace {
+std::multimap<size_t, void*> g_cache;
+std::unordered_map<void*, size_t> g_live;
+size_t g_cached_bytes = 0;
+constexpr size_t kCacheLimit = size_t(16) << 30;
+
+size_t round_size(size_t b)
+{
+    if (b <= (size_t(1) << 20))
+        return (b + 255) & ~size_t(255);
+    return (b + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+}
+
+void release_cache()
+{
+    if (g_cache.empty())
+        return;
+    (void) hipStreamSynchronize(g_stream);
+    for (auto& kv : g_cache)
+        (void) hipFree(kv.second);
+    g_cache.clear();
+    g_cached_bytes = 0;
+}
+} // namespace
+
 int device_alloc(void** p, size_t bytes)
 {
     *p = nullptr;
-    if (bytes == 0)
-        bytes = 16;
-    FA_HIP(hipMalloc(p, bytes));
+    size_t const want = round_size(bytes == 0 ? 16 : bytes);
+    auto it = g_cache.lower_bound(want);
+    if (it != g_cache.end() && it->first <= 2 * want)
+    {
+        *p = it->second;
+        g_live[*p] = it->first;
+        g_cached_bytes -= it->first;
+        g_cache.erase(it);
+        return FA_OK;
+    }
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess)
+    {
+        (void) hipGetLastError();
+        release_cache();
+        e = hipMalloc(p, want);
+    }
+    if (e != hipSuccess)
+    {
+        *p = nullptr;
+        return fail(FA_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    g_live[*p] = want;
     return FA_OK;
 }
 
 void device_free(void* p)
 {
-    if (p)
+    if (!p)
+        return;
+    auto it = g_live.find(p);
+    if (it == g_live.end())
+    {
         (void) hipFree(p);
+        return;
+    }
+    size_t const sz = it->second;
+    g_live.erase(it);
+    if (g_cached_bytes + sz > kCacheLimit)
+    {
+        (void) hipStreamSynchronize(g_stream);
+        (void) hipFree(p);
+        return;
+    }
+    g_cache.emplace(sz, p);
+    g_cached_bytes += sz;
 }
 
 int scratch(size_t slot, size_t bytes, void** ptr)
@@ -267,8 +340,7 @@ int fa_malloc(void** d_ptr, size_t bytes)
 
 int fa_free(void* d_ptr)
 {
-    if (d_ptr)
-        FA_HIP(hipFree(d_ptr));
+    fa::device_free(d_ptr);
     return FA_OK;
 }
 

@@ -96,6 +96,10 @@ typedef struct fa_nq fa_nq;
 const char* fa_last_error(void);
 int fa_init(void);
 int fa_device_name(char* buf, int len);
+/* Device memory from the library's caching allocator (no counterpart in the
+ * reference).  fa_free returns the block to the cache without synchronising;
+ * work that uses it must be ordered on the library stream (all entry points
+ * are) or have completed. */
 int fa_malloc(void** d_ptr, size_t bytes);
 int fa_free(void* d_ptr);
 int fa_memcpy_h2d(void* d_dst, const void* src, size_t bytes);
