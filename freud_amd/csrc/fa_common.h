@@ -497,7 +497,7 @@ struct BatchRdfScratch
     uint32_t* nclus;  // total_rows + 1
     uint32_t* cl_off; // total_rows + 1
     uint4* table;     // max_clusters
-    uint32_t* queue;  // 1 work-queue counter
+    uint32_t* queue;  // kXcds work-queue counters
     const float* T;   // bins + 1 r_sq thresholds (thresholds_for)
     void* scan_tmp;
     size_t scan_bytes;

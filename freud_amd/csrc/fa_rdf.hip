@@ -652,11 +652,15 @@ __device__ inline void flush_wave_hist(uint32_t* wh, uint32_t nbins, uint32_t* _
 
 // Persistent batched RDF: every wave takes chunks of kChunk cluster entries
 // from a device work queue until the batch is exhausted, so frames of any size
-// share one launch and the GPU stays full to the end.  Each wave bins into its
-// own LDS histogram and flushes it to its frame's counts (bins per frame) when
-// the frame changes and at exit.  Frames with points outside their box take
-// the generic per-point search for the cluster's points (weight 1 per ordered
-// pair).
+// share one launch and the GPU stays full to the end.  There is one queue per
+// XCD over a contiguous eighth of the clusters (workgroups are dispatched
+// round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8): the waves of
+// an XCD work through neighbouring clusters and share its L2; a wave whose
+// range is exhausted takes chunks from the other XCDs' queues.  Each wave bins
+// into its own LDS histogram and flushes it to its frame's counts (bins per
+// frame) when the frame changes and at exit.  Frames with points outside their
+// box take the generic per-point search for the cluster's points (weight 1 per
+// ordered pair).
 constexpr uint32_t kChunk = 4;
 
 template<int R>
@@ -678,15 +682,23 @@ __global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, cons
     __syncthreads();
     uint32_t const ncl = *ncl_ptr;
     uint32_t cur = 0xffffffffu;
-    for (;;)
+    uint32_t const x0 = blockIdx.x % fa::kXcds;
+    for (uint32_t xi = 0; xi < fa::kXcds;)
     {
-        uint32_t base = 0;
+        uint32_t const x = (x0 + xi) % fa::kXcds;
+        uint32_t const r_lo = uint32_t((uint64_t(ncl) * x) / fa::kXcds);
+        uint32_t const r_hi = uint32_t((uint64_t(ncl) * (x + 1)) / fa::kXcds);
+        uint32_t off = 0;
         if (lane == 0)
-            base = atomicAdd(queue, kChunk);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= ncl)
-            break;
-        uint32_t const stop = min(base + kChunk, ncl);
+            off = atomicAdd(&queue[x], kChunk);
+        off = __builtin_amdgcn_readfirstlane(off);
+        uint32_t const base = r_lo + off;
+        if (base >= r_hi)
+        {
+            ++xi; // this range is done: help the next XCD's
+            continue;
+        }
+        uint32_t const stop = min(base + kChunk, r_hi);
         for (uint32_t I = base; I < stop; ++I)
         {
             uint32_t const f = cl_frame[I];
@@ -753,7 +765,7 @@ int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const Batch
     if (S.max_clusters)
         FA_KLAUNCH(brow_table_kernel, dim3(tblocks), dim3(kBlock), 0, s, G, S.cl_off, uint32_t(S.max_clusters),
                    S.table, cl_frame);
-    if (int rc = zero_u32(S.queue, 1))
+    if (int rc = zero_u32(S.queue, kXcds))
         return rc;
     // persistent grid: enough workgroups to fill every CU a few times over
     static int n_cu = 0;
