@@ -201,7 +201,7 @@ int make_grid(fa_batch* b, float r, int R, BatchGrid& g)
     size_t const np = std::max<size_t>(b->total, 1);
     if (int rc = fa::device_alloc((void**) &g.frames, sizeof(BatchFrame) * std::max<size_t>(fr.size(), 1)))
         return rc;
-    if (int rc = fa::device_alloc((void**) &g.cpos, sizeof(float4) * np))
+    if (int rc = fa::device_alloc((void**) &g.cpos, sizeof(float4) * (np + fa::kCposPad)))
         return rc;
     if (int rc = fa::device_alloc((void**) &g.wrapc, sizeof(int) * np))
         return rc;
@@ -583,7 +583,7 @@ int fa_batch_create(uint32_t n_frames, const fa_box* boxes, const uint64_t* poin
         return fail_out(fa::fail(FA_ERR_DEVICE, "hipHostMalloc failed"));
     if (do_rdf(b))
     {
-        if (int rc = fa::device_alloc((void**) &b->d_T, sizeof(float) * (plan->rdf_bins + 1)))
+        if (int rc = fa::device_alloc((void**) &b->d_T, sizeof(float) * fa::rdf_table_floats(plan->rdf_bins)))
             return fail_out(rc);
         if (int rc = fa::thresholds_for(plan->rdf_bins, plan->rdf_r_max, b->d_T))
             return fail_out(rc);

@@ -169,7 +169,11 @@ __host__ __device__ inline void wrap(const DevBox& b, float vx, float vy, float 
 // float4 (x, y, z, bits(j)); cells are x-fastest so the cells of one stencil
 // row are contiguous in memory.  `wrapc` packs each point's integer image
 // offset floor(f) (10 bits per axis) and is only read when some point lies
-// outside the box.
+// outside the box.  The cell-ordered point arrays carry kCposPad entries past
+// the last point, so a kernel may load whole 8-point chunks beyond the end of a
+// range (the extra points are masked off) without clamping every address.
+constexpr uint32_t kCposPad = 64;
+
 struct DevGrid
 {
     const float4* cpos;
@@ -507,6 +511,8 @@ int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const Batch
 // The bins + 1 r_sq thresholds of RegularAxis(bins, 0, r_max) binning of
 // sqrt(r_sq), computed on the host and copied (synchronously) to d_T.
 int thresholds_for(uint32_t bins, float r_max, float* d_T);
+// Floats of the bin table thresholds_for writes (thresholds + lookup table).
+size_t rdf_table_floats(uint32_t bins);
 int batch_steinhardt(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts,
                      const uint32_t* nb, const float* dist, const uint32_t* d_offsets,
                      const unsigned long long* d_total, uint64_t capacity, uint32_t l, float* d_qlm, float* d_ql,

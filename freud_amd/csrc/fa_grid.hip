@@ -263,7 +263,7 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
         g.cpos = nullptr;
         g.wrapc = nullptr;
         g.cell_of_point = nullptr;
-        if (int rc = device_alloc((void**) &g.cpos, sizeof(float4) * size_t(n)))
+        if (int rc = device_alloc((void**) &g.cpos, sizeof(float4) * (size_t(n) + kCposPad)))
             return rc;
         if (int rc = device_alloc((void**) &g.wrapc, sizeof(int) * size_t(n)))
             return rc;

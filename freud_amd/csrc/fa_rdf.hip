@@ -24,6 +24,9 @@
 // directions because (q + image) rounds differently for the two directions.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "fa_common.h"
@@ -36,6 +39,8 @@ using fa::DevGrid;
 using fa::LINKCELL_FORMULA;
 
 constexpr int kBlock = 256;
+// word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format)
+constexpr int kBufferWord3 = 0x00020000;
 constexpr uint32_t kMaxLdsBins = 4096;
 
 // ------------------------------------------------------------------ thresholds
@@ -71,10 +76,104 @@ float first_rsq_with_bin_at_least(int k, float vmin, float vmax, float inv_width
     return __builtin_bit_cast(float, lo);
 }
 
+// Bin table: the nbins + 1 thresholds T, then inv_delta and x_fast (floats),
+// then a kLut-entry lookup table over r_sq in [0, T[nbins]).  The cell of an
+// r_sq x is uint32(x * inv_delta); inv_delta is the largest float not above
+// kLut / T[nbins] for which every x < T[nbins] has a cell below kLut.  A cell's
+// entry is a bin k such that every x of the cell inside [T[0], T[nbins]) has
+// bin k or k + 1, so one compare with T[k + 1] gives the reference's bin.
+// Cells that span more bins (near r = 0, where bins are thin in r_sq) hold
+// kLutNone; they all lie below x_fast (the first r_sq of the cell after the
+// last such cell), and r_sq below x_fast take the square-root path.  Entries
+// are found on the host by bisection over float bit patterns with the device's
+// own float expression for the cell.
+constexpr uint32_t kLut = 4096;
+constexpr uint16_t kLutNone = 0xffffu;
+constexpr uint32_t kLutMaxBins = 16382; // LDS entries hold 4 * (k + 1) < kLutNone
+
+uint32_t lut_cell(float x, float inv_delta)
+{
+    volatile float p = x * inv_delta;
+    float const q = p;
+    return q >= float(kLut) ? kLut : uint32_t(q);
+}
+
+__host__ __device__ constexpr uint32_t table_floats(uint32_t nbins)
+{
+    return nbins + 3 + uint32_t((kLut * sizeof(uint16_t) + sizeof(float) - 1) / sizeof(float));
+}
+
+std::vector<float> build_table(uint32_t nbins, float vmin, float vmax)
+{
+    float const width = (vmax - vmin) / float(nbins);
+    float const inv_width = 1.0f / width;
+    std::vector<float> tab(table_floats(nbins), 0.0f);
+    float* T = tab.data();
+    for (uint32_t k = 0; k <= nbins; ++k)
+        T[k] = first_rsq_with_bin_at_least(int(k), vmin, vmax, inv_width, nbins);
+    float const tn = T[nbins];
+    float inv_delta = 0.0f;
+    if (tn > 0.0f && std::isfinite(tn))
+    {
+        volatile float idv = float(kLut) / tn;
+        inv_delta = idv;
+        float const last = std::nextafter(tn, 0.0f);
+        while (inv_delta > 0.0f && lut_cell(last, inv_delta) >= kLut)
+            inv_delta = std::nextafter(inv_delta, 0.0f);
+    }
+    T[nbins + 1] = inv_delta;
+    T[nbins + 2] = std::numeric_limits<float>::infinity(); // x_fast
+    uint16_t* lut = reinterpret_cast<uint16_t*>(T + nbins + 3);
+    auto bin_of = [&](float x) { // reference bin of x in [T0, Tn)
+        return uint32_t(std::upper_bound(T, T + nbins + 1, x) - T) - 1;
+    };
+    if (!(inv_delta > 0.0f) || nbins > kLutMaxBins)
+    {
+        for (uint32_t c = 0; c < kLut; ++c)
+            lut[c] = kLutNone;
+        return tab;
+    }
+    // first float x >= 0 whose cell is >= c
+    auto first_in = [&](uint32_t c) {
+        uint32_t lo = 0, hi = __builtin_bit_cast(uint32_t, tn);
+        if (lut_cell(tn, inv_delta) < c)
+            return tn;
+        while (lo < hi)
+        {
+            uint32_t const mid = lo + (hi - lo) / 2;
+            if (lut_cell(__builtin_bit_cast(float, mid), inv_delta) >= c)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        return __builtin_bit_cast(float, lo);
+    };
+    float start = first_in(0);
+    float x_fast = 0.0f;
+    for (uint32_t c = 0; c < kLut; ++c)
+    {
+        float const next = (c + 1 < kLut) ? first_in(c + 1) : tn;
+        float a = std::max(start, T[0]);
+        float b = std::nextafter(std::min(next, tn), 0.0f); // last x of the cell below Tn
+        if (!(a <= b))
+            lut[c] = 0; // no r_sq of this cell is binned
+        else
+        {
+            uint32_t const ka = bin_of(a), kb = bin_of(b);
+            lut[c] = (kb - ka <= 1) ? uint16_t(ka) : kLutNone;
+            if (lut[c] == kLutNone)
+                x_fast = next;
+        }
+        start = next;
+    }
+    T[nbins + 2] = x_fast;
+    return tab;
+}
+
 struct Thresholds
 {
-    std::vector<float> host;
     float* dev = nullptr;
+    size_t cap = 0;
     uint32_t nbins = 0;
     float vmin = 0, vmax = 0;
 };
@@ -87,20 +186,16 @@ int thresholds(uint32_t nbins, float vmin, float vmax, float** d_out)
         *d_out = g_thr.dev;
         return FA_OK;
     }
-    float const width = (vmax - vmin) / float(nbins);
-    float const inv_width = 1.0f / width;
-    std::vector<float> T(nbins + 1);
-    for (uint32_t k = 0; k <= nbins; ++k)
-        T[k] = first_rsq_with_bin_at_least(int(k), vmin, vmax, inv_width, nbins);
-    if (!g_thr.dev || g_thr.nbins < nbins)
+    std::vector<float> const tab = build_table(nbins, vmin, vmax);
+    if (!g_thr.dev || g_thr.cap < tab.size())
     {
         fa::device_free(g_thr.dev);
         g_thr.dev = nullptr;
-        if (int rc = fa::device_alloc((void**) &g_thr.dev, sizeof(float) * (nbins + 1)))
+        if (int rc = fa::device_alloc((void**) &g_thr.dev, sizeof(float) * tab.size()))
             return rc;
+        g_thr.cap = tab.size();
     }
-    FA_HIP(hipMemcpyAsync(g_thr.dev, T.data(), sizeof(float) * (nbins + 1), hipMemcpyHostToDevice, fa::stream()));
-    g_thr.host = T;
+    FA_HIP(hipMemcpy(g_thr.dev, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
     g_thr.nbins = nbins;
     g_thr.vmin = vmin;
     g_thr.vmax = vmax;
@@ -131,6 +226,91 @@ __device__ inline uint32_t bin_in_range(float x, const float* __restrict__ T, ui
     return uint32_t(k);
 }
 
+// LDS form of the bin table: T[0..nbins], inv_delta and x_fast, then the kLut
+// cell entries as uint32 LDS byte addresses of T[k + 1] (cells below x_fast
+// point at T[1]: their lookups are discarded), then the histogram.
+__host__ __device__ constexpr uint32_t lds_table_floats(uint32_t nbins)
+{
+    return nbins + 3 + kLut;
+}
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
+typedef __attribute__((address_space(3))) const float lds_cf32;
+
+__device__ inline uint32_t lds_addr(const void* p)
+{
+    return uint32_t(uintptr_t((const __attribute__((address_space(3))) char*) (p)));
+}
+
+// Binning targets of the row kernels; add2 bins two r_sq values whose flags
+// say they are counted.  LutBins: table and histogram in LDS (up to
+// kMaxLdsBins bins).  For r_sq >= x_fast the entry e of the cell is the
+// address of T[k + 1], so one LDS read and one compare pick bin k or k + 1 and
+// the histogram slot is e (+ 4) + hist_c; both lookups are issued for every
+// lane before the masked atomics, so their LDS latencies overlap.  r_sq below
+// x_fast (pairs at r near 0) take the square-root path in a branch that the
+// wave skips when none of its lanes needs it.  GlobalBins: thresholds and counts in device memory.
+struct LutBins
+{
+    const float* sT;
+    uint32_t nbins;
+    float inv_delta, x_fast, vmin, inv_width;
+    uint32_t lut, hist, hist_c;
+    __device__ LutBins(const float* t, uint32_t nb, float vmn, float iw, const uint32_t* h)
+        : sT(t), nbins(nb), inv_delta(t[nb + 1]), x_fast(t[nb + 2]), vmin(vmn), inv_width(iw),
+          lut(lds_addr(t + nb + 3)), hist(lds_addr(h)), hist_c(lds_addr(h) - lds_addr(t) - 4u)
+    {
+    }
+    __device__ float t0() const { return sT[0]; }
+    __device__ float tn() const { return sT[nbins]; }
+    __device__ __forceinline__ uint32_t fast_slot(float x) const
+    {
+        uint32_t const e = *(lds_cu32*) (uintptr_t) (lut + 4u * min(uint32_t(x * inv_delta), kLut - 1u));
+        float const t = *(lds_cf32*) (uintptr_t) e;
+        return e + (x >= t ? 4u : 0u) + hist_c;
+    }
+    __device__ static void atomic(uint32_t a, uint32_t w)
+    {
+        __atomic_fetch_add((lds_u32*) (uintptr_t) a, w, __ATOMIC_RELAXED);
+    }
+    __device__ __forceinline__ void add2(float x0, bool ok0, float x1, bool ok1, uint32_t w) const
+    {
+        bool const f0 = ok0 && x0 >= x_fast, f1 = ok1 && x1 >= x_fast;
+        uint32_t const a0 = fast_slot(x0), a1 = fast_slot(x1);
+        asm volatile("" ::"v"(a0), "v"(a1)); // both lookups ahead of the masked atomics
+        if (f0)
+            atomic(a0, w);
+        if (f1)
+            atomic(a1, w);
+        bool const s0 = ok0 && !f0, s1 = ok1 && !f1;
+        if (s0 || s1) // a divergent branch: skipped when no lane takes it
+        {
+            if (s0)
+                atomic(hist + 4u * bin_in_range(x0, sT, nbins, vmin, inv_width), w);
+            if (s1)
+                atomic(hist + 4u * bin_in_range(x1, sT, nbins, vmin, inv_width), w);
+        }
+    }
+};
+
+struct GlobalBins
+{
+    const float* T;
+    uint32_t nbins;
+    float vmin, inv_width;
+    uint32_t* counts;
+    __device__ float t0() const { return T[0]; }
+    __device__ float tn() const { return T[nbins]; }
+    __device__ void add2(float x0, bool ok0, float x1, bool ok1, uint32_t w) const
+    {
+        if (ok0)
+            atomicAdd(counts + bin_in_range(x0, T, nbins, vmin, inv_width), w);
+        if (ok1)
+            atomicAdd(counts + bin_in_range(x1, T, nbins, vmin, inv_width), w);
+    }
+};
+
 // The reference's bin of r_sq, given T in LDS; nbins when outside [T0, Tn).
 __device__ inline uint32_t bin_of_rsq(float x, const float* __restrict__ T, uint32_t nbins, float vmin,
                                       float inv_width)
@@ -157,15 +337,32 @@ __device__ inline uint32_t bin_of_rsq(float x, const float* __restrict__ T, uint
     return uint32_t(k);
 }
 
+// LDS copy of the bin table (lds_table_floats layout); hist_prologue also
+// clears the LDS histogram that follows it.  Kernels with device-memory counts
+// read the table from device memory instead.
+__device__ inline void table_prologue(float* sT, const float* __restrict__ T, uint32_t nbins)
+{
+    uint32_t const nt = nbins + 3;
+    for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x)
+        sT[t] = T[t];
+    const uint16_t* glut = reinterpret_cast<const uint16_t*>(T + nt);
+    uint32_t* slut = reinterpret_cast<uint32_t*>(sT + nt);
+    uint32_t const tb = lds_addr(sT);
+    for (uint32_t t = threadIdx.x; t < kLut; t += blockDim.x)
+    {
+        uint32_t const k = glut[t];
+        slut[t] = tb + 4u * (k == kLutNone ? 1u : k + 1u);
+    }
+}
+
 template<bool LDS_HIST>
 __device__ inline void hist_prologue(uint32_t* hist, float* sT, const float* __restrict__ T, uint32_t nbins)
 {
-    for (uint32_t t = threadIdx.x; t <= nbins; t += blockDim.x)
-    {
-        sT[t] = T[t];
-        if (LDS_HIST && t < nbins)
-            hist[t] = 0;
-    }
+    if (!LDS_HIST)
+        return;
+    table_prologue(sT, T, nbins);
+    for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+        hist[t] = 0;
     __syncthreads();
 }
 
@@ -243,37 +440,42 @@ __global__ void row_cluster_table_kernel(const uint32_t* __restrict__ cstart, ui
 // Latency hiding: lane t first resolves stencil row t (its up to 3 periodic
 // pieces and their cell-range bounds), so the cstart reads of all rows are in
 // flight together; the wave then walks the rows, broadcasting each row's
-// bounds with readlane, and loads the next chunk before binning the current.
-//
-// The pair work of one cluster entry e = (frame-local row, start, count,
-// cx_lo | cx_hi << 16) for one wave, binned into h (LDS).
-template<int R>
-__device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e, float q_rmax_sq, float q_rmin_sq,
-                                     const float* sT, uint32_t nbins, float vmin, float inv_width, uint32_t* h)
+// bounds with readlane, and loads the next two chunks before binning the
+// current two.  Piece bounds are wave-uniform, so the chunk loop runs on the
+// scalar unit; only a piece's first chunk (which may hold the cluster itself,
+// where a pair counts from its earlier point) and its last (which may run past
+// the piece; the point arrays are padded for it) test positions per lane.
+// Lanes without a query point carry a NaN coordinate, which fails every range
+// compare.
+template<int R, bool IS2D, bool HAS_LO, class Bins>
+__device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row, uint32_t A0, uint32_t na,
+                                 int cx_lo, int cx_hi, float q_rmax_sq, float lo, float hi, const Bins& B)
 {
     constexpr int kRowsY = 2 * R + 1;
+    constexpr int Rz = IS2D ? 0 : R;
+    constexpr int nrows = kRowsY * (2 * Rz + 1);
     uint32_t const lane = threadIdx.x & 63u;
     int const dim0 = g.dim[0], dim1 = g.dim[1], dim2 = g.dim[2];
-    // pairs counted: q_rmin_sq <= r_sq < q_rmax_sq (the query) and inside the histogram
-    float const lo = fmaxf(q_rmin_sq, sT[0]), hi = fminf(q_rmax_sq, sT[nbins]);
     // row trimming: a candidate row (dy, dz) is at least (|dy|-1) cell extents away
     // along y and (|dz|-1) along z; conservative margins cover the rounding of
     // the cell assignment
     float const r_lim = sqrtf(q_rmax_sq) * 1.0001f;
     float const wx = g.cw[0] * 0.9999f, wy = g.cw[1] * 0.9999f, wz = g.cw[2] * 0.9999f;
     bool const ortho = b.xy == 0.0f && b.xz == 0.0f && b.yz == 0.0f;
-    uint32_t const A0 = e.y, na = e.z;
-    int const cx_lo = int(e.w & 0xffffu), cx_hi = int(e.w >> 16);
-    int const cy0 = int(e.x % uint32_t(dim1));
-    int const cz0 = int(e.x / uint32_t(dim1));
+    int const cy0 = int(row % uint32_t(dim1));
+    int const cz0 = int(row / uint32_t(dim1));
     uint32_t const a = lane >> 3, bl = lane & 7u;
-    bool const a_ok = a < na;
-    uint32_t const pos_a = A0 + (a_ok ? a : 0u);
+    uint32_t const pos_a = A0 + min(a, na - 1u);
     float4 pa = g.cpos[pos_a];
-    if (b.is2D)
+    if (IS2D)
         pa.z = 0.0f;
-    int const Rz = b.is2D ? 0 : R;
-    int const nrows = kRowsY * (2 * Rz + 1);
+    if (a >= na)
+        pa.x = __builtin_nanf("");
+    // chunk loads through a buffer resource: wave-uniform chunk offset in an
+    // SGPR, the lane's offset in a VGPR (point arrays stay below 4 GiB)
+    uint32_t const voff = bl * uint32_t(sizeof(float4));
+    __amdgpu_buffer_rsrc_t const rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(g.cpos), short(0), -1, kBufferWord3);
     for (int r0 = 0; r0 < nrows; r0 += 64)
     {
         // lane t: stencil row r0 + t -> pieces (beg, end, image code); code < 0: no piece
@@ -341,28 +543,88 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
                 if (beg >= end)
                     continue;
                 uint32_t const weight = wraps ? 1u : 2u;
+                // chunks starting below ord_lim may hold pairs of the cluster with
+                // itself or with its earlier points
+                uint32_t const ord_lim = wraps ? 0u : A0 + na;
                 float const qx = pa.x + b.img[code][0];
                 float const qy = pa.y + b.img[code][1];
                 float const qz = pa.z + b.img[code][2];
-                float4 cur = g.cpos[min(beg + bl, end - 1u)];
-                for (uint32_t jb = beg; jb < end; jb += 8)
+                auto ld = [&](uint32_t p) {
+                    auto const v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, p * uint32_t(sizeof(float4)), 0);
+                    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.0f);
+                };
+                auto rsq = [&](const float4& c) {
+                    float const rx = c.x - qx;
+                    float const ry = c.y - qy;
+                    if constexpr (IS2D)
+                        return rx * rx + ry * ry;
+                    float const rz = c.z - qz;
+                    return (rx * rx + ry * ry) + rz * rz;
+                };
+                auto two = [&](const float4& u, const float4& v, uint32_t jb) {
+                    float const x0 = rsq(u), x1 = rsq(v);
+                    bool ok0 = x0 < hi, ok1 = x1 < hi;
+                    if constexpr (HAS_LO)
+                    {
+                        ok0 = ok0 && x0 >= lo;
+                        ok1 = ok1 && x1 >= lo;
+                    }
+                    if (!(jb + 16u <= end && jb >= ord_lim))
+                    {
+                        uint32_t const p0 = jb + bl, p1 = jb + 8u + bl;
+                        ok0 = ok0 && p0 < end && (jb >= ord_lim || p0 > pos_a);
+                        ok1 = ok1 && p1 < end && (jb + 8u >= ord_lim || p1 > pos_a);
+                    }
+                    B.add2(x0, ok0, x1, ok1, weight);
+                };
+                // two register sets in turn: the loads of the next 16 points are in
+                // flight while the current 16 are binned
+                float4 c0 = ld(beg), c1 = ld(beg + 8u);
+                for (uint32_t jb = beg;;)
                 {
-                    float4 nxt = cur;
-                    if (jb + 8u < end)
-                        nxt = g.cpos[min(jb + 8u + bl, end - 1u)];
-                    uint32_t const pos_b = jb + bl;
-                    bool const valid = a_ok && pos_b < end && (wraps || pos_b > pos_a);
-                    float const pjz = b.is2D ? 0.0f : cur.z;
-                    float const rx = cur.x - qx;
-                    float const ry = cur.y - qy;
-                    float const rz = pjz - qz;
-                    float const x = (rx * rx + ry * ry) + rz * rz;
-                    if (valid && x >= lo && x < hi)
-                        atomicAdd(&h[bin_in_range(x, sT, nbins, vmin, inv_width)], weight);
-                    cur = nxt;
+                    float4 const d0 = ld(jb + 16u), d1 = ld(jb + 24u);
+                    two(c0, c1, jb);
+                    jb += 16u;
+                    if (jb >= end)
+                        break;
+                    c0 = ld(jb + 16u);
+                    c1 = ld(jb + 24u);
+                    two(d0, d1, jb);
+                    jb += 16u;
+                    if (jb >= end)
+                        break;
                 }
             }
         }
+    }
+}
+
+// The pair work of one cluster entry e = (frame-local row, start, count,
+// cx_lo | cx_hi << 16) for one wave (e is wave-uniform).
+template<int R, class Bins>
+__device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e, float q_rmax_sq, float q_rmin_sq,
+                                     const Bins& B)
+{
+    uint32_t const row = __builtin_amdgcn_readfirstlane(e.x);
+    uint32_t const A0 = __builtin_amdgcn_readfirstlane(e.y);
+    uint32_t const na = __builtin_amdgcn_readfirstlane(e.z);
+    uint32_t const cx = __builtin_amdgcn_readfirstlane(e.w);
+    int const cx_lo = int(cx & 0xffffu), cx_hi = int(cx >> 16);
+    // pairs counted: q_rmin_sq <= r_sq < q_rmax_sq (the query) and inside the histogram
+    float const lo = fmaxf(q_rmin_sq, B.t0()), hi = fminf(q_rmax_sq, B.tn());
+    if (b.is2D)
+    {
+        if (lo > 0.0f)
+            rows_body<R, true, true>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+        else
+            rows_body<R, true, false>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+    }
+    else
+    {
+        if (lo > 0.0f)
+            rows_body<R, false, true>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+        else
+            rows_body<R, false, false>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
     }
 }
 
@@ -375,13 +637,18 @@ __global__ __launch_bounds__(kBlock) void rdf_rows_kernel(DevBox b, DevGrid g, c
 {
     extern __shared__ float smem[];
     float* sT = smem;
-    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + nbins + 1);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + lds_table_floats(nbins));
     hist_prologue<LDS_HIST>(hist, sT, T, nbins);
     constexpr uint32_t kWaves = kBlock / 64;
     uint32_t const I = fa::xcd_tile(blockIdx.x, gridDim.x) * kWaves + (threadIdx.x >> 6);
     if (I < *ncl_ptr)
-        rdf_rows_wave<R>(b, g, table[I], q_rmax_sq, q_rmin_sq, sT, nbins, vmin, inv_width,
-                         LDS_HIST ? hist : counts);
+    {
+        if constexpr (LDS_HIST)
+            rdf_rows_wave<R>(b, g, table[I], q_rmax_sq, q_rmin_sq,
+                             LutBins(sT, nbins, vmin, inv_width, hist));
+        else
+            rdf_rows_wave<R>(b, g, table[I], q_rmax_sq, q_rmin_sq, GlobalBins{T, nbins, vmin, inv_width, counts});
+    }
     hist_epilogue<LDS_HIST>(hist, nbins, counts);
 }
 
@@ -395,8 +662,9 @@ __global__ __launch_bounds__(kBlock) void rdf_self_kernel(DevBox b, DevGrid g, u
 {
     extern __shared__ float smem[];
     float* sT = smem;
-    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + nbins + 1);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + lds_table_floats(nbins));
     uint32_t* h = LDS_HIST ? hist : counts;
+    const float* tab = LDS_HIST ? sT : T;
     hist_prologue<LDS_HIST>(hist, sT, T, nbins);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
@@ -408,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void rdf_self_kernel(DevBox b, DevGrid g, u
                                                 return;
                                             if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq)
                                             {
-                                                uint32_t const bin = bin_of_rsq(r_sq, sT, nbins, vmin, inv_width);
+                                                uint32_t const bin = bin_of_rsq(r_sq, tab, nbins, vmin, inv_width);
                                                 if (bin < nbins)
                                                     atomicAdd(&h[bin], 1u);
                                             }
@@ -427,8 +695,9 @@ __global__ __launch_bounds__(kBlock) void rdf_query_kernel(DevBox b, DevGrid g, 
 {
     extern __shared__ float smem[];
     float* sT = smem;
-    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + nbins + 1);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(smem + lds_table_floats(nbins));
     uint32_t* h = LDS_HIST ? hist : counts;
+    const float* tab = LDS_HIST ? sT : T;
     hist_prologue<LDS_HIST>(hist, sT, T, nbins);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
     {
@@ -439,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void rdf_query_kernel(DevBox b, DevGrid g, 
                                                 return;
                                             if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq)
                                             {
-                                                uint32_t const bin = bin_of_rsq(r_sq, sT, nbins, vmin, inv_width);
+                                                uint32_t const bin = bin_of_rsq(r_sq, tab, nbins, vmin, inv_width);
                                                 if (bin < nbins)
                                                     atomicAdd(&h[bin], 1u);
                                             }
@@ -515,7 +784,8 @@ int run_rows(const RowsArgs& a, int R, uint32_t* d_counts, bool* handled)
     hipStream_t s = fa::stream();
     int dims[3];
     fa::grid_dims_for(nq->box, nq->n, a.q_r_max, R, dims);
-    if (dims[0] > 65535)
+    // cluster entries hold cell x in 16 bits; chunk loads use 32-bit byte offsets
+    if (dims[0] > 65535 || uint64_t(nq->n) + fa::kCposPad > (1ull << 28))
         return FA_OK;
     if (int rc = fa::ensure_grid(nq, dims))
         return rc;
@@ -543,7 +813,7 @@ int run_rows(const RowsArgs& a, int R, uint32_t* d_counts, bool* handled)
                cl_off, uint32_t(max_clusters), table);
     unsigned const wblocks = unsigned((max_clusters + kBlock / 64 - 1) / (kBlock / 64));
     bool const lds = a.bins <= kMaxLdsBins;
-    size_t const shmem = sizeof(float) * (a.bins + 1) + (lds ? sizeof(uint32_t) * a.bins : 0);
+    size_t const shmem = lds ? sizeof(float) * lds_table_floats(a.bins) + sizeof(uint32_t) * a.bins : 0;
 #define FA_ROWS(RR)                                                                                                 \
     case RR:                                                                                                        \
         if (lds)                                                                                                    \
@@ -674,11 +944,10 @@ __global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, cons
     extern __shared__ float smem[];
     float* sT = smem;
     uint32_t const wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    uint32_t* wh = reinterpret_cast<uint32_t*>(smem + nbins + 1) + wave * nbins;
-    for (uint32_t t = threadIdx.x; t <= nbins; t += blockDim.x)
-        sT[t] = T[t];
+    uint32_t* wh = reinterpret_cast<uint32_t*>(smem + lds_table_floats(nbins)) + wave * nbins;
     for (uint32_t t = lane; t < nbins; t += 64)
         wh[t] = 0u;
+    table_prologue(sT, T, nbins);
     __syncthreads();
     uint32_t const ncl = *ncl_ptr;
     uint32_t cur = 0xffffffffu;
@@ -713,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, cons
             uint4 const e = table[I];
             if (!F.any_outside)
             {
-                rdf_rows_wave<R>(F.box, g, e, q_rmax_sq, 0.0f, sT, nbins, vmin, inv_width, wh);
+                rdf_rows_wave<R>(F.box, g, e, q_rmax_sq, 0.0f, LutBins(sT, nbins, vmin, inv_width, wh));
             }
             else if (lane < e.z)
             {
@@ -753,6 +1022,8 @@ int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const Batch
     hipStream_t s = stream();
     if (bins == 0 || bins > 2048)
         return fail(FA_ERR_INVALID, "batched RDF supports 1..2048 bins");
+    if (uint64_t(G.total_points) + kCposPad > (1ull << 28))
+        return fail(FA_ERR_INVALID, "batched RDF supports up to 2^28 - 32 points per batch");
     const float* T = S.T;
     float const inv_width = 1.0f / ((r_max - 0.0f) / float(bins));
     unsigned const rblocks = (G.total_rows + kBlock - 1) / kBlock;
@@ -776,7 +1047,7 @@ int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const Batch
         FA_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
     }
     unsigned const blocks = unsigned(std::max(1, n_cu) * 8);
-    size_t const shmem = sizeof(float) * (bins + 1) + sizeof(uint32_t) * bins * (kBlock / 64);
+    size_t const shmem = sizeof(float) * lds_table_floats(bins) + sizeof(uint32_t) * bins * (kBlock / 64);
     float const qmax_sq = r_max * r_max;
 #define FA_BROWS(RR)                                                                                                \
     case RR:                                                                                                        \
@@ -798,17 +1069,33 @@ int batch_rdf(const BatchGrid& G, int R, float r_max, uint32_t bins, const Batch
 
 int thresholds_for(uint32_t bins, float r_max, float* d_T)
 {
-    float const inv_width = 1.0f / ((r_max - 0.0f) / float(bins));
-    std::vector<float> T(bins + 1);
-    for (uint32_t k = 0; k <= bins; ++k)
-        T[k] = first_rsq_with_bin_at_least(int(k), 0.0f, r_max, inv_width, bins);
-    FA_HIP(hipMemcpy(d_T, T.data(), sizeof(float) * (bins + 1), hipMemcpyHostToDevice));
+    std::vector<float> const tab = build_table(bins, 0.0f, r_max);
+    FA_HIP(hipMemcpy(d_T, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
     return FA_OK;
+}
+
+size_t rdf_table_floats(uint32_t bins)
+{
+    return table_floats(bins);
 }
 
 } // namespace fa
 
 extern "C" {
+
+int fa_rdf_bin_table(uint32_t bins, float r_min, float r_max, float* out, uint64_t cap)
+{
+    if (bins == 0 || !(r_max > r_min))
+        return fa::fail(FA_ERR_INVALID, "bin table needs bins > 0 and r_max > r_min");
+    std::vector<float> const tab = build_table(bins, r_min, r_max);
+    if (out)
+    {
+        if (cap < tab.size())
+            return fa::fail(FA_ERR_INVALID, "output too small for the bin table");
+        std::memcpy(out, tab.data(), sizeof(float) * tab.size());
+    }
+    return int(tab.size());
+}
 
 int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, float q_r_max, float q_r_min,
                             int exclude_ii, uint32_t bins, float r_min, float r_max, uint32_t* d_counts)
@@ -829,7 +1116,7 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
     if (int rc = thresholds(bins, r_min, r_max, &T))
         return rc;
     bool const lds = bins <= kMaxLdsBins;
-    size_t const shmem = sizeof(float) * (bins + 1) + (lds ? sizeof(uint32_t) * bins : 0);
+    size_t const shmem = lds ? sizeof(float) * lds_table_floats(bins) + sizeof(uint32_t) * bins : 0;
     bool const self = (d_query_points == nq->d_points) && (n_query == nq->n);
 
     if (self && nq->kind == FA_NQ_AABB)

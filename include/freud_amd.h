@@ -219,6 +219,15 @@ int fa_nlist_copy(const fa_nlist* src, fa_nlist* dst);
  * BondHistogramCompute::accumulateGeneral, BondHistogramCompute.h:115-126): a
  * fused ball query (query radius q_r_max/q_r_min) + RegularAxis binning
  * (freud/util/Histogram.h:152-174) into d_counts (bins uint32, accumulated). */
+/* The host-built bin table the RDF kernels use (no counterpart in the
+ * reference; exposed so that its exactness can be checked on the host):
+ * bins + 1 r_sq thresholds T[k] = the smallest float r_sq whose
+ * RegularAxis::bin (Histogram.h:152-174) of sqrtf(r_sq) is >= k, then
+ * inv_delta, then x_fast, then a 4096-entry uint16 table (cell c =
+ * uint32(r_sq * inv_delta), below 4096 for every r_sq < T[bins]; entry k: the
+ * cell's bins are k or k + 1; 0xffff: more, only in cells below x_fast).
+ * Returns the number of floats (out may be NULL to query it). */
+int fa_rdf_bin_table(uint32_t bins, float r_min, float r_max, float* out, uint64_t cap);
 int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, float q_r_max, float q_r_min,
                             int exclude_ii, uint32_t bins, float r_min, float r_max, uint32_t* d_counts);
 /* Same, binning the distances of an existing NeighborList. */

@@ -138,3 +138,35 @@ def test_gsd_reader():
     npt.assert_allclose(fr.box[:3], 20.0)
     fr = open_gsd(os.path.join(GOLDEN, "steinhardt_average", "Test_Configuration.gsd"))[-1]
     assert fr.positions.shape == (3288, 3)
+
+
+def test_rdf_bin_table_is_exact(oracle):
+    """The RDF kernels bin r_sq through a host-built table (thresholds + lookup);
+    emulate the device arithmetic on many r_sq values and compare with the
+    reference's RegularAxis bin of the correctly rounded sqrt."""
+    rng = np.random.default_rng(0)
+    for bins, r_min, r_max in ((200, 0.0, 5.0), (100, 0.0, 4.0), (37, 0.25, 3.9), (150, 0.5, 2.0), (7, 0.0, 1.0)):
+        n = _lib.lib().fa_rdf_bin_table(bins, r_min, r_max, None, 0)
+        assert n > bins
+        tab = np.zeros(n, np.float32)
+        assert _lib.lib().fa_rdf_bin_table(bins, r_min, r_max, tab.ctypes.data, n) == n
+        T = tab[: bins + 1]
+        inv_delta, x_fast = tab[bins + 1], tab[bins + 2]
+        lut = tab[bins + 3:].view(np.uint16)[:4096]
+        # r_sq spread over the range plus the thresholds themselves and their float neighbours
+        x = np.concatenate([rng.uniform(0, 1.05 * float(T[-1]), 200000).astype(np.float32), T,
+                            np.nextafter(T, np.float32(0)), np.nextafter(T, np.float32(np.inf))]).astype(np.float32)
+        x = np.sort(x[(x >= T[0]) & (x < T[-1])])
+        p = x * inv_delta
+        assert p.max() < 4096  # the kernels index the table without a clamp
+        c = p.astype(np.uint32)
+        k = lut[c].astype(np.int64)
+        ok = k != 0xFFFF
+        assert np.all(ok[x >= x_fast])  # multi-bin cells lie below x_fast
+        got = k[ok] + (x[ok] >= T[k[ok] + 1])
+        # x is sorted and bins are monotone in x: equal histograms of two
+        # non-decreasing bin sequences mean equal bins element by element
+        assert np.all(np.diff(got) >= 0)
+        ref = oracle.rdf_bin(np.sqrt(x[ok]), bins, r_min, r_max)  # histogram of the reference bins
+        npt.assert_array_equal(np.bincount(got, minlength=bins), ref)
+        assert ok.mean() > 0.9
