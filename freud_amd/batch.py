@@ -143,8 +143,11 @@ class TrajectoryBatch:
 
     @property
     def config(self):
-        """Launch shape chosen by the autotuner: ball and RDF cells r/R wide."""
-        a, b, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        """Launch shape chosen by the autotuner: ball and RDF cells r/R wide (RDF
+        cells rdf_F times narrower along x)."""
+        a, b, f, g = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         cap = ctypes.c_uint64()
-        check(lib().fa_batch_config(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(cap), ctypes.byref(g)))
-        return {"ball_R": a.value, "rdf_R": b.value, "bond_capacity": int(cap.value), "graph": bool(g.value)}
+        check(lib().fa_batch_config(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(f), ctypes.byref(cap),
+                                    ctypes.byref(g)))
+        return {"ball_R": a.value, "rdf_R": b.value, "rdf_F": f.value, "bond_capacity": int(cap.value),
+                "graph": bool(g.value)}

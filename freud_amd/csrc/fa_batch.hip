@@ -110,6 +110,7 @@ struct fa_batch
 
     // RDF
     int rdf_R = 2;
+    int rdf_F = 1; // RDF cells are F times narrower along x
     BatchGrid gr;
     fa::BatchRdfScratch rs{};
     uint32_t* d_rdf = nullptr; // n_frames x bins
@@ -161,9 +162,15 @@ void drop_graph(fa_batch* b)
     b->graph = nullptr;
 }
 
+// RDF launch shapes the tuner tries: (R, Fx), cells r / R wide and Fx times
+// narrower along x.
+constexpr int kRdfShape[][2] = {{2, 1}, {3, 1}, {4, 1}, {2, 4}, {2, 8}};
+constexpr int kRdfShapes = int(sizeof(kRdfShape) / sizeof(kRdfShape[0]));
+
 // Frame descriptions and device arrays of a grid whose cells are at least
-// r / R wide in every frame.
-int make_grid(fa_batch* b, float r, int R, BatchGrid& g)
+// r / R wide in every frame, and r / (R Fx) along x where the cell count
+// allows it.
+int make_grid(fa_batch* b, float r, int R, BatchGrid& g, int Fx = 1)
 {
     free_grid(g);
     std::vector<BatchFrame> fr(b->n_frames);
@@ -175,6 +182,12 @@ int make_grid(fa_batch* b, float r, int R, BatchGrid& g)
         F.box = fa::make_devbox(b->boxes[f]);
         uint32_t const n = uint32_t(b->pt_off[f + 1] - b->pt_off[f]);
         fa::grid_dims_for(b->boxes[f], n, r, R, F.dim);
+        int fx = Fx;
+        if (F.dim[0] * fx > 65535 ||
+            double(F.dim[0]) * fx * F.dim[1] * F.dim[2] > std::max<double>(8.0 * n, 64.0))
+            fx = 1;
+        F.dim[0] *= fx;
+        F.rx = R * fx;
         {
             float npd[3];
             fa::nearest_plane_distance(b->boxes[f], npd);
@@ -365,7 +378,7 @@ int read_status(fa_batch* b, bool* fitted)
 }
 
 // The autotuner's choice of cell size for the ball query (R = 1, 2) and the
-// RDF (R = 2, 3, 4), timed on this batch's own data.
+// RDF (the kRdfShape launch shapes), timed on this batch's own data.
 int tune(fa_batch* b)
 {
     double pts = 0, vol = 0;
@@ -418,10 +431,11 @@ int tune(fa_batch* b)
         if (v < 0 && fa::tuning_enabled())
         {
             float best = 0;
-            for (int cand = 0; cand < 3; ++cand)
+            for (int cand = 0; cand < kRdfShapes; ++cand)
             {
-                b->rdf_R = 2 + cand;
-                if (int rc = make_grid(b, b->plan.rdf_r_max, b->rdf_R, b->gr))
+                b->rdf_R = kRdfShape[cand][0];
+                b->rdf_F = kRdfShape[cand][1];
+                if (int rc = make_grid(b, b->plan.rdf_r_max, b->rdf_R, b->gr, b->rdf_F))
                     return rc;
                 if (int rc = alloc_rdf_scratch(b))
                     return rc;
@@ -438,8 +452,9 @@ int tune(fa_batch* b)
             }
             fa::record_variant(fa::TUNE_RDF_SELF, key, v);
         }
-        b->rdf_R = 2 + std::max(v, 0);
-        if (int rc = make_grid(b, b->plan.rdf_r_max, b->rdf_R, b->gr))
+        b->rdf_R = kRdfShape[std::max(v, 0) % kRdfShapes][0];
+        b->rdf_F = kRdfShape[std::max(v, 0) % kRdfShapes][1];
+        if (int rc = make_grid(b, b->plan.rdf_r_max, b->rdf_R, b->gr, b->rdf_F))
             return rc;
         if (int rc = alloc_rdf_scratch(b))
             return rc;
@@ -761,7 +776,7 @@ int fa_batch_ql(fa_batch* b, float** d_ql, float** d_qlm, float* order)
     return FA_OK;
 }
 
-int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, uint64_t* capacity, int* graph_ready)
+int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, int* rdf_F, uint64_t* capacity, int* graph_ready)
 {
     if (!b)
         return fa::fail(FA_ERR_INVALID, "null argument");
@@ -769,6 +784,8 @@ int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, uint64_t* capacity, in
         *ball_R = b->ball_R;
     if (rdf_R)
         *rdf_R = b->rdf_R;
+    if (rdf_F)
+        *rdf_F = b->rdf_F;
     if (capacity)
         *capacity = b->capacity;
     if (graph_ready)

@@ -456,6 +456,7 @@ struct BatchFrame
     uint32_t tile_base; // first tile of the frame
     int any_outside;    // set by the grid build when a point lies outside the box
     float cw[3];        // cell extent along each plane normal
+    int rx;             // cells along x that cover the grid's r (R times the x refinement)
 };
 
 // One cell grid over every frame of a batch (device arrays).

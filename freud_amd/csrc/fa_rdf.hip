@@ -427,29 +427,46 @@ __global__ void row_cluster_table_kernel(const uint32_t* __restrict__ cstart, ui
     table[c] = make_uint4(r, p, last - p + 1u, cell_of_pos(p) | (cell_of_pos(last) << 16));
 }
 
-// One wave per cluster; lane = (a, b) = (lane / 8, lane % 8): query point a of
-// the cluster against point b of each 8-point chunk of the stencil rows.  The
-// stencil spans cells [cx_lo - R, cx_hi + R] x [cy - R, cy + R] x [cz - R, cz + R]
-// (cells at least r_max / R wide), walked as contiguous pieces of a row that
-// share one periodic shift.  A pair related without a periodic shift is found
-// from both of its points with bitwise-equal r_sq (x - y == -(y - x)), so it
-// is counted once, from the point earlier in cell order, with weight 2; pairs
-// across the box edge are counted from both sides with weight 1, as the
-// reference's (q + image) arithmetic rounds differently for the two directions.
+// Exclusive prefix sum over the 64 lanes of a wave.
+__device__ inline uint32_t wave_exclusive_scan(uint32_t v)
+{
+    uint32_t const lane = threadIdx.x & 63u;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        uint32_t const y = __shfl_up(x, d);
+        if (lane >= uint32_t(d))
+            x += y;
+    }
+    return x - v;
+}
+
+// One wave per cluster (up to 8 consecutive cell-ordered points of a row).
+// The stencil spans cells [cx_lo - R, cx_hi + R] x [cy - R, cy + R] x
+// [cz - R, cz + R] (cells at least r_max / R wide), walked as contiguous pieces
+// of a row that share one periodic shift.  A pair related without a periodic
+// shift is found from both of its points with bitwise-equal r_sq
+// (x - y == -(y - x)), so it is counted once, from the point earlier in cell
+// order, with weight 2; pairs across the box edge are counted from both sides
+// with weight 1, as the reference's (q + image) arithmetic rounds differently
+// for the two directions.
 //
-// Latency hiding: lane t first resolves stencil row t (its up to 3 periodic
-// pieces and their cell-range bounds), so the cstart reads of all rows are in
-// flight together; the wave then walks the rows, broadcasting each row's
-// bounds with readlane, and loads the next two chunks before binning the
-// current two.  Piece bounds are wave-uniform, so the chunk loop runs on the
-// scalar unit; only a piece's first chunk (which may hold the cluster itself,
-// where a pair counts from its earlier point) and its last (which may run past
-// the piece; the point arrays are padded for it) test positions per lane.
-// Lanes without a query point carry a NaN coordinate, which fails every range
-// compare.
+// Lane t first resolves stencil row t (its up to 3 periodic pieces and their
+// cell-range bounds), so the cstart reads of all rows are in flight together.
+// Unshifted pieces (the bulk) are concatenated in row order and taken 64
+// candidates at a time, one per lane, against the cluster's query points held
+// in scalar registers: every candidate is loaded once per wave and only the
+// last block of a cluster is partly empty.  Shifted pieces (clusters near the
+// box edge) run as 8 x 8 chunks: lane (a, b) = (lane / 8, lane % 8) pairs query
+// a with point b of each 8-point chunk; the chunk loop runs on the scalar unit
+// with the next two chunks in flight, and only the last chunk of a piece (which
+// may run past it; the point arrays are padded for it) tests positions per
+// lane.  Query lanes and candidates that do not exist carry a NaN coordinate,
+// which fails every range compare.
 template<int R, bool IS2D, bool HAS_LO, class Bins>
 __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row, uint32_t A0, uint32_t na,
-                                 int cx_lo, int cx_hi, float q_rmax_sq, float lo, float hi, const Bins& B)
+                                 int cx_lo, int cx_hi, int RX, float q_rmax_sq, float lo, float hi, const Bins& B)
 {
     constexpr int kRowsY = 2 * R + 1;
     constexpr int Rz = IS2D ? 0 : R;
@@ -471,11 +488,36 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
         pa.z = 0.0f;
     if (a >= na)
         pa.x = __builtin_nanf("");
+    // the cluster's query points, wave-uniform (NaN beyond na)
+    float qsx[8], qsy[8], qsz[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+        qsx[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa.x), 8 * k));
+        qsy[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa.y), 8 * k));
+        qsz[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa.z), 8 * k));
+    }
     // chunk loads through a buffer resource: wave-uniform chunk offset in an
     // SGPR, the lane's offset in a VGPR (point arrays stay below 4 GiB)
     uint32_t const voff = bl * uint32_t(sizeof(float4));
     __amdgpu_buffer_rsrc_t const rs =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(g.cpos), short(0), -1, kBufferWord3);
+    {
+        // pairs within the cluster: lane (a, b) with a < b < na, no periodic shift
+        float4 const c = g.cpos[A0 + min(bl, na - 1u)];
+        float const rx = c.x - pa.x;
+        float const ry = c.y - pa.y;
+        float x = rx * rx + ry * ry;
+        if constexpr (!IS2D)
+        {
+            float const rz = c.z - pa.z;
+            x = x + rz * rz;
+        }
+        bool ok = x < hi && bl > a && bl < na;
+        if constexpr (HAS_LO)
+            ok = ok && x >= lo;
+        B.add2(x, ok, 0.0f, false, 2u);
+    }
     for (int r0 = 0; r0 < nrows; r0 += 64)
     {
         // lane t: stencil row r0 + t -> pieces (beg, end, image code); code < 0: no piece
@@ -488,14 +530,14 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
             int const uz = cz0 + dz, uy = cy0 + dy;
             int const sz = fa::floor_div(uz, dim2), sy = fa::floor_div(uy, dim1);
             float const gy = float(max(0, abs(dy) - 1)) * wy, gz = float(max(0, abs(dz) - 1)) * wz;
-            int xr = R;
+            int xr = RX;
             bool reach = true;
             if (ortho)
             {
                 float const rem = r_lim * r_lim - (gy * gy + gz * gz);
                 reach = rem > 0.0f;
                 if (reach && wx > 0.0f)
-                    xr = min(R, int(sqrtf(rem) / wx) + 1);
+                    xr = min(RX, int(sqrtf(rem) / wx) + 1);
             }
             else
                 reach = fmaxf(gy, gz) < r_lim;
@@ -527,25 +569,98 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
             }
         }
         int const nr = min(64, nrows - r0);
+        // Unshifted pieces (image code 13): one per row at most, from the point
+        // after the cluster on (pairs are counted from their earlier point, weight
+        // 2; pairs within the cluster were counted above).  Their candidates are
+        // concatenated in row order and taken 64 at a time, one candidate per
+        // lane, against the cluster's query points in turn: each candidate is
+        // loaded once per wave.
+        uint32_t b13 = 0u, l13 = 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            if (pc[k] == 13)
+            {
+                uint32_t const bb = max(pb[k], A0 + na);
+                if (pe[k] > bb)
+                {
+                    b13 = bb;
+                    l13 = pe[k] - bb;
+                }
+            }
+        }
+        uint32_t const p13 = wave_exclusive_scan(l13); // virtual start of row t's candidates
+        uint32_t const e13 = p13 + l13;
+        uint32_t const sh13 = b13 - p13; // position = virtual index + sh13 within row t
+        uint32_t const tot = uint32_t(__builtin_amdgcn_readlane(int(e13), 63));
+        uint64_t live = __builtin_amdgcn_ballot_w64(l13 > 0u); // rows not yet exhausted
+        // positions of the candidates [v0, v0 + 64) (one per lane); rows that end
+        // inside the block are retired
+        auto locate = [&](uint32_t v0) {
+            uint32_t const v = v0 + lane;
+            uint32_t pos = v + uint32_t(__builtin_amdgcn_readlane(int(sh13), __builtin_ctzll(live)));
+            for (uint64_t m = live & (live - 1u); m; m &= m - 1u)
+            {
+                int const r = __builtin_ctzll(m);
+                uint32_t const sp = uint32_t(__builtin_amdgcn_readlane(int(p13), r));
+                if (sp >= v0 + 64u)
+                    break;
+                uint32_t const p2 = v + uint32_t(__builtin_amdgcn_readlane(int(sh13), r));
+                pos = v >= sp ? p2 : pos;
+            }
+            while (live && uint32_t(__builtin_amdgcn_readlane(int(e13), __builtin_ctzll(live))) <= v0 + 64u)
+                live &= live - 1u;
+            return pos;
+        };
+        auto fetch = [&](uint32_t pos) {
+            return __builtin_amdgcn_raw_buffer_load_b128(rs, pos * uint32_t(sizeof(float4)), 0u, 0);
+        };
+        // the next block's candidates are loaded before the current block is binned
+        auto cvn = fetch(tot ? locate(0u) : 0u);
+        for (uint32_t v0 = 0; v0 < tot; v0 += 64u)
+        {
+            auto const cv = cvn;
+            if (v0 + 64u < tot)
+                cvn = fetch(locate(v0 + 64u));
+            float const cx = v0 + lane < tot ? __uint_as_float(cv[0]) : __builtin_nanf("");
+            float const cy = __uint_as_float(cv[1]);
+            float const cz = IS2D ? 0.0f : __uint_as_float(cv[2]);
+            auto rsq = [&](int k) {
+                float const rx = cx - qsx[k];
+                float const ry = cy - qsy[k];
+                if constexpr (IS2D)
+                    return rx * rx + ry * ry;
+                float const rz = cz - qsz[k];
+                return (rx * rx + ry * ry) + rz * rz;
+            };
+#pragma unroll
+            for (int k = 0; k < 8; k += 2)
+            {
+                if (uint32_t(k) >= na)
+                    break;
+                float const x0 = rsq(k), x1 = rsq(k + 1); // x1 is NaN when k + 1 >= na
+                bool ok0 = x0 < hi, ok1 = x1 < hi;
+                if constexpr (HAS_LO)
+                {
+                    ok0 = ok0 && x0 >= lo;
+                    ok1 = ok1 && x1 >= lo;
+                }
+                B.add2(x0, ok0, x1, ok1, 2u);
+            }
+        }
+        // shifted pieces (across the box edge), weight 1, as 8 x 8 chunks
         for (int r = 0; r < nr; ++r)
         {
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
                 int const code = __builtin_amdgcn_readlane(pc[k], r);
-                if (code < 0)
+                if (code < 0 || code == 13)
                     continue;
-                uint32_t beg = uint32_t(__builtin_amdgcn_readlane(int(pb[k]), r));
+                uint32_t const beg = uint32_t(__builtin_amdgcn_readlane(int(pb[k]), r));
                 uint32_t const end = uint32_t(__builtin_amdgcn_readlane(int(pe[k]), r));
-                bool const wraps = code != 13;
-                if (!wraps)
-                    beg = max(beg, A0);
                 if (beg >= end)
                     continue;
-                uint32_t const weight = wraps ? 1u : 2u;
-                // chunks starting below ord_lim may hold pairs of the cluster with
-                // itself or with its earlier points
-                uint32_t const ord_lim = wraps ? 0u : A0 + na;
                 float const qx = pa.x + b.img[code][0];
                 float const qy = pa.y + b.img[code][1];
                 float const qz = pa.z + b.img[code][2];
@@ -569,13 +684,12 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
                         ok0 = ok0 && x0 >= lo;
                         ok1 = ok1 && x1 >= lo;
                     }
-                    if (!(jb + 16u <= end && jb >= ord_lim))
+                    if (jb + 16u > end)
                     {
-                        uint32_t const p0 = jb + bl, p1 = jb + 8u + bl;
-                        ok0 = ok0 && p0 < end && (jb >= ord_lim || p0 > pos_a);
-                        ok1 = ok1 && p1 < end && (jb + 8u >= ord_lim || p1 > pos_a);
+                        ok0 = ok0 && jb + bl < end;
+                        ok1 = ok1 && jb + 8u + bl < end;
                     }
-                    B.add2(x0, ok0, x1, ok1, weight);
+                    B.add2(x0, ok0, x1, ok1, 1u);
                 };
                 // two register sets in turn: the loads of the next 16 points are in
                 // flight while the current 16 are binned
@@ -602,8 +716,8 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
 // The pair work of one cluster entry e = (frame-local row, start, count,
 // cx_lo | cx_hi << 16) for one wave (e is wave-uniform).
 template<int R, class Bins>
-__device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e, float q_rmax_sq, float q_rmin_sq,
-                                     const Bins& B)
+__device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e, int RX, float q_rmax_sq,
+                                     float q_rmin_sq, const Bins& B)
 {
     uint32_t const row = __builtin_amdgcn_readfirstlane(e.x);
     uint32_t const A0 = __builtin_amdgcn_readfirstlane(e.y);
@@ -615,22 +729,22 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
     if (b.is2D)
     {
         if (lo > 0.0f)
-            rows_body<R, true, true>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+            rows_body<R, true, true>(b, g, row, A0, na, cx_lo, cx_hi, RX, q_rmax_sq, lo, hi, B);
         else
-            rows_body<R, true, false>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+            rows_body<R, true, false>(b, g, row, A0, na, cx_lo, cx_hi, RX, q_rmax_sq, lo, hi, B);
     }
     else
     {
         if (lo > 0.0f)
-            rows_body<R, false, true>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+            rows_body<R, false, true>(b, g, row, A0, na, cx_lo, cx_hi, RX, q_rmax_sq, lo, hi, B);
         else
-            rows_body<R, false, false>(b, g, row, A0, na, cx_lo, cx_hi, q_rmax_sq, lo, hi, B);
+            rows_body<R, false, false>(b, g, row, A0, na, cx_lo, cx_hi, RX, q_rmax_sq, lo, hi, B);
     }
 }
 
 template<int R, bool LDS_HIST>
 __global__ __launch_bounds__(kBlock) void rdf_rows_kernel(DevBox b, DevGrid g, const uint4* __restrict__ table,
-                                                          const uint32_t* __restrict__ ncl_ptr, float q_rmax_sq,
+                                                          const uint32_t* __restrict__ ncl_ptr, int RX, float q_rmax_sq,
                                                           float q_rmin_sq, const float* __restrict__ T,
                                                           uint32_t nbins, float vmin, float inv_width,
                                                           uint32_t* __restrict__ counts)
@@ -644,10 +758,10 @@ __global__ __launch_bounds__(kBlock) void rdf_rows_kernel(DevBox b, DevGrid g, c
     if (I < *ncl_ptr)
     {
         if constexpr (LDS_HIST)
-            rdf_rows_wave<R>(b, g, table[I], q_rmax_sq, q_rmin_sq,
-                             LutBins(sT, nbins, vmin, inv_width, hist));
+            rdf_rows_wave<R>(b, g, table[I], RX, q_rmax_sq, q_rmin_sq, LutBins(sT, nbins, vmin, inv_width, hist));
         else
-            rdf_rows_wave<R>(b, g, table[I], q_rmax_sq, q_rmin_sq, GlobalBins{T, nbins, vmin, inv_width, counts});
+            rdf_rows_wave<R>(b, g, table[I], RX, q_rmax_sq, q_rmin_sq,
+                             GlobalBins{T, nbins, vmin, inv_width, counts});
     }
     hist_epilogue<LDS_HIST>(hist, nbins, counts);
 }
@@ -772,18 +886,32 @@ struct RowsArgs
     uint32_t bins;
     float r_min, inv_width, q_r_max;
 };
-constexpr int kRowVariants = 3; // R = 2, 3, 4
+// Launch variants of the self RDF: cells at least r_max / R wide across the
+// rows and F times narrower along them (x), so a row's candidate range follows
+// the sphere's chord more closely.
+struct RowVariant
+{
+    int R, F;
+};
+constexpr RowVariant kRowVariant[] = {{2, 1}, {3, 1}, {4, 1}, {2, 2}, {2, 4}, {3, 2}, {2, 8}, {3, 4}};
+constexpr int kRowVariants = int(sizeof(kRowVariant) / sizeof(kRowVariant[0]));
 
-// Self RDF through the row-cluster kernel with cells at least r_max / R wide.
-// Sets *handled = false (and does nothing) when the kernel does not apply:
-// points outside the box, or rows too long for the cluster encoding.
-int run_rows(const RowsArgs& a, int R, uint32_t* d_counts, bool* handled)
+// Self RDF through the row-cluster kernel with cells at least r_max / R wide
+// (r_max / (R F) along x).  Sets *handled = false (and does nothing) when the
+// kernel does not apply: points outside the box, or rows too long for the
+// cluster encoding.
+int run_rows(const RowsArgs& a, RowVariant rv, uint32_t* d_counts, bool* handled)
 {
     *handled = false;
     fa_nq* nq = a.nq;
     hipStream_t s = fa::stream();
+    int const R = rv.R;
     int dims[3];
     fa::grid_dims_for(nq->box, nq->n, a.q_r_max, R, dims);
+    int F = rv.F;
+    if (dims[0] * F > 65535 || double(dims[0]) * F * dims[1] * dims[2] > std::max<double>(8.0 * nq->n, 64.0))
+        F = 1;
+    dims[0] *= F;
     // cluster entries hold cell x in 16 bits; chunk loads use 32-bit byte offsets
     if (dims[0] > 65535 || uint64_t(nq->n) + fa::kCposPad > (1ull << 28))
         return FA_OK;
@@ -818,11 +946,11 @@ int run_rows(const RowsArgs& a, int R, uint32_t* d_counts, bool* handled)
     case RR:                                                                                                        \
         if (lds)                                                                                                    \
             FA_KLAUNCH((rdf_rows_kernel<RR, true>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,            \
-                       cl_off + nrows, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,        \
+                       cl_off + nrows, R * F, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,  \
                        a.inv_width, d_counts);                                                                      \
         else                                                                                                        \
             FA_KLAUNCH((rdf_rows_kernel<RR, false>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,           \
-                       cl_off + nrows, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,        \
+                       cl_off + nrows, R * F, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,  \
                        a.inv_width, d_counts);                                                                      \
         break;
     switch (R)
@@ -982,13 +1110,13 @@ __global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, cons
             uint4 const e = table[I];
             if (!F.any_outside)
             {
-                rdf_rows_wave<R>(F.box, g, e, q_rmax_sq, 0.0f, LutBins(sT, nbins, vmin, inv_width, wh));
+                rdf_rows_wave<R>(F.box, g, e, F.rx, q_rmax_sq, 0.0f, LutBins(sT, nbins, vmin, inv_width, wh));
             }
             else if (lane < e.z)
             {
                 float4 const q = g.cpos[e.y + lane];
                 uint32_t const qi = __float_as_uint(q.w);
-                fa::for_each_candidate<AABB_FORMULA>(F.box, g, q.x, q.y, q.z, R, false,
+                fa::for_each_candidate<AABB_FORMULA>(F.box, g, q.x, q.y, q.z, F.rx, false,
                                                      [&](uint32_t j, float, float, float, float r_sq, int) {
                                                          if (j == qi || !(r_sq < q_rmax_sq))
                                                              return;
@@ -1138,11 +1266,11 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
                 float ms = 0.0f;
                 bool ok = false;
                 // first run builds the grid; the second is timed
-                if (int rc = run_rows(ra, 2 + v, trial, &ok))
+                if (int rc = run_rows(ra, kRowVariant[v], trial, &ok))
                     return rc;
                 if (!ok)
                     break;
-                if (int rc = fa::time_launch([&] { return run_rows(ra, 2 + v, trial, &ok); }, &ms))
+                if (int rc = fa::time_launch([&] { return run_rows(ra, kRowVariant[v], trial, &ok); }, &ms))
                     return rc;
                 if (variant < 0 || ms < best)
                 {
@@ -1153,7 +1281,7 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
             if (variant >= 0)
                 fa::record_variant(fa::TUNE_RDF_SELF, key, variant);
         }
-        if (int rc = run_rows(ra, 2 + std::max(variant, 0), d_counts, &handled))
+        if (int rc = run_rows(ra, kRowVariant[std::max(variant, 0) % kRowVariants], d_counts, &handled))
             return rc;
         if (handled)
         {

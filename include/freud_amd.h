@@ -168,9 +168,9 @@ int fa_batch_rdf_counts(fa_batch* b, uint32_t** d_counts);
  * 0..l, -1..-l) of the last run; order (host, n_frames) = each frame's
  * system q_l.  Any output pointer may be NULL. */
 int fa_batch_ql(fa_batch* b, float** d_ql, float** d_qlm, float* order);
-/* The launch shape the autotuner chose (cells r/R wide), the bond capacity and
- * whether a graph is captured. */
-int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, uint64_t* capacity, int* graph_ready);
+/* The launch shape the autotuner chose (cells r/R wide; RDF cells rdf_F times
+ * narrower along x), the bond capacity and whether a graph is captured. */
+int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, int* rdf_F, uint64_t* capacity, int* graph_ready);
 
 /* ---------------------------------------------------------------- box
  * freud/box/Box.h:212-329: op 0 makeFractional, 1 makeAbsolute, 2 wrap over n

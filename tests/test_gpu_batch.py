@@ -131,3 +131,29 @@ def test_batch_matches_single_frame_api():
         npt.assert_array_equal(b.rdf_counts[f], rdf.bin_counts)
         st = freud_amd.order.Steinhardt(L_Q).compute(aq, neighbors=nl)
         npt.assert_allclose(b.ql()[f], st.particle_order, atol=1e-6, equal_nan=True)
+
+
+def test_batch_rdf_every_launch_shape(oracle):
+    """Each batched RDF launch shape the autotuner can pick ((R, Fx) = (2, 1),
+    (3, 1), (4, 1), (2, 4), (2, 8): cells r/R wide, Fx times narrower along x)
+    gives the oracle's counts in every frame."""
+    from freud_amd import _lib
+
+    frames = _frames(seed=11)
+    pts_total = sum(len(p) for _, p in frames)
+    vol = sum(b.volume for b, _ in frames)
+    key = int(np.floor(2.0 * np.log2(pts_total / vol * R_RDF ** 3) + 0.5)) + 1000
+    ref = [oracle.rdf_accumulate(_bt(box), pts, pts, BINS, R_RDF, exclude_ii=True, nthreads=8)
+           for box, pts in frames]
+    before = _lib.tune_table()
+    try:
+        for shape in range(5):
+            _lib.tune_load({(1, key): shape})
+            b = _make(frames).compute(use_graph=False)
+            cfg = b.config
+            assert (cfg["rdf_R"], cfg["rdf_F"]) == ((2, 1), (3, 1), (4, 1), (2, 4), (2, 8))[shape]
+            counts = b.rdf_counts
+            for f in range(len(frames)):
+                npt.assert_array_equal(counts[f], ref[f], err_msg=f"shape {shape} frame {f}")
+    finally:
+        _lib.tune_load({(1, key): before.get((1, key), -1)})
