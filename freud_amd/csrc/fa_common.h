@@ -180,8 +180,8 @@ struct DevGrid
     const int* wrapc;
     const uint32_t* cstart; // ncells + 1
     int dim[3];
-    int any_outside;
-    float cw[3]; // cell extent along each plane normal (nearest plane distance / dim)
+    const int* outside; // device flag: some point lies outside the box (set by the grid build)
+    float cw[3];        // cell extent along each plane normal (nearest plane distance / dim)
 };
 
 __device__ __host__ inline int floor_div(int a, int b)
@@ -256,7 +256,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
     if (FORMULA == AABB_FORMULA)
     {
         int const nb0 = -wq[0] - sx, nb1 = -wq[1] - sy, nb2 = -wq[2] - sz;
-        if (!g.any_outside)
+        if (!*g.outside)
         {
             if (nb0 < -1 || nb0 > 1 || nb1 < -1 || nb1 > 1 || nb2 < -1 || nb2 > 1)
                 return;
@@ -316,14 +316,14 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
 // cells at Chebyshev offset exactly R (AABB formula only).
 template<int FORMULA, typename F>
 __device__ inline void for_each_candidate(const DevBox& b, const DevGrid& g, float qx, float qy, float qz, int R,
-                                          bool ring_only, F&& f)
+                                          bool ring_only, F&& f, int RX = -1)
 {
     int cq[3], wq[3];
     cell_of(b, g.dim, qx, qy, qz, cq, wq);
     int lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        stencil_interval<FORMULA>(R, g.dim[k], lo[k], hi[k]);
+        stencil_interval<FORMULA>((k == 0 && RX >= 0) ? RX : R, g.dim[k], lo[k], hi[k]);
     if (b.is2D)
         lo[2] = hi[2] = 0;
     float const qzz = (FORMULA == AABB_FORMULA && b.is2D) ? 0.0f : qz;
@@ -390,7 +390,7 @@ struct Grid
     int* wrapc = nullptr;
     uint32_t* cstart = nullptr;
     uint32_t* cell_of_point = nullptr; // scratch
-    int any_outside = 0;
+    int* outside = nullptr;            // device flag: some point lies outside the box
     float cw[3] = {0.0f, 0.0f, 0.0f};
     DevGrid dev() const
     {
@@ -401,7 +401,7 @@ struct Grid
         g.dim[0] = dim[0];
         g.dim[1] = dim[1];
         g.dim[2] = dim[2];
-        g.any_outside = any_outside;
+        g.outside = outside;
         g.cw[0] = cw[0];
         g.cw[1] = cw[1];
         g.cw[2] = cw[2];
@@ -480,7 +480,7 @@ __device__ inline DevGrid frame_grid(const BatchFrame& f, const BatchGrid& G)
     g.dim[0] = f.dim[0];
     g.dim[1] = f.dim[1];
     g.dim[2] = f.dim[2];
-    g.any_outside = f.any_outside;
+    g.outside = &f.any_outside;
     g.cw[0] = f.cw[0];
     g.cw[1] = f.cw[1];
     g.cw[2] = f.cw[2];

@@ -244,6 +244,7 @@ Grid::~Grid()
     device_free(wrapc);
     device_free(cstart);
     device_free(cell_of_point);
+    device_free(outside);
 }
 
 int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, const int dims[3])
@@ -289,15 +290,20 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
         for (int k = 0; k < 3; ++k)
             g.cw[k] = (k == 2 && box.is2D) ? 0.0f : npd[k] / float(dims[k]);
     }
-    // scratch: counts (ncell) + fill (ncell) + any_outside flag
+    // scratch: counts (ncell) + fill (ncell); the outside flag lives with the
+    // grid, read by the kernels (no host round trip)
     void* tmp = nullptr;
     size_t const cb = sizeof(uint32_t) * size_t(ncell);
     if (int rc = scratch(2, 2 * cb + 256, &tmp))
         return rc;
     uint32_t* counts = static_cast<uint32_t*>(tmp);
     uint32_t* fill = counts + ncell;
-    int* d_any = reinterpret_cast<int*>(fill + ncell);
-    FA_HIP(hipMemsetAsync(tmp, 0, 2 * cb + sizeof(int), s));
+    if (!g.outside)
+        if (int rc = device_alloc((void**) &g.outside, 256))
+            return rc;
+    int* d_any = g.outside;
+    FA_HIP(hipMemsetAsync(tmp, 0, 2 * cb, s));
+    FA_HIP(hipMemsetAsync(d_any, 0, sizeof(int), s));
     unsigned const blocks = (n + 255) / 256;
     if (n)
     {
@@ -313,8 +319,6 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
                            dims[2], g.cell_of_point, g.cstart, fill, g.cpos, g.wrapc);
         FA_HIP(hipGetLastError());
     }
-    FA_HIP(hipMemcpyAsync(&g.any_outside, d_any, sizeof(int), hipMemcpyDeviceToHost, s));
-    FA_HIP(hipStreamSynchronize(s));
     return FA_OK;
 }
 

@@ -226,6 +226,32 @@ __device__ inline uint32_t bin_in_range(float x, const float* __restrict__ T, ui
     return uint32_t(k);
 }
 
+// The reference's bin of r_sq, given T in LDS; nbins when outside [T0, Tn).
+__device__ inline uint32_t bin_of_rsq(float x, const float* __restrict__ T, uint32_t nbins, float vmin,
+                                      float inv_width)
+{
+    if (!(x >= T[0]) || !(x < T[nbins]))
+        return nbins;
+    float const d = __builtin_amdgcn_sqrtf(x);
+    float v = (d - vmin) * inv_width;
+    int k = int(fminf(fmaxf(v, 0.0f), float(nbins - 1)));
+    // the estimate is off by at most one bin; the loops are exact for any k
+    float const t0 = T[k], t1 = T[k + 1];
+    if (x < t0)
+    {
+        --k;
+        while (k > 0 && x < T[k])
+            --k;
+    }
+    else if (x >= t1)
+    {
+        ++k;
+        while (k < int(nbins) - 1 && x >= T[k + 1])
+            ++k;
+    }
+    return uint32_t(k);
+}
+
 // LDS form of the bin table: T[0..nbins], inv_delta and x_fast, then the kLut
 // cell entries as uint32 LDS byte addresses of T[k + 1] (cells below x_fast
 // point at T[1]: their lookups are discarded), then the histogram.
@@ -274,6 +300,13 @@ struct LutBins
     {
         __atomic_fetch_add((lds_u32*) (uintptr_t) a, w, __ATOMIC_RELAXED);
     }
+    // one r_sq of any value, weight 1
+    __device__ void add1(float x) const
+    {
+        uint32_t const bin = bin_of_rsq(x, sT, nbins, vmin, inv_width);
+        if (bin < nbins)
+            atomic(hist + 4u * bin, 1u);
+    }
     __device__ __forceinline__ void add2(float x0, bool ok0, float x1, bool ok1, uint32_t w) const
     {
         bool const f0 = ok0 && x0 >= x_fast, f1 = ok1 && x1 >= x_fast;
@@ -302,6 +335,12 @@ struct GlobalBins
     uint32_t* counts;
     __device__ float t0() const { return T[0]; }
     __device__ float tn() const { return T[nbins]; }
+    __device__ void add1(float x) const
+    {
+        uint32_t const bin = bin_of_rsq(x, T, nbins, vmin, inv_width);
+        if (bin < nbins)
+            atomicAdd(counts + bin, 1u);
+    }
     __device__ void add2(float x0, bool ok0, float x1, bool ok1, uint32_t w) const
     {
         if (ok0)
@@ -311,31 +350,6 @@ struct GlobalBins
     }
 };
 
-// The reference's bin of r_sq, given T in LDS; nbins when outside [T0, Tn).
-__device__ inline uint32_t bin_of_rsq(float x, const float* __restrict__ T, uint32_t nbins, float vmin,
-                                      float inv_width)
-{
-    if (!(x >= T[0]) || !(x < T[nbins]))
-        return nbins;
-    float const d = __builtin_amdgcn_sqrtf(x);
-    float v = (d - vmin) * inv_width;
-    int k = int(fminf(fmaxf(v, 0.0f), float(nbins - 1)));
-    // the estimate is off by at most one bin; the loops are exact for any k
-    float const t0 = T[k], t1 = T[k + 1];
-    if (x < t0)
-    {
-        --k;
-        while (k > 0 && x < T[k])
-            --k;
-    }
-    else if (x >= t1)
-    {
-        ++k;
-        while (k < int(nbins) - 1 && x >= T[k + 1])
-            ++k;
-    }
-    return uint32_t(k);
-}
 
 // LDS copy of the bin table (lds_table_floats layout); hist_prologue also
 // clears the LDS histogram that follows it.  Kernels with device-memory counts
@@ -724,6 +738,26 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
     uint32_t const na = __builtin_amdgcn_readfirstlane(e.z);
     uint32_t const cx = __builtin_amdgcn_readfirstlane(e.w);
     int const cx_lo = int(cx & 0xffffu), cx_hi = int(cx >> 16);
+    if (*g.outside)
+    {
+        // some point of the grid lies outside the box: lane a searches the stencil
+        // of query point a through the general cell walk (AABB formula with each
+        // point's own image offset) and counts each ordered pair once
+        uint32_t const lane = threadIdx.x & 63u;
+        if (lane < na)
+        {
+            float4 const q = g.cpos[A0 + lane];
+            uint32_t const qi = __float_as_uint(q.w);
+            fa::for_each_candidate<AABB_FORMULA>(
+                b, g, q.x, q.y, q.z, R, false,
+                [&](uint32_t j, float, float, float, float r_sq, int) {
+                    if (j != qi && r_sq < q_rmax_sq && r_sq >= q_rmin_sq)
+                        B.add1(r_sq);
+                },
+                RX);
+        }
+        return;
+    }
     // pairs counted: q_rmin_sq <= r_sq < q_rmax_sq (the query) and inside the histogram
     float const lo = fmaxf(q_rmin_sq, B.t0()), hi = fminf(q_rmax_sq, B.tn());
     if (b.is2D)
@@ -743,7 +777,7 @@ __device__ inline void rdf_rows_wave(const DevBox& b, const DevGrid& g, uint4 e,
 }
 
 template<int R, bool LDS_HIST>
-__global__ __launch_bounds__(kBlock) void rdf_rows_kernel(DevBox b, DevGrid g, const uint4* __restrict__ table,
+__global__ __launch_bounds__(kBlock, 8) void rdf_rows_kernel(DevBox b, DevGrid g, const uint4* __restrict__ table,
                                                           const uint32_t* __restrict__ ncl_ptr, int RX, float q_rmax_sq,
                                                           float q_rmin_sq, const float* __restrict__ T,
                                                           uint32_t nbins, float vmin, float inv_width,
@@ -917,8 +951,6 @@ int run_rows(const RowsArgs& a, RowVariant rv, uint32_t* d_counts, bool* handled
         return FA_OK;
     if (int rc = fa::ensure_grid(nq, dims))
         return rc;
-    if (nq->cur->any_outside)
-        return FA_OK;
     *handled = true;
     DevGrid const g = nq->cur->dev();
     DevBox const& db = nq->cur->dbox;
@@ -1107,25 +1139,7 @@ __global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, cons
             }
             const fa::BatchFrame& F = G.frames[f];
             DevGrid const g = fa::frame_grid(F, G);
-            uint4 const e = table[I];
-            if (!F.any_outside)
-            {
-                rdf_rows_wave<R>(F.box, g, e, F.rx, q_rmax_sq, 0.0f, LutBins(sT, nbins, vmin, inv_width, wh));
-            }
-            else if (lane < e.z)
-            {
-                float4 const q = g.cpos[e.y + lane];
-                uint32_t const qi = __float_as_uint(q.w);
-                fa::for_each_candidate<AABB_FORMULA>(F.box, g, q.x, q.y, q.z, F.rx, false,
-                                                     [&](uint32_t j, float, float, float, float r_sq, int) {
-                                                         if (j == qi || !(r_sq < q_rmax_sq))
-                                                             return;
-                                                         uint32_t const bin =
-                                                             bin_of_rsq(r_sq, sT, nbins, vmin, inv_width);
-                                                         if (bin < nbins)
-                                                             atomicAdd(&wh[bin], 1u);
-                                                     });
-            }
+            rdf_rows_wave<R>(F.box, g, table[I], F.rx, q_rmax_sq, 0.0f, LutBins(sT, nbins, vmin, inv_width, wh));
         }
     }
     if (cur != 0xffffffffu)
