@@ -264,15 +264,21 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
             float const ix = qx + b.img[code][0];
             float const iy = qy + b.img[code][1];
             float const iz = qzz + b.img[code][2];
+            if (beg >= end)
+                return;
+            // the next candidate is loaded before the current one is used (the
+            // point arrays are padded, so cpos[end] is readable)
+            float4 p = g.cpos[beg];
             for (uint32_t t = beg; t < end; ++t)
             {
-                float4 const p = g.cpos[t];
+                float4 const pn = g.cpos[t + 1];
                 float const pz = b.is2D ? 0.0f : p.z;
                 float const rx = p.x - ix;
                 float const ry = p.y - iy;
                 float const rz = pz - iz;
                 float const r_sq = (rx * rx + ry * ry) + rz * rz;
                 f(__float_as_uint(p.w), rx, ry, rz, r_sq, code);
+                p = pn;
             }
         }
         else
@@ -297,13 +303,17 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
     }
     else
     {
+        if (beg >= end)
+            return;
+        float4 p = g.cpos[beg];
         for (uint32_t t = beg; t < end; ++t)
         {
-            float4 const p = g.cpos[t];
+            float4 const pn = g.cpos[t + 1];
             float rx, ry, rz;
             wrap(b, p.x - qx, p.y - qy, p.z - qz, rx, ry, rz);
             float const r_sq = (rx * rx + ry * ry) + rz * rz;
             f(__float_as_uint(p.w), rx, ry, rz, r_sq, 13);
+            p = pn;
         }
     }
 }

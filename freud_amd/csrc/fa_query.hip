@@ -88,9 +88,8 @@ __device__ inline void ball_count_body(const DevBox& b, const DevGrid& g, const 
     uint32_t c = 0;
     fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
                                     [&](uint32_t j, float, float, float, float r_sq, int) {
-                                        if (exclude_ii && j == i)
-                                            return;
-                                        if (r_sq < r_max_sq && r_sq >= r_min_sq)
+                                        // range first: the index is needed for in-range candidates only
+                                        if (r_sq < r_max_sq && r_sq >= r_min_sq && !(exclude_ii && j == i))
                                             ++c;
                                     });
     counts[i] = c;
@@ -212,9 +211,8 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
         uint32_t k = beg;
         fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
                                         [&](uint32_t j, float, float, float, float r_sq, int code) {
-                                            if (exclude_ii && j == i)
-                                                return;
-                                            if (r_sq < r_max_sq && r_sq >= r_min_sq && k < end)
+                                            if (r_sq < r_max_sq && r_sq >= r_min_sq && k < end &&
+                                                !(exclude_ii && j == i))
                                             {
                                                 uint32_t const jc = (j << 5) | uint32_t(code);
                                                 if (BYDIST)
@@ -320,11 +318,12 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
     {
         fa::for_each_candidate<AABB_FORMULA>(b, g, qx, qy, qz, R, true,
                                              [&](uint32_t j, float, float, float, float r_sq, int code) {
-                                                 if (exclude_ii && j == i)
-                                                     return;
                                                  if (!(r_sq < rlim_sq))
                                                      return;
-                                                 if (!(sqrtf(r_sq) >= r_min))
+                                                 if (exclude_ii && j == i)
+                                                     return;
+                                                 // the distance test only matters for r_min > 0
+                                                 if (r_min > 0.0f && !(sqrtf(r_sq) >= r_min))
                                                      return;
                                                  ++found;
                                                  best.insert((uint64_t(__float_as_uint(r_sq)) << 32)
@@ -386,9 +385,7 @@ __global__ __launch_bounds__(kBlock) void knn_linkcell_kernel(DevBox b, DevGrid 
     uint32_t found = 0;
     fa::for_each_candidate<LINKCELL_FORMULA>(b, g, qx, qy, qz, R, false,
                                              [&](uint32_t j, float, float, float, float r_sq, int) {
-                                                 if (exclude_ii && j == i)
-                                                     return;
-                                                 if (r_sq < r_max_sq && r_sq >= r_min_sq)
+                                                 if (r_sq < r_max_sq && r_sq >= r_min_sq && !(exclude_ii && j == i))
                                                  {
                                                      ++found;
                                                      best.insert((uint64_t(__float_as_uint(r_sq)) << 32)

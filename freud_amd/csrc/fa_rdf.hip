@@ -820,9 +820,7 @@ __global__ __launch_bounds__(kBlock) void rdf_self_kernel(DevBox b, DevGrid g, u
         uint32_t const qi = __float_as_uint(q.w);
         fa::for_each_candidate<FORMULA>(b, g, q.x, q.y, q.z, R, false,
                                         [&](uint32_t j, float, float, float, float r_sq, int) {
-                                            if (exclude_ii && j == qi)
-                                                return;
-                                            if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq)
+                                            if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq && !(exclude_ii && j == qi))
                                             {
                                                 uint32_t const bin = bin_of_rsq(r_sq, tab, nbins, vmin, inv_width);
                                                 if (bin < nbins)
@@ -852,9 +850,7 @@ __global__ __launch_bounds__(kBlock) void rdf_query_kernel(DevBox b, DevGrid g, 
         float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
         fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
                                         [&](uint32_t j, float, float, float, float r_sq, int) {
-                                            if (exclude_ii && j == i)
-                                                return;
-                                            if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq)
+                                            if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq && !(exclude_ii && j == i))
                                             {
                                                 uint32_t const bin = bin_of_rsq(r_sq, tab, nbins, vmin, inv_width);
                                                 if (bin < nbins)
