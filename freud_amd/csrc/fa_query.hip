@@ -27,7 +27,18 @@ using fa::DevGrid;
 using fa::LINKCELL_FORMULA;
 
 constexpr int kBlock = 256;
-constexpr int kKeyBytes = 32768; // LDS staging per block
+constexpr int kKeyBytes = 32768; // largest LDS key staging per block
+extern __shared__ __align__(16) unsigned char fa_dyn_lds[];
+
+// Dynamic LDS bytes for a fill block's keys: 1.3 x the mean bonds of kBlock
+// query points, between 4 KB and kKeyBytes.
+size_t fill_key_bytes(uint64_t bonds, uint64_t points, size_t key_size)
+{
+    double const per = points ? double(bonds) / double(points) : 0.0;
+    double const want = std::ceil(per * kBlock * 1.3) * double(key_size);
+    size_t const b = size_t(std::min(std::max(want, 4096.0), double(kKeyBytes)));
+    return (b + 15) & ~size_t(15);
+}
 
 __device__ inline uint32_t float_order_bits(float v)
 {
@@ -177,11 +188,13 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
                                       int exclude_ii, int R, const uint32_t* __restrict__ offsets,
                                       uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                       float* __restrict__ w_out, float* __restrict__ v_out,
-                                      void* __restrict__ gkeys_raw)
+                                      void* __restrict__ gkeys_raw, uint32_t key_cap)
 {
     using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
-    constexpr uint32_t CAP = kKeyBytes / sizeof(K);
-    __shared__ K s_keys[CAP];
+    // key staging: dynamic LDS sized by the host from the bonds per point, so
+    // the kernel keeps its occupancy (blocks over the capacity use gkeys_raw)
+    K* s_keys = reinterpret_cast<K*>(fa_dyn_lds);
+    uint32_t const CAP = key_cap;
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
@@ -241,11 +254,11 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
                                                            const uint32_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                                            float* __restrict__ w_out, float* __restrict__ v_out,
-                                                           void* __restrict__ gkeys_raw)
+                                                           void* __restrict__ gkeys_raw, uint32_t key_cap)
 {
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     ball_fill_body<FORMULA, BYDIST, SELF>(b, g, qpts, p0, min(uint32_t(kBlock), nq - p0), pts, r_max_sq, r_min_sq,
-                                          exclude_ii, R, offsets, nb_out, d_out, w_out, v_out, gkeys_raw);
+                                          exclude_ii, R, offsets, nb_out, d_out, w_out, v_out, gkeys_raw, key_cap);
 }
 
 // Segments with the reference's convention (0 for query points without bonds).
@@ -492,11 +505,11 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
                                                            const uint32_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                                            float* __restrict__ w_out, float* __restrict__ v_out,
-                                                           uint64_t* __restrict__ tmp)
+                                                           uint64_t* __restrict__ tmp, uint32_t key_cap)
 {
     using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
-    constexpr uint32_t CAP = kKeyBytes / sizeof(K);
-    __shared__ K s_keys[CAP];
+    K* s_keys = reinterpret_cast<K*>(fa_dyn_lds); // key_cap entries (dynamic LDS)
+    uint32_t const CAP = key_cap;
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
@@ -598,7 +611,8 @@ __global__ __launch_bounds__(kBlock) void bball_fill_kernel(fa::BatchGrid G, con
                                                             const unsigned long long* __restrict__ total_bonds,
                                                             uint64_t capacity, uint32_t* __restrict__ nb_out,
                                                             float* __restrict__ d_out, float* __restrict__ w_out,
-                                                            float* __restrict__ v_out, void* __restrict__ gkeys)
+                                                            float* __restrict__ v_out, void* __restrict__ gkeys,
+                                                            uint32_t key_cap)
 {
     // a run whose bonds exceed the allocation writes nothing (the host sees
     // the 64-bit total and runs again with room for it)
@@ -611,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void bball_fill_kernel(fa::BatchGrid G, con
     const float* fp = pts + 3 * size_t(F.pt_base);
     ball_fill_body<AABB_FORMULA, false, true>(F.box, g, fp, F.pt_base + local0, min(fa::kTile, F.n - local0), fp,
                                               r_max_sq, 0.0f, 1, R, offsets + F.pt_base, nb_out, d_out, w_out, v_out,
-                                              gkeys);
+                                              gkeys, key_cap);
 }
 
 } // namespace
@@ -632,9 +646,12 @@ int batch_ball_fill(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t n
                     int R, const uint32_t* d_offsets, const unsigned long long* d_total, uint64_t capacity,
                     uint32_t* nb, float* dist, float* wts, float* vec, void* gkeys)
 {
+    // the key staging is sized from the allocation (bonds per point of the batch)
+    size_t const kb = fill_key_bytes(capacity, G.total_points, sizeof(uint32_t));
     if (ntiles)
-        FA_KLAUNCH(bball_fill_kernel, dim3(ntiles), dim3(kBlock), 0, stream(), G, d_tile_frame, ntiles, d_pts,
-                   r_max * r_max, R, d_offsets, d_total, capacity, nb, dist, wts, vec, gkeys);
+        FA_KLAUNCH(bball_fill_kernel, dim3(ntiles), dim3(kBlock), kb, stream(), G, d_tile_frame, ntiles, d_pts,
+                   r_max * r_max, R, d_offsets, d_total, capacity, nb, dist, wts, vec, gkeys,
+                   uint32_t(kb / sizeof(uint32_t)));
     FA_HIP(hipGetLastError());
     return FA_OK;
 }
@@ -882,11 +899,13 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
     void* gkeys = nullptr;
     if (int rc = fa::scratch(3, std::max<uint64_t>(total, 1) * (by_distance ? 8 : 4), &gkeys))
         return rc;
+    size_t const kb = fill_key_bytes(total, n_query, by_distance ? 8 : 4);
+    uint32_t const key_cap = uint32_t(kb / (by_distance ? 8 : 4));
     auto launch = [&](auto kern) {
         FA_PROF_BEGIN("ball_fill_kernel");
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query, nq->d_points, r_max_sq,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), kb, s, db, g, qpts, n_query, nq->d_points, r_max_sq,
                            r_min_sq, a.exclude_ii, R, out->offsets, out->neighbors, out->distances, out->weights,
-                           out->vectors, gkeys);
+                           out->vectors, gkeys, key_cap);
         FA_PROF_END();
     };
     if (n_query)
@@ -1101,10 +1120,14 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
     bool const self = (qpts == nq->d_points) && (n_query == nq->n);
     DevGrid const g = nq->cur->dev();
+    // at most k keys per query point: the staging holds a whole block's
+    size_t const ksz = by_distance ? 8 : 4;
+    size_t const kb = std::min<size_t>(size_t(k) * kBlock * ksz, size_t(kKeyBytes));
+    uint32_t const key_cap = uint32_t(kb / ksz);
     auto launch = [&](auto kern) {
         FA_PROF_BEGIN("knn_write_kernel");
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, *dbp, g, qpts, n_query, nq->d_points, k, sel,
-                           out->offsets, out->neighbors, out->distances, out->weights, out->vectors, wtmp);
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), kb, s, *dbp, g, qpts, n_query, nq->d_points, k, sel,
+                           out->offsets, out->neighbors, out->distances, out->weights, out->vectors, wtmp, key_cap);
         FA_PROF_END();
     };
     if (nq->kind == FA_NQ_AABB)
