@@ -9,38 +9,79 @@
 
 namespace {
 
-__global__ void grid_assign_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, int d0, int d1, int d2,
-                                   uint32_t* __restrict__ cell_of_point, uint32_t* __restrict__ counts,
-                                   int* __restrict__ any_outside)
+// arr[idx] += 1 for every valid lane of the wave, as one atomic per distinct
+// idx among the first groups of equal indices (points arriving in a spatially
+// coherent order share cells); returns each lane's old value plus its rank in
+// its group, i.e. what a per-lane atomicAdd(&arr[idx], 1) could have returned.
+__device__ inline uint32_t wave_aggregated_increment(uint32_t* arr, uint32_t idx, bool valid)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    int const dim[3] = {d0, d1, d2};
-    int c[3], w[3];
-    fa::cell_of(b, dim, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], c, w);
-    uint32_t const cell = uint32_t((c[2] * d1 + c[1]) * d0 + c[0]);
-    cell_of_point[i] = cell;
-    if (w[0] != 0 || w[1] != 0 || w[2] != 0)
-        atomicOr(any_outside, 1);
-    atomicAdd(&counts[cell], 1u);
+    uint32_t const lane = threadIdx.x & 63u;
+    uint64_t const below = (uint64_t(1) << lane) - 1u;
+    uint64_t pending = __builtin_amdgcn_ballot_w64(valid);
+    uint32_t leader = lane, rank = 0u, cnt = 1u;
+    for (int it = 0; it < 16 && pending; ++it)
+    {
+        int const L = __builtin_ctzll(pending);
+        uint32_t const lidx = uint32_t(__builtin_amdgcn_readlane(int(idx), L));
+        uint64_t const same = __builtin_amdgcn_ballot_w64(idx == lidx) & pending;
+        if ((same >> lane) & 1u)
+        {
+            leader = uint32_t(L);
+            rank = uint32_t(__builtin_popcountll(same & below));
+        }
+        if (lane == uint32_t(L))
+            cnt = uint32_t(__builtin_popcountll(same));
+        pending &= ~same;
+    }
+    // lanes left over (many distinct cells in the wave) count for themselves
+    if ((pending >> lane) & 1u)
+    {
+        leader = lane;
+        rank = 0u;
+        cnt = 1u;
+    }
+    uint32_t base = 0u;
+    if (valid && leader == lane)
+        base = atomicAdd(&arr[idx], cnt);
+    return uint32_t(__shfl(int(base), int(leader))) + rank;
 }
 
-__global__ void grid_scatter_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, int d0, int d1, int d2,
-                                    const uint32_t* __restrict__ cell_of_point, const uint32_t* __restrict__ cstart,
-                                    uint32_t* __restrict__ fill, float4* __restrict__ cpos, int* __restrict__ wrapc)
+__global__ void grid_assign_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, int d0, int d1, int d2,
+                                   uint32_t* __restrict__ cell_of_point, int* __restrict__ wrap_of_point,
+                                   uint32_t* __restrict__ counts, int* __restrict__ any_outside)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    bool const valid = i < n;
+    uint32_t cell = 0u;
+    if (valid)
+    {
+        int const dim[3] = {d0, d1, d2};
+        int c[3], w[3];
+        fa::cell_of(b, dim, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], c, w);
+        cell = uint32_t((c[2] * d1 + c[1]) * d0 + c[0]);
+        cell_of_point[i] = cell;
+        if (wrap_of_point)
+            wrap_of_point[i] = fa::pack_wrap(w);
+        if (w[0] != 0 || w[1] != 0 || w[2] != 0)
+            atomicOr(any_outside, 1);
+    }
+    wave_aggregated_increment(counts, cell, valid);
+}
+
+__global__ void grid_scatter_kernel(const float* __restrict__ pts, uint32_t n,
+                                    const uint32_t* __restrict__ cell_of_point, const int* __restrict__ wrap_of_point,
+                                    const uint32_t* __restrict__ cstart, uint32_t* __restrict__ fill,
+                                    float4* __restrict__ cpos, int* __restrict__ wrapc)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool const valid = i < n;
+    uint32_t const cell = valid ? cell_of_point[i] : 0u;
+    uint32_t const rank = wave_aggregated_increment(fill, cell, valid);
+    if (!valid)
         return;
-    uint32_t const cell = cell_of_point[i];
-    uint32_t const slot = cstart[cell] + atomicAdd(&fill[cell], 1u);
-    float const x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
-    cpos[slot] = make_float4(x, y, z, __uint_as_float(i));
-    int const dim[3] = {d0, d1, d2};
-    int c[3], w[3];
-    fa::cell_of(b, dim, x, y, z, c, w);
-    wrapc[slot] = fa::pack_wrap(w);
+    uint32_t const slot = cstart[cell] + rank;
+    cpos[slot] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], __uint_as_float(i));
+    wrapc[slot] = wrap_of_point[i];
 }
 
 // ---------------------------------------------------------------- batches
@@ -52,16 +93,19 @@ __global__ __launch_bounds__(256) void bgrid_assign_kernel(fa::BatchGrid G, cons
     uint32_t const t = blockIdx.x;
     const fa::BatchFrame& F = G.frames[tile_frame[t]];
     uint32_t const local = (t - F.tile_base) * fa::kTile + threadIdx.x;
-    if (local >= F.n)
-        return;
-    uint32_t const i = F.pt_base + local;
-    int c[3], w[3];
-    fa::cell_of(F.box, F.dim, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], c, w);
-    uint32_t const cell = F.cell_base + uint32_t((c[2] * F.dim[1] + c[1]) * F.dim[0] + c[0]);
-    G.cell_of_point[i] = cell;
-    if (w[0] != 0 || w[1] != 0 || w[2] != 0)
-        atomicOr(&G.frames[tile_frame[t]].any_outside, 1);
-    atomicAdd(&G.fill[cell], 1u);
+    bool const valid = local < F.n;
+    uint32_t cell = 0u;
+    if (valid)
+    {
+        uint32_t const i = F.pt_base + local;
+        int c[3], w[3];
+        fa::cell_of(F.box, F.dim, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], c, w);
+        cell = F.cell_base + uint32_t((c[2] * F.dim[1] + c[1]) * F.dim[0] + c[0]);
+        G.cell_of_point[i] = cell;
+        if (w[0] != 0 || w[1] != 0 || w[2] != 0)
+            atomicOr(&G.frames[tile_frame[t]].any_outside, 1);
+    }
+    wave_aggregated_increment(G.fill, cell, valid);
 }
 
 __global__ __launch_bounds__(256) void bgrid_scatter_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
@@ -70,11 +114,13 @@ __global__ __launch_bounds__(256) void bgrid_scatter_kernel(fa::BatchGrid G, con
     uint32_t const t = blockIdx.x;
     const fa::BatchFrame& F = G.frames[tile_frame[t]];
     uint32_t const local = (t - F.tile_base) * fa::kTile + threadIdx.x;
-    if (local >= F.n)
-        return;
+    bool const valid = local < F.n;
     uint32_t const i = F.pt_base + local;
-    uint32_t const cell = G.cell_of_point[i];
-    uint32_t const slot = G.cstart[cell] + atomicAdd(&G.fill[cell], 1u);
+    uint32_t const cell = valid ? G.cell_of_point[i] : 0u;
+    uint32_t const rank = wave_aggregated_increment(G.fill, cell, valid);
+    if (!valid)
+        return;
+    uint32_t const slot = G.cstart[cell] + rank;
     float const x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
     G.cpos[slot] = make_float4(x, y, z, __uint_as_float(local));
     int c[3], w[3];
@@ -200,7 +246,7 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
     DevBox const db = make_devbox(box);
     unsigned const blocks = (n + 255) / 256;
     FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, db, d_points, n, dims[0], dims[1], dims[2],
-               cell_of_point, counts, d_any);
+               cell_of_point, nullptr, counts, d_any);
     if (int rc = exclusive_scan_u32(counts, cstart, ncell, nullptr))
         return rc;
     FA_KLAUNCH(order_scatter_kernel, dim3(blocks), dim3(256), 0, s, n, cell_of_point, cstart, fill, c.order);
@@ -290,14 +336,15 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
         for (int k = 0; k < 3; ++k)
             g.cw[k] = (k == 2 && box.is2D) ? 0.0f : npd[k] / float(dims[k]);
     }
-    // scratch: counts (ncell) + fill (ncell); the outside flag lives with the
-    // grid, read by the kernels (no host round trip)
+    // scratch: counts (ncell) + fill (ncell) + each point's packed image offset;
+    // the outside flag lives with the grid, read by the kernels (no host round trip)
     void* tmp = nullptr;
     size_t const cb = sizeof(uint32_t) * size_t(ncell);
-    if (int rc = scratch(2, 2 * cb + 256, &tmp))
+    if (int rc = scratch(2, 2 * cb + sizeof(int) * size_t(n) + 256, &tmp))
         return rc;
     uint32_t* counts = static_cast<uint32_t*>(tmp);
     uint32_t* fill = counts + ncell;
+    int* wrap_of_point = reinterpret_cast<int*>(fill + ncell);
     if (!g.outside)
         if (int rc = device_alloc((void**) &g.outside, 256))
             return rc;
@@ -308,15 +355,15 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
     if (n)
     {
         FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, g.dbox, d_points, n, dims[0], dims[1],
-                           dims[2], g.cell_of_point, counts, d_any);
+                           dims[2], g.cell_of_point, wrap_of_point, counts, d_any);
         FA_HIP(hipGetLastError());
     }
     if (int rc = exclusive_scan_u32(counts, g.cstart, ncell, nullptr))
         return rc;
     if (n)
     {
-        FA_KLAUNCH(grid_scatter_kernel, dim3(blocks), dim3(256), 0, s, g.dbox, d_points, n, dims[0], dims[1],
-                           dims[2], g.cell_of_point, g.cstart, fill, g.cpos, g.wrapc);
+        FA_KLAUNCH(grid_scatter_kernel, dim3(blocks), dim3(256), 0, s, d_points, n, g.cell_of_point, wrap_of_point,
+                   g.cstart, fill, g.cpos, g.wrapc);
         FA_HIP(hipGetLastError());
     }
     return FA_OK;
