@@ -148,8 +148,17 @@ __host__ __device__ inline void make_absolute(const DevBox& b, float fx, float f
 }
 
 // freud/util/utils.h:29-32
+// fmod(fmod(a, 1) + 1, 1).  For |a| < 1 (the fractional coordinates of any
+// vector shorter than half the box) fmod(a, 1) == a, and t = a + 1 lies in
+// [0, 2], where fmod(t, 1) == t - floor(t) exactly (Sterbenz): the same bits
+// without the two library fmods.
 __host__ __device__ inline float modulus_positive(float a)
 {
+    if (fabsf(a) < 1.0f)
+    {
+        float const t = a + 1.0f;
+        return t - floorf(t);
+    }
     return fmodf(fmodf(a, 1.0f) + 1.0f, 1.0f);
 }
 

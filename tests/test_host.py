@@ -170,3 +170,20 @@ def test_rdf_bin_table_is_exact(oracle):
         ref = oracle.rdf_bin(np.sqrt(x[ok]), bins, r_min, r_max)  # histogram of the reference bins
         npt.assert_array_equal(np.bincount(got, minlength=bins), ref)
         assert ok.mean() > 0.9
+
+
+def test_modulus_positive_shortcut_is_exact():
+    """The kernels compute the reference's modulusPositive (utils.h:29-32),
+    fmod(fmod(a, 1) + 1, 1), as t - floor(t) with t = a + 1 when |a| < 1;
+    compare both on sampled float bit patterns and the edge values."""
+    rng = np.random.default_rng(5)
+    bits = rng.integers(0, 2 ** 32, 4_000_000, dtype=np.uint64).astype(np.uint32)
+    a = bits.view(np.float32)
+    one = np.float32(1.0)
+    edge = np.array([0.0, -0.0, 1e-45, -1e-45, 2.0 ** -24, -(2.0 ** -24), np.nextafter(one, 0),
+                     -np.nextafter(one, 0), 0.5, -0.5], np.float32)
+    a = np.concatenate([a[np.abs(a) < 1], edge]).astype(np.float32)
+    ref = np.fmod(np.fmod(a, one) + one, one)
+    t = (a + one).astype(np.float32)
+    fast = (t - np.floor(t)).astype(np.float32)
+    npt.assert_array_equal(fast.view(np.uint32), ref.view(np.uint32))
