@@ -418,32 +418,69 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
     }
 }
 
-__global__ void sum_partials_kernel(const float2* __restrict__ part, uint32_t nblocks, int nm, float2* __restrict__ out)
+// Sum of the blocks' partial q_lm rows (nm <= kMaxNm values each): one block
+// of kSumThreads threads, each adding its rows (tid, tid + kSumThreads, ...)
+// into registers with the loads of four rows in flight together, then a fixed
+// reduction through wave shuffles and LDS (the same order on every run).
+constexpr int kMaxNm = 41; // 2l + 1 for l <= 20
+constexpr int kSumThreads = 1024;
+__global__ __launch_bounds__(kSumThreads) void sum_partials_kernel(const float2* __restrict__ part, uint32_t nblocks,
+                                                                   int nm, float2* __restrict__ out)
 {
-    __shared__ float2 s_red[kBlock];
-    for (int t = 0; t < nm; ++t)
+    __shared__ float2 s_red[kSumThreads / 64][kMaxNm];
+    uint32_t const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (int t0 = 0; t0 < nm; t0 += 8)
     {
-        float2 acc = make_float2(0.0f, 0.0f);
-        for (uint32_t bl = threadIdx.x; bl < nblocks; bl += blockDim.x)
+        // columns t0 .. t0 + 7 of every row
+        float2 acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            acc[c] = make_float2(0.0f, 0.0f);
+        for (uint32_t bl = threadIdx.x; bl < nblocks; bl += 4 * kSumThreads)
         {
-            float2 const v = part[uint64_t(bl) * nm + t];
-            acc.x += v.x;
-            acc.y += v.y;
-        }
-        s_red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1)
-        {
-            if (threadIdx.x < s)
+            float2 v[4][8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
             {
-                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
-                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
+                uint32_t const row = bl + uint32_t(r) * kSumThreads;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                    v[r][c] = (row < nblocks && t0 + c < nm) ? part[uint64_t(row) * nm + t0 + c]
+                                                             : make_float2(0.0f, 0.0f);
             }
-            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                {
+                    acc[c].x += v[r][c].x;
+                    acc[c].y += v[r][c].y;
+                }
         }
-        if (threadIdx.x == 0)
-            out[t] = s_red[0];
-        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            float x = acc[c].x, y = acc[c].y;
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1)
+            {
+                x += __shfl_xor(x, d);
+                y += __shfl_xor(y, d);
+            }
+            if (lane == 0 && t0 + c < nm)
+                s_red[wave][t0 + c] = make_float2(x, y);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(nm))
+    {
+        float2 r = make_float2(0.0f, 0.0f);
+        for (uint32_t w = 0; w < kSumThreads / 64; ++w)
+        {
+            r.x += s_red[w][threadIdx.x].x;
+            r.y += s_red[w][threadIdx.x].y;
+        }
+        out[threadIdx.x] = r;
     }
 }
 
@@ -1164,7 +1201,7 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         std::vector<std::complex<float>> hsys(nm, 0.0f);
         if (n)
         {
-            FA_KLAUNCH(sum_partials_kernel, dim3(1), dim3(kBlock), 0, s, average ? part_ave : part, blocks, nm,
+            FA_KLAUNCH(sum_partials_kernel, dim3(1), dim3(kSumThreads), 0, s, average ? part_ave : part, blocks, nm,
                                sys);
             FA_HIP(hipGetLastError());
             FA_HIP(hipMemcpyAsync(hsys.data(), sys, sizeof(float2) * nm, hipMemcpyDeviceToHost, s));

@@ -240,6 +240,21 @@ def test_steinhardt(oracle, name, box, pts):
             npt.assert_allclose(a, b, atol=1e-5)
 
 
+def test_steinhardt_low_and_high_l(oracle):
+    """l = 0..3 and 14..20 (the system order reduces up to 2l + 1 = 41 values
+    per particle), plain and averaged, against the oracle and its FP64 run."""
+    name, box, pts = SYSTEMS[0]
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.6, exclude_ii=True)).toNeighborList()
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.6, exclude_ii=True), nthreads=8)
+    ls = [0, 1, 2, 3, 14, 16, 20]
+    for kw in (dict(), dict(average=True)):
+        st = freud_amd.order.Steinhardt(ls, **kw).compute((box, pts), neighbors=nl)
+        ref64 = oracle.steinhardt(_bt(box), pts, ref_nl, ls, nthreads=8, f64=True, **kw)
+        ok = np.isfinite(ref64["particle_order"])
+        npt.assert_allclose(st.particle_order[ok], ref64["particle_order"][ok], atol=1e-5, rtol=0)
+        npt.assert_allclose(st.order, ref64["order"], atol=1e-5)
+
+
 def test_steinhardt_gc_golden():
     # tests/validation/test_steinhardt_average.py:82-109 of the reference
     fr = open_gsd(os.path.join(GOLDEN, "steinhardt_average", "Test_Configuration.gsd"))[-1]
