@@ -313,7 +313,7 @@ template<int KMAX> struct TopK
 template<int KMAX, bool SELF>
 __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
                                                           uint32_t nq, uint32_t k, float rlim_sq, float rlim,
-                                                          float r_min, int exclude_ii, float wmin,
+                                                          float rmin_thr, int exclude_ii, float wmin,
                                                           float half_min_plane, int rmax_ring,
                                                           uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
                                                           uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag)
@@ -335,8 +335,9 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
                                                      return;
                                                  if (exclude_ii && j == i)
                                                      return;
-                                                 // the distance test only matters for r_min > 0
-                                                 if (r_min > 0.0f && !(sqrtf(r_sq) >= r_min))
+                                                 // sqrtf(r_sq) >= r_min, as one compare (rmin_thr
+                                                 // is the smallest r_sq that passes it)
+                                                 if (!(r_sq >= rmin_thr))
                                                      return;
                                                  ++found;
                                                  best.insert((uint64_t(__float_as_uint(r_sq)) << 32)
@@ -828,21 +829,42 @@ int validate_query_args(const fa_nq* nq, fa_query_args& a, uint32_t n_query)
 
 namespace {
 
+// The smallest float x >= 0 with sqrtf(x) >= r (correctly rounded square
+// root, monotone in x), by bisection over float bit patterns; 0 for r <= 0.
+float sqrt_threshold(float r)
+{
+    if (!(r > 0.0f))
+        return 0.0f;
+    uint32_t lo = 0, hi = 0x7f800000u;
+    while (lo < hi)
+    {
+        uint32_t const mid = lo + (hi - lo) / 2;
+        volatile float const x = __builtin_bit_cast(float, mid);
+        volatile float const q = std::sqrt(float(x));
+        if (float(q) >= r)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return __builtin_bit_cast(float, lo);
+}
+
 template<int KMAX>
 int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, uint32_t nq, uint32_t k, float rlim,
                     float r_min, int exclude_ii, float wmin, float hmp, int rmax_ring, uint32_t* counts,
                     uint64_t* sel, uint32_t* flagged, uint32_t* nflag, const float* self_points, uint32_t self_n)
 {
     float const rlim_sq = rlim * rlim;
+    float const rmin_thr = sqrt_threshold(r_min);
     bool const self = (qpts == self_points) && (nq == self_n);
     FA_PROF_BEGIN("knn_aabb_kernel");
     if (self)
         hipLaunchKernelGGL((knn_aabb_kernel<KMAX, true>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, r_min, exclude_ii, wmin, hmp, rmax_ring,
+                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring,
                            counts, sel, flagged, nflag);
     else
         hipLaunchKernelGGL((knn_aabb_kernel<KMAX, false>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, r_min, exclude_ii, wmin, hmp, rmax_ring,
+                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring,
                            counts, sel, flagged, nflag);
     FA_PROF_END();
     FA_HIP(hipGetLastError());

@@ -105,6 +105,19 @@ def test_knn(oracle, name, box, pts, k):
     _assert_nlist_equal(nl, ref)
 
 
+@pytest.mark.parametrize("name,box,pts", SYSTEMS[:2], ids=[s[0] for s in SYSTEMS[:2]])
+def test_knn_r_min(oracle, name, box, pts):
+    """kNN with r_min > 0, including r_min equal to an existing bond length
+    (a candidate exactly at r_min is kept: sqrt(r_sq) >= r_min)."""
+    d = freud_amd.AABBQuery(box, pts).query(pts, dict(num_neighbors=4, exclude_ii=True)).toNeighborList().distances
+    for r_min in (0.35, float(np.sort(d)[len(d) // 3])):
+        qa = dict(num_neighbors=8, r_min=r_min, exclude_ii=True)
+        nl = freud_amd.AABBQuery(box, pts).query(pts, qa).toNeighborList()
+        ref = oracle.query(_bt(box), pts, pts, qa, nthreads=8)
+        _assert_nlist_equal(nl, ref)
+        assert np.all(nl.distances >= np.float32(r_min))
+
+
 def test_knn_linkcell(oracle):
     name, box, pts = SYSTEMS[1]
     nq = freud_amd.LinkCell(box, pts, cell_width=2.5)
