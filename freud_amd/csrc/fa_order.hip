@@ -153,16 +153,17 @@ struct BondSrc
 };
 
 // Block-uniform: stages the bonds of points [p0, p0 + nloc) (offsets-relative)
-// when they fit.  Every thread of the block must call it.
+// when they fit in cap bonds (cap == 0: never).  Every thread of the block must
+// call it.
 __device__ inline BondSrc stage_bonds(uint32_t p0, uint32_t nloc, const uint32_t* __restrict__ offsets,
                                       const uint32_t* __restrict__ nbr, const float* __restrict__ dist,
-                                      uint32_t* s_j, float* s_d, bool allow)
+                                      uint32_t* s_j, float* s_d, uint32_t cap)
 {
     BondSrc src{nbr, dist, s_j, s_d, 0u, false};
-    if (!allow || nloc == 0)
+    if (cap == 0 || nloc == 0)
         return src;
     uint32_t const b0 = offsets[p0], b1 = offsets[p0 + nloc];
-    if (b1 - b0 > kStage)
+    if (b1 - b0 > cap)
         return src;
     for (uint32_t t = threadIdx.x; t < b1 - b0; t += blockDim.x)
     {
@@ -174,6 +175,12 @@ __device__ inline BondSrc stage_bonds(uint32_t p0, uint32_t nloc, const uint32_t
     src.staged = true;
     return src;
 }
+
+// Dynamic LDS of the kernels over one NeighborList that stage bonds: cap
+// neighbour indices followed by cap distances, cap sized by the host from the
+// mean bond count (stage_capacity); 0 bytes when the points are visited in a
+// processing order, where a block's bonds are not one range.
+extern __shared__ uint32_t fa_stage_lds[];
 
 template<int L>
 __device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i, const BondSrc& src,
@@ -218,27 +225,34 @@ __device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__
 }
 
 // Block partial sum of q_lm(i) * invn over the block's valid points (system
-// q_lm, Steinhardt.cc:213-216), written to out[0..NM).
+// q_lm, Steinhardt.cc:213-216), written to out[0..NM).  Each wave folds its
+// 64 lanes with butterfly shuffles, one barrier, then 2 NM threads add the
+// block's four wave sums.  Every loop over q is unrolled so that q stays in
+// registers.  Every thread of the (kBlock-thread) block must call it.
 template<int L> __device__ inline void block_qlm_partial(bool valid, const float* q, float invn, float2* out)
 {
     constexpr int NM = 2 * L + 1;
-    __shared__ float2 s_red[kBlock];
-    for (int t = 0; t < NM; ++t)
+    constexpr int kWaves = kBlock / 64;
+    __shared__ float s_w[kWaves][2 * NM];
+    uint32_t const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < 2 * NM; ++t)
     {
-        s_red[threadIdx.x] = valid ? make_float2(q[2 * t] * invn, q[2 * t + 1] * invn) : make_float2(0.0f, 0.0f);
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1)
-        {
-            if (threadIdx.x < s)
-            {
-                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
-                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0)
-            out[t] = s_red[0];
-        __syncthreads();
+        float v = valid ? q[t] * invn : 0.0f;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1)
+            v += __shfl_xor(v, d);
+        if (lane == 0)
+            s_w[wave][t] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * NM)
+    {
+        float v = s_w[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w)
+            v += s_w[w][threadIdx.x];
+        reinterpret_cast<float*>(out)[threadIdx.x] = v;
     }
 }
 
@@ -249,16 +263,15 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
                                                      int weighted, int want_sys, float2* __restrict__ qlm,
                                                      float* __restrict__ ql, int ql_stride, int ql_col,
                                                      float2* __restrict__ sys_partial,
-                                                     const uint32_t* __restrict__ order)
+                                                     const uint32_t* __restrict__ order, uint32_t stage_cap)
 {
     constexpr int NM = 2 * L + 1;
-    __shared__ uint32_t s_j[kStage];
-    __shared__ float s_d[kStage];
     uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
     uint32_t const t0 = tile0 + threadIdx.x;
     bool const valid = t0 < n;
     uint32_t const i = (order && valid) ? order[t0] : t0;
-    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, s_j, s_d, order == nullptr);
+    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, fa_stage_lds,
+                                    reinterpret_cast<float*>(fa_stage_lds + stage_cap), stage_cap);
     float q[2 * NM];
     if (valid)
     {
@@ -271,6 +284,141 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
     }
     if (want_sys)
         block_qlm_partial<L>(valid, q, 1.0f / float(n), sys_partial + uint64_t(blockIdx.x) * NM);
+}
+
+// Several even l <= LMAX in one bond pass (Steinhardt with a list of l):
+// the bond geometry is computed once and one Legendre sweep up to LMAX feeds
+// every even l; q_lm, q_l and the block partial sums are then written for the
+// l in `mask`.  Accumulators: (l/2)^2 + ... for the even l (49 complex for
+// LMAX = 12).
+constexpr int kFuseLMax = 12;
+struct MultiLOut
+{
+    float2* qlm[kLMax + 1];  // (n, 2l + 1) per requested l
+    float2* part[kLMax + 1]; // block partial sums per requested l (nullptr: none)
+    int col[kLMax + 1];      // column of l in ql (n, ql_stride)
+};
+
+__host__ __device__ constexpr int even_base(int l)
+{
+    return (l / 2) * (l / 2);
+}
+
+template<int L, int LMAX>
+__device__ inline void emit_even_l(const float2* acc, float total_w, uint32_t mask, bool valid, uint32_t i, float invn,
+                                   const MultiLOut& o, float* __restrict__ ql, int ql_stride)
+{
+    if constexpr (L <= LMAX)
+    {
+        if ((mask >> L) & 1u)
+        {
+            constexpr int NM = 2 * L + 1;
+            float q[2 * NM];
+            float norm = 0.0f;
+#pragma unroll
+            for (int m = 0; m <= L; ++m)
+            {
+                float const re = acc[even_base(L) + m].x / total_w, im = acc[even_base(L) + m].y / total_w;
+                q[2 * m] = re;
+                q[2 * m + 1] = im;
+                if (m > 0)
+                {
+                    float const sgn = (m & 1) ? -1.0f : 1.0f;
+                    q[2 * (L + m)] = sgn * re;
+                    q[2 * (L + m) + 1] = -sgn * im;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NM; ++t)
+                norm += q[2 * t] * q[2 * t] + q[2 * t + 1] * q[2 * t + 1];
+            if (valid)
+            {
+                float2* out = o.qlm[L] + uint64_t(i) * NM;
+#pragma unroll
+                for (int t = 0; t < NM; ++t)
+                    out[t] = make_float2(q[2 * t], q[2 * t + 1]);
+                ql[uint64_t(i) * ql_stride + o.col[L]] = sqrtf(norm * float(4.0 * M_PI / NM));
+            }
+            if (o.part[L])
+                block_qlm_partial<L>(valid, q, invn, o.part[L] + uint64_t(blockIdx.x) * NM);
+        }
+        emit_even_l<L + 2, LMAX>(acc, total_w, mask, valid, i, invn, o, ql, ql_stride);
+    }
+}
+
+template<int LMAX>
+__global__ __launch_bounds__(kBlock) void qlm_even_multi_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
+                                                                const uint32_t* __restrict__ nbr,
+                                                                const float* __restrict__ dist,
+                                                                const float* __restrict__ wts,
+                                                                const uint32_t* __restrict__ offsets, int weighted,
+                                                                uint32_t mask, MultiLOut o, float* __restrict__ ql,
+                                                                int ql_stride, const uint32_t* __restrict__ order,
+                                                                uint32_t stage_cap)
+{
+    uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    uint32_t const t0 = tile0 + threadIdx.x;
+    bool const valid = t0 < n;
+    uint32_t const i = (order && valid) ? order[t0] : t0;
+    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, fa_stage_lds,
+                                    reinterpret_cast<float*>(fa_stage_lds + stage_cap), stage_cap);
+    float2 acc[even_base(LMAX + 2)];
+#pragma unroll
+    for (int t = 0; t < even_base(LMAX + 2); ++t)
+        acc[t] = make_float2(0.0f, 0.0f);
+    float total_w = 0.0f;
+    if (valid)
+    {
+        uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+        for (uint32_t e = e0; e < e1; ++e)
+        {
+            float const w = weighted ? wts[e] : 1.0f;
+            float ct, st, cp, sp;
+            bond_angles(b, pts, i, src.j(e), src.d(e), ct, st, cp, sp);
+            float pmm = 0.28209479177387814f;
+            float er = 1.0f, ei = 0.0f;
+#pragma unroll
+            for (int m = 0; m <= LMAX; ++m)
+            {
+                if (m > 0)
+                {
+                    pmm = c_ylm.amm[m] * st * pmm;
+                    float const nr = er * cp - ei * sp;
+                    float const ni = er * sp + ei * cp;
+                    er = nr;
+                    ei = ni;
+                }
+                // the same recurrence and operation order as accumulate_ylm<L>
+                float p0 = pmm, p1 = 0.0f;
+#pragma unroll
+                for (int l = m; l <= LMAX; ++l)
+                {
+                    float p;
+                    if (l == m)
+                        p = pmm;
+                    else if (l == m + 1)
+                    {
+                        p1 = c_ylm.bm1[m] * ct * pmm;
+                        p = p1;
+                    }
+                    else
+                    {
+                        p = c_ylm.a[l][m] * (ct * p1 - c_ylm.b[l][m] * p0);
+                        p0 = p1;
+                        p1 = p;
+                    }
+                    if ((l & 1) == 0)
+                    {
+                        float const wp = w * p;
+                        acc[even_base(l) + m].x += wp * er;
+                        acc[even_base(l) + m].y += wp * ei;
+                    }
+                }
+            }
+            total_w += w;
+        }
+    }
+    emit_even_l<0, LMAX>(acc, total_w, mask, valid, i, 1.0f / float(n), o, ql, ql_stride);
 }
 
 // Batched q_l: one workgroup per 256-point tile of one frame; bonds hold
@@ -296,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void bqlm_kernel(fa::BatchGrid G, const uin
     uint32_t const i0 = (t - F.tile_base) * fa::kTile;
     uint32_t const i = i0 + threadIdx.x;
     bool const valid = i < F.n;
-    BondSrc const src = stage_bonds(i0, min(fa::kTile, F.n - i0), offsets + F.pt_base, nbr, dist, s_j, s_d, true);
+    BondSrc const src = stage_bonds(i0, min(fa::kTile, F.n - i0), offsets + F.pt_base, nbr, dist, s_j, s_d, kStage);
     float q[2 * NM];
     if (valid)
     {
@@ -647,14 +795,13 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
                                                              const float* __restrict__ coef10,
                                                              const float* __restrict__ coef12, float* __restrict__ feat,
                                                              float2* __restrict__ qlm6, float* __restrict__ ql6,
-                                                             const uint32_t* __restrict__ order)
+                                                             const uint32_t* __restrict__ order, uint32_t stage_cap)
 {
     constexpr int LM = 12;
-    __shared__ uint32_t s_j[kStage];
-    __shared__ float s_d[kStage];
     uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
     uint32_t const t0 = tile0 + threadIdx.x;
-    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, s_j, s_d, order == nullptr);
+    BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, fa_stage_lds,
+                                    reinterpret_cast<float*>(fa_stage_lds + stage_cap), stage_cap);
     if (t0 >= n)
         return;
     uint32_t const i = order ? order[t0] : t0;
@@ -1069,13 +1216,26 @@ float reduce_wigner3j_host(const std::complex<float>* src, int l)
     return result;
 }
 
+// Bond staging capacity of a kBlock-point tile: 1.3 x the mean bonds per point,
+// rounded to 64, within [512, kStage]; 0 when the points are visited in a
+// processing order (a tile's bonds are then not one range).
+uint32_t stage_capacity(const fa_nlist* nl, uint32_t n, const uint32_t* order)
+{
+    if (order || n == 0)
+        return 0;
+    double const want = std::ceil(double(nl->num_bonds) / double(n) * kBlock * 1.3);
+    uint32_t const cap = uint32_t(std::min(std::max(want, 512.0), double(kStage)));
+    return (cap + 63u) & ~63u;
+}
+
 template<int L>
 void launch_qlm_l(const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted, int want_sys,
                   float2* qlm, float* ql, int stride, int col, float2* part, const uint32_t* order)
 {
     unsigned const blocks = (n + kBlock - 1) / kBlock;
-    FA_KLAUNCH(qlm_kernel<L>, dim3(blocks), dim3(kBlock), 0, fa::stream(), db, pts, n, nl->neighbors,
-               nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part, order);
+    uint32_t const cap = stage_capacity(nl, n, order);
+    FA_KLAUNCH(qlm_kernel<L>, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, pts, n, nl->neighbors,
+               nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part, order, cap);
 }
 
 template<int L>
@@ -1099,6 +1259,33 @@ int dispatch_l(int l, const fa::DevBox& db, const float* pts, uint32_t n, const 
     if constexpr (L < kLMax)
         return dispatch_l<L + 1>(l, db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part, order);
     return 0;
+}
+
+template<int LMAX>
+int launch_qlm_even_multi(int lmax, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl,
+                          int weighted, uint32_t mask, const MultiLOut& o, float* ql, int stride,
+                          const uint32_t* order)
+{
+    if (lmax == LMAX)
+    {
+        unsigned const blocks = (n + kBlock - 1) / kBlock;
+        uint32_t const cap = stage_capacity(nl, n, order);
+        FA_KLAUNCH(qlm_even_multi_kernel<LMAX>, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, pts,
+                   n, nl->neighbors, nl->distances, nl->weights, nl->offsets, weighted, mask, o, ql, stride, order,
+                   cap);
+        FA_HIP(hipGetLastError());
+        return FA_OK;
+    }
+    if constexpr (LMAX + 2 <= kFuseLMax)
+        return launch_qlm_even_multi<LMAX + 2>(lmax, db, pts, n, nl, weighted, mask, o, ql, stride, order);
+    return fa::fail(FA_ERR_INVALID, "fused q_lm pass: l out of range");
+}
+
+int launch_qlm_even_multi(int lmax, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl,
+                          int weighted, uint32_t mask, const MultiLOut& o, float* ql, int stride,
+                          const uint32_t* order)
+{
+    return launch_qlm_even_multi<2>(lmax, db, pts, n, nl, weighted, mask, o, ql, stride, order);
 }
 
 template<int L>
@@ -1135,23 +1322,56 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
     const uint32_t* sorder = nullptr;
     if (int rc = spatial_order(*box, d_points, n, &sorder))
         return rc;
-    size_t moff = 0;
+    // Partial sums: blocks x (2l + 1) complex per l for the base q_lm (all l
+    // live at once, for the fused pass), one such slab for the averaged q_lm,
+    // and 2l + 1 complex for the result.
+    uint32_t even_mask = 0;
+    int lmax = 0;
+    bool fuse = n_l >= 2;
+    size_t part_total = 0, nm_max = 1;
     for (int k = 0; k < n_l; ++k)
     {
         int const l = int(ls[k]);
         if (l > kLMax)
             return fail(FA_ERR_INVALID, "freud_amd supports l <= 20");
-        int const nm = 2 * l + 1;
-        // partial sums: blocks x nm complex for base and average, + nm result
-        void* tmp = nullptr;
-        size_t const pb = sizeof(float2) * size_t(blocks) * nm;
-        if (int rc = scratch(6, 2 * pb + sizeof(float2) * nm + 256, &tmp))
+        fuse = fuse && (l % 2 == 0) && l <= kFuseLMax && !((even_mask >> l) & 1u);
+        even_mask |= 1u << l;
+        lmax = std::max(lmax, l);
+        part_total += size_t(blocks) * (2 * l + 1);
+        nm_max = std::max<size_t>(nm_max, 2 * l + 1);
+    }
+    void* tmp = nullptr;
+    size_t const pb_max = sizeof(float2) * size_t(blocks) * nm_max;
+    if (int rc = scratch(6, sizeof(float2) * part_total + pb_max + sizeof(float2) * nm_max + 256, &tmp))
+        return rc;
+    float2* const part_base = static_cast<float2*>(tmp);
+    float2* const part_ave = part_base + part_total;
+    float2* const sys = reinterpret_cast<float2*>(reinterpret_cast<char*>(part_ave) + pb_max);
+    if (fuse && n)
+    {
+        MultiLOut o{};
+        size_t mo = 0, po = 0;
+        for (int k = 0; k < n_l; ++k)
+        {
+            int const l = int(ls[k]);
+            o.qlm[l] = reinterpret_cast<float2*>(d_qlm) + size_t(n) * mo;
+            o.part[l] = average ? nullptr : part_base + po;
+            o.col[l] = k;
+            mo += 2 * l + 1;
+            po += size_t(blocks) * (2 * l + 1);
+        }
+        if (int rc = launch_qlm_even_multi(lmax, db, d_points, n, nl, weighted, even_mask, o, d_ql, n_l, sorder))
             return rc;
-        float2* part = static_cast<float2*>(tmp);
-        float2* part_ave = reinterpret_cast<float2*>(static_cast<char*>(tmp) + pb);
-        float2* sys = reinterpret_cast<float2*>(static_cast<char*>(tmp) + 2 * pb);
+    }
+    size_t moff = 0, poff = 0;
+    for (int k = 0; k < n_l; ++k)
+    {
+        int const l = int(ls[k]);
+        int const nm = 2 * l + 1;
+        float2* const part = part_base + poff;
+        poff += size_t(blocks) * nm;
         float2* qlm_l = reinterpret_cast<float2*>(d_qlm) + size_t(n) * moff;
-        if (n)
+        if (n && !fuse)
         {
             dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part, sorder);
             FA_HIP(hipGetLastError());
@@ -1348,9 +1568,10 @@ int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const f
     const uint32_t* sorder = nullptr;
     if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
         return rc;
-    FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), 0, fa::stream(), db, d_points, n, nl->neighbors,
-               nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
-               reinterpret_cast<float2*>(d_qlm6), d_ql6, sorder);
+    uint32_t const cap = stage_capacity(nl, n, sorder);
+    FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, d_points, n,
+               nl->neighbors, nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
+               reinterpret_cast<float2*>(d_qlm6), d_ql6, sorder, cap);
     FA_HIP(hipGetLastError());
     return FA_OK;
 }

@@ -268,6 +268,32 @@ def test_steinhardt_low_and_high_l(oracle):
         npt.assert_allclose(st.order, ref64["order"], atol=1e-5)
 
 
+def test_steinhardt_weighted_and_fused_l(oracle):
+    """Weighted bonds (random weights through NeighborList.from_arrays) against
+    the oracle.  A list of distinct even l <= 12 is evaluated in one fused bond
+    pass; lists with an odd or repeated l go l by l.  The fused pass performs
+    the same float operations as the single-l pass, so each of its columns
+    equals the single-l result bit for bit."""
+    name, box, pts = SYSTEMS[1]
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.6, exclude_ii=True), nthreads=8)
+    w = np.random.default_rng(5).uniform(0.2, 2.0, len(ref_nl["ij"])).astype(np.float32)
+    nl = freud_amd.NeighborList.from_arrays(len(pts), len(pts), ref_nl["ij"][:, 0], ref_nl["ij"][:, 1],
+                                            ref_nl["vectors"], w)
+    ref_w = dict(ref_nl, weights=w)
+    for ls in ([4, 6, 8, 10, 12], [12, 2, 6], [6, 6, 4], [4, 5, 6]):
+        for kw in (dict(weighted=True), dict(weighted=True, average=True), dict(weighted=True, wl=True)):
+            st = freud_amd.order.Steinhardt(ls, **kw).compute((box, pts), neighbors=nl)
+            ref = oracle.steinhardt(_bt(box), pts, ref_w, ls, nthreads=8, **kw)
+            ok = np.isfinite(ref["particle_order"])
+            npt.assert_allclose(st.particle_order[ok], ref["particle_order"][ok], atol=1e-5, rtol=0)
+            npt.assert_allclose(st.order, ref["order"], atol=1e-5)
+            for k, l in enumerate(ls):
+                one = freud_amd.order.Steinhardt(l, **kw).compute((box, pts), neighbors=nl)
+                npt.assert_array_equal(st.particle_order[:, k], one.particle_order)
+                npt.assert_array_equal(st.particle_harmonics[k], one.particle_harmonics)
+                assert st.order[k] == one.order
+
+
 def test_steinhardt_gc_golden():
     # tests/validation/test_steinhardt_average.py:82-109 of the reference
     fr = open_gsd(os.path.join(GOLDEN, "steinhardt_average", "Test_Configuration.gsd"))[-1]
@@ -487,6 +513,11 @@ def test_spatially_ordered_paths(oracle):
         r2 = oracle.steinhardt(_bt(box), pts, ref_nl, [6], nthreads=16, **kw)
         npt.assert_allclose(s2.particle_order, np.ravel(r2["particle_order"]), atol=1e-5)
         npt.assert_allclose(s2.order, np.ravel(r2["order"])[0], atol=1e-5)
+    # several even l: the fused bond pass in the spatial order
+    s3 = freud_amd.order.Steinhardt([4, 6, 8, 12]).compute((box, pts), neighbors=nl)
+    r3 = oracle.steinhardt(_bt(box), pts, ref_nl, [4, 6, 8, 12], nthreads=16)
+    npt.assert_allclose(s3.particle_order, r3["particle_order"], atol=1e-5)
+    npt.assert_allclose(s3.order, r3["order"], atol=1e-5)
 
 
 @pytest.mark.parametrize("name,box,pts", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
