@@ -436,7 +436,7 @@ def run_stress(args):
     avg_ms = tot / cnt
     # dominant kernel: the fused q_lm pass against the FP32 vector peak, any
     # other against HBM with its compulsory bytes
-    if dom.startswith("qlm_even_multi"):
+    if "qlm_even_multi" in dom:
         roof = qlm_roofline(dom, avg_ms, bt, nt, [int(v) for v in ls])
     else:
         ab = algorithmic_bytes(dom, nt, bt, 0) or 0

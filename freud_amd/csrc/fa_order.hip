@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <vector>
 
 #include "fa_common.h"
@@ -34,40 +35,65 @@ struct YlmCoef
     float a[kLMax + 1][kLMax + 1];        // a(l, m)
     float b[kLMax + 1][kLMax + 1];        // b(l, m)
 };
-__constant__ YlmCoef c_ylm;
-bool g_ylm_ready = false;
-
-int ensure_ylm_coef()
+// Square root by Newton's iteration from above, for the compile-time table
+// (within an ulp of the double result; the host check below confirms that
+// every coefficient rounds to the same float as std::sqrt).
+constexpr double ce_sqrt(double x)
 {
-    if (g_ylm_ready)
-        return FA_OK;
+    if (!(x > 0.0))
+        return 0.0;
+    double g = x > 1.0 ? x : 1.0;
+    for (int it = 0; it < 100; ++it)
+    {
+        double const n = 0.5 * (g + x / g);
+        if (!(n < g))
+            break;
+        g = n;
+    }
+    return g;
+}
+
+template<class Sqrt> constexpr YlmCoef make_ylm_coef(Sqrt sq)
+{
     YlmCoef h{};
     for (int m = 0; m <= kLMax; ++m)
     {
-        h.amm[m] = m ? float(-std::sqrt((2.0 * m + 1.0) / (2.0 * m))) : 0.0f;
-        h.bm1[m] = float(std::sqrt(2.0 * m + 3.0));
-        for (int l = 0; l <= kLMax; ++l)
+        h.amm[m] = m ? float(-sq((2.0 * m + 1.0) / (2.0 * m))) : 0.0f;
+        h.bm1[m] = float(sq(2.0 * m + 3.0));
+        for (int l = m + 2; l <= kLMax; ++l)
         {
-            if (l >= m + 2)
-            {
-                h.a[l][m] = float(std::sqrt((4.0 * l * l - 1.0) / (double(l) * l - double(m) * m)));
-                h.b[l][m] = float(std::sqrt(((l - 1.0) * (l - 1.0) - double(m) * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0)));
-            }
+            h.a[l][m] = float(sq((4.0 * l * l - 1.0) / (double(l) * l - double(m) * m)));
+            h.b[l][m] = float(sq(((l - 1.0) * (l - 1.0) - double(m) * m) / (4.0 * (l - 1.0) * (l - 1.0) - 1.0)));
         }
     }
-    FA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ylm), &h, sizeof(h)));
-    g_ylm_ready = true;
+    return h;
+}
+
+// The coefficients are compile-time constants: unrolled recurrences carry
+// them as instruction literals instead of holding ~400 of them in registers.
+constexpr YlmCoef kYlm = make_ylm_coef([](double x) { return ce_sqrt(x); });
+bool g_ylm_checked = false;
+
+int ensure_ylm_coef()
+{
+    if (g_ylm_checked)
+        return FA_OK;
+    YlmCoef const h = make_ylm_coef([](double x) { return std::sqrt(x); });
+    if (std::memcmp(&h, &kYlm, sizeof(h)) != 0)
+        return fa::fail(FA_ERR_RUNTIME, "Y_lm recurrence coefficients differ from the correctly rounded values");
+    g_ylm_checked = true;
     return FA_OK;
 }
 
 // Bond direction in the reference's terms: delta = box.wrap(p_j - p_i)
 // (Steinhardt.cc:155), cos(theta) = clamp(delta.z / d, -1, 1) with theta = 0
 // for d == 0 (Steinhardt.cc:167-180), e^{i phi} with phi = atan2(delta.y, delta.x).
-__device__ inline void bond_angles(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i, uint32_t j,
-                                   float d, float& ct, float& st, float& cp, float& sp)
+// (vx, vy, vz) = p_j - p_i before wrapping.
+__device__ inline void bond_angles(const fa::DevBox& b, float vx, float vy, float vz, float d, float& ct, float& st,
+                                   float& cp, float& sp)
 {
     float dx, dy, dz;
-    fa::wrap(b, pts[3 * j] - pts[3 * i], pts[3 * j + 1] - pts[3 * i + 1], pts[3 * j + 2] - pts[3 * i + 2], dx, dy, dz);
+    fa::wrap(b, vx, vy, vz, dx, dy, dz);
     if (d == 0.0f)
     {
         ct = 1.0f;
@@ -102,7 +128,7 @@ template<int L> __device__ inline void accumulate_ylm(float ct, float st, float 
     {
         if (m > 0)
         {
-            pmm = c_ylm.amm[m] * st * pmm;
+            pmm = kYlm.amm[m] * st * pmm;
             float const nr = er * cp - ei * sp;
             float const ni = er * sp + ei * cp;
             er = nr;
@@ -114,11 +140,11 @@ template<int L> __device__ inline void accumulate_ylm(float ct, float st, float 
         else
         {
             float p0 = pmm;
-            float p1 = c_ylm.bm1[m] * ct * pmm;
+            float p1 = kYlm.bm1[m] * ct * pmm;
 #pragma unroll
             for (int l = m + 2; l <= L; ++l)
             {
-                float const p2 = c_ylm.a[l][m] * (ct * p1 - c_ylm.b[l][m] * p0);
+                float const p2 = kYlm.a[l][m] * (ct * p1 - kYlm.b[l][m] * p0);
                 p0 = p1;
                 p1 = p2;
             }
@@ -126,7 +152,52 @@ template<int L> __device__ inline void accumulate_ylm(float ct, float st, float 
         }
         float const wp = w * plm;
         acc[m].x += wp * er;
-        acc[m].y += wp * ei;
+        if (m > 0) // e^{i 0 phi} = 1: the m = 0 imaginary sum stays +0
+            acc[m].y += wp * ei;
+    }
+}
+
+// One associated-Legendre sweep up to LMAX for one bond, with the recurrence
+// and operation order of accumulate_ylm; calls sink(l, m, P_lm, cos m phi,
+// sin m phi) for every even l >= LMIN and 0 <= m <= l (l and m are constants
+// once the loops are unrolled).
+template<int LMAX, int LMIN, class Sink>
+__device__ inline void even_ylm_sweep(float ct, float st, float cp, float sp, Sink&& sink)
+{
+    float pmm = 0.28209479177387814f;
+    float er = 1.0f, ei = 0.0f;
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m)
+    {
+        if (m > 0)
+        {
+            pmm = kYlm.amm[m] * st * pmm;
+            float const nr = er * cp - ei * sp;
+            float const ni = er * sp + ei * cp;
+            er = nr;
+            ei = ni;
+        }
+        float p0 = pmm, p1 = 0.0f;
+#pragma unroll
+        for (int l = m; l <= LMAX; ++l)
+        {
+            float p;
+            if (l == m)
+                p = pmm;
+            else if (l == m + 1)
+            {
+                p1 = kYlm.bm1[m] * ct * pmm;
+                p = p1;
+            }
+            else
+            {
+                p = kYlm.a[l][m] * (ct * p1 - kYlm.b[l][m] * p0);
+                p0 = p1;
+                p1 = p;
+            }
+            if (l >= LMIN && (l & 1) == 0)
+                sink(l, m, p, er, ei);
+        }
     }
 }
 
@@ -176,6 +247,43 @@ __device__ inline BondSrc stage_bonds(uint32_t p0, uint32_t nloc, const uint32_t
     return src;
 }
 
+// Calls f(ct, st, cp, sp, w) for the bonds e0 <= e < e1 of point i, in order.
+// The neighbour position, distance and weight of bond e + 1 and the neighbour
+// index of bond e + 2 are loaded before bond e is evaluated, so that the
+// gathers of the next bond overlap the arithmetic of this one.
+template<class F>
+__device__ inline void for_each_bond(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i,
+                                     const BondSrc& src, const float* __restrict__ wts, bool weighted, uint32_t e0,
+                                     uint32_t e1, F&& f)
+{
+    if (e0 >= e1)
+        return;
+    float const xi = pts[3 * i], yi = pts[3 * i + 1], zi = pts[3 * i + 2];
+    uint32_t jn = src.j(e0);
+    float xj = pts[3 * jn], yj = pts[3 * jn + 1], zj = pts[3 * jn + 2];
+    float d = src.d(e0);
+    float w = weighted ? wts[e0] : 1.0f;
+    if (e0 + 1 < e1)
+        jn = src.j(e0 + 1);
+    for (uint32_t e = e0; e < e1; ++e)
+    {
+        float const vx = xj - xi, vy = yj - yi, vz = zj - zi, de = d, we = w;
+        if (e + 1 < e1)
+        {
+            xj = pts[3 * jn];
+            yj = pts[3 * jn + 1];
+            zj = pts[3 * jn + 2];
+            d = src.d(e + 1);
+            w = weighted ? wts[e + 1] : 1.0f;
+            if (e + 2 < e1)
+                jn = src.j(e + 2);
+        }
+        float ct, st, cp, sp;
+        bond_angles(b, vx, vy, vz, de, ct, st, cp, sp);
+        f(ct, st, cp, sp, we);
+    }
+}
+
 // Dynamic LDS of the kernels over one NeighborList that stage bonds: cap
 // neighbour indices followed by cap distances, cap sized by the host from the
 // mean bond count (stage_capacity); 0 bytes when the points are visited in a
@@ -193,16 +301,11 @@ __device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__
     for (int m = 0; m <= L; ++m)
         acc[m] = make_float2(0.0f, 0.0f);
     float total_w = 0.0f;
-    uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
-    for (uint32_t e = e0; e < e1; ++e)
-    {
-        uint32_t const j = src.j(e);
-        float const w = weighted ? wts[e] : 1.0f;
-        float ct, st, cp, sp;
-        bond_angles(b, pts, i, j, src.d(e), ct, st, cp, sp);
-        accumulate_ylm<L>(ct, st, cp, sp, w, acc);
-        total_w += w;
-    }
+    for_each_bond(b, pts, i, src, wts, weighted, offsets[i], offsets[i + 1],
+                  [&](float ct, float st, float cp, float sp, float w) {
+                      accumulate_ylm<L>(ct, st, cp, sp, w, acc);
+                      total_w += w;
+                  });
     float norm = 0.0f;
 #pragma unroll
     for (int m = 0; m <= L; ++m)
@@ -346,13 +449,12 @@ __device__ inline void emit_even_l(const float2* acc, float total_w, uint32_t ma
     }
 }
 
-template<int LMAX>
+template<int LMAX, bool WEIGHTED>
 __global__ __launch_bounds__(kBlock) void qlm_even_multi_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
                                                                 const uint32_t* __restrict__ nbr,
                                                                 const float* __restrict__ dist,
                                                                 const float* __restrict__ wts,
-                                                                const uint32_t* __restrict__ offsets, int weighted,
-                                                                uint32_t mask, MultiLOut o, float* __restrict__ ql,
+                                                                const uint32_t* __restrict__ offsets, uint32_t mask, MultiLOut o, float* __restrict__ ql,
                                                                 int ql_stride, const uint32_t* __restrict__ order,
                                                                 uint32_t stage_cap)
 {
@@ -368,56 +470,17 @@ __global__ __launch_bounds__(kBlock) void qlm_even_multi_kernel(fa::DevBox b, co
         acc[t] = make_float2(0.0f, 0.0f);
     float total_w = 0.0f;
     if (valid)
-    {
-        uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
-        for (uint32_t e = e0; e < e1; ++e)
-        {
-            float const w = weighted ? wts[e] : 1.0f;
-            float ct, st, cp, sp;
-            bond_angles(b, pts, i, src.j(e), src.d(e), ct, st, cp, sp);
-            float pmm = 0.28209479177387814f;
-            float er = 1.0f, ei = 0.0f;
-#pragma unroll
-            for (int m = 0; m <= LMAX; ++m)
-            {
-                if (m > 0)
-                {
-                    pmm = c_ylm.amm[m] * st * pmm;
-                    float const nr = er * cp - ei * sp;
-                    float const ni = er * sp + ei * cp;
-                    er = nr;
-                    ei = ni;
-                }
-                // the same recurrence and operation order as accumulate_ylm<L>
-                float p0 = pmm, p1 = 0.0f;
-#pragma unroll
-                for (int l = m; l <= LMAX; ++l)
-                {
-                    float p;
-                    if (l == m)
-                        p = pmm;
-                    else if (l == m + 1)
-                    {
-                        p1 = c_ylm.bm1[m] * ct * pmm;
-                        p = p1;
-                    }
-                    else
-                    {
-                        p = c_ylm.a[l][m] * (ct * p1 - c_ylm.b[l][m] * p0);
-                        p0 = p1;
-                        p1 = p;
-                    }
-                    if ((l & 1) == 0)
-                    {
-                        float const wp = w * p;
-                        acc[even_base(l) + m].x += wp * er;
-                        acc[even_base(l) + m].y += wp * ei;
-                    }
-                }
-            }
-            total_w += w;
-        }
-    }
+        for_each_bond(b, pts, i, src, wts, WEIGHTED, offsets[i], offsets[i + 1],
+                      [&](float ct, float st, float cp, float sp, float w) {
+                          even_ylm_sweep<LMAX, 0>(ct, st, cp, sp, [&](int l, int m, float p, float er, float ei) {
+                              // w * p == p exactly for w == 1
+                              float const wp = WEIGHTED ? w * p : p;
+                              acc[even_base(l) + m].x += wp * er;
+                              if (m > 0) // e^{i 0 phi} = 1: the m = 0 imaginary sum stays +0
+                                  acc[even_base(l) + m].y += wp * ei;
+                          });
+                          total_w += w;
+                      });
     emit_even_l<0, LMAX>(acc, total_w, mask, valid, i, 1.0f / float(n), o, ql, ql_stride);
 }
 
@@ -810,50 +873,13 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
     for (int t = 0; t < feat_base(14); ++t)
         acc[t] = make_float2(0.0f, 0.0f);
     uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
-    for (uint32_t e = e0; e < e1; ++e)
-    {
-        float ct, st, cp, sp;
-        bond_angles(b, pts, i, src.j(e), src.d(e), ct, st, cp, sp);
-        float pmm = 0.28209479177387814f;
-        float er = 1.0f, ei = 0.0f;
-#pragma unroll
-        for (int m = 0; m <= LM; ++m)
-        {
+    for_each_bond(b, pts, i, src, nullptr, false, e0, e1, [&](float ct, float st, float cp, float sp, float) {
+        even_ylm_sweep<LM, 4>(ct, st, cp, sp, [&](int l, int m, float p, float er, float ei) {
+            acc[feat_base(l) + m].x += p * er;
             if (m > 0)
-            {
-                pmm = c_ylm.amm[m] * st * pmm;
-                float const nr = er * cp - ei * sp;
-                float const ni = er * sp + ei * cp;
-                er = nr;
-                ei = ni;
-            }
-            float p0 = pmm, p1 = 0.0f;
-#pragma unroll
-            for (int l = m; l <= LM; ++l)
-            {
-                float p;
-                if (l == m)
-                    p = pmm;
-                else if (l == m + 1)
-                {
-                    p1 = c_ylm.bm1[m] * ct * pmm;
-                    p = p1;
-                }
-                else
-                {
-                    p = c_ylm.a[l][m] * (ct * p1 - c_ylm.b[l][m] * p0);
-                    p0 = p1;
-                    p1 = p;
-                }
-                if (l >= 4 && (l & 1) == 0)
-                {
-                    float const wp = 1.0f * p;
-                    acc[feat_base(l) + m].x += wp * er;
-                    acc[feat_base(l) + m].y += wp * ei;
-                }
-            }
-        }
-    }
+                acc[feat_base(l) + m].y += p * ei;
+        });
+    });
     uint32_t const nb = e1 - e0;
     float const total_w = float(nb);
     float const inv_nb = 1.0f / total_w;
@@ -1270,9 +1296,14 @@ int launch_qlm_even_multi(int lmax, const fa::DevBox& db, const float* pts, uint
     {
         unsigned const blocks = (n + kBlock - 1) / kBlock;
         uint32_t const cap = stage_capacity(nl, n, order);
-        FA_KLAUNCH(qlm_even_multi_kernel<LMAX>, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, pts,
-                   n, nl->neighbors, nl->distances, nl->weights, nl->offsets, weighted, mask, o, ql, stride, order,
-                   cap);
+        if (weighted)
+            FA_KLAUNCH((qlm_even_multi_kernel<LMAX, true>), dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(),
+                       db, pts, n, nl->neighbors, nl->distances, nl->weights, nl->offsets, mask, o, ql, stride, order,
+                       cap);
+        else
+            FA_KLAUNCH((qlm_even_multi_kernel<LMAX, false>), dim3(blocks), dim3(kBlock), size_t(cap) * 8,
+                       fa::stream(), db, pts, n, nl->neighbors, nl->distances, nl->weights, nl->offsets, mask, o, ql,
+                       stride, order, cap);
         FA_HIP(hipGetLastError());
         return FA_OK;
     }
