@@ -174,8 +174,16 @@ struct OrderCache
     bool use = false; // false: the caller's order is already coherent
     uint32_t* order = nullptr;
     uint32_t cap = 0;
+    uint64_t stamp = 0; // last use (0: never)
 };
-OrderCache g_order;
+// A few point arrays keep their decision and order at once (a workload that
+// alternates between systems would otherwise rebuild the order per call).
+// The order only steers which thread handles which point, so an entry whose
+// array was overwritten in place still yields correct results.
+constexpr int kOrderEntries = 4;
+OrderCache g_order[kOrderEntries];
+uint64_t g_order_clock = 0;
+int g_order_last = -1;
 
 } // namespace
 
@@ -186,12 +194,22 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
     *out = nullptr;
     if (n < kSpatialOrderMin)
         return FA_OK;
-    OrderCache& c = g_order;
-    if (c.pts == d_points && c.n == n && std::memcmp(&c.box, &box, sizeof(box)) == 0)
+    int victim = 0;
+    for (int e = 0; e < kOrderEntries; ++e)
     {
-        *out = c.use ? c.order : nullptr;
-        return FA_OK;
+        OrderCache& c = g_order[e];
+        if (c.pts == d_points && c.n == n && std::memcmp(&c.box, &box, sizeof(box)) == 0)
+        {
+            c.stamp = ++g_order_clock;
+            g_order_last = e;
+            *out = c.use ? c.order : nullptr;
+            return FA_OK;
+        }
+        if (c.stamp < g_order[victim].stamp)
+            victim = e;
     }
+    OrderCache& c = g_order[victim];
+    c.pts = nullptr;
     hipStream_t s = stream();
     double const v = double(box_volume(box));
     double const spacing = box.is2D ? std::sqrt(v / n) : std::cbrt(v / n);
@@ -214,6 +232,8 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
             c.n = n;
             c.box = box;
             c.use = false;
+            c.stamp = ++g_order_clock;
+            g_order_last = victim;
             return FA_OK;
         }
     }
@@ -255,13 +275,19 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
     c.n = n;
     c.box = box;
     c.use = true;
+    c.stamp = ++g_order_clock;
+    g_order_last = victim;
     *out = c.order;
     return FA_OK;
 }
 
+// The order that spatial_order returned last, if it covers n points.
 const uint32_t* cached_spatial_order(uint32_t n)
 {
-    return (g_order.pts && g_order.use && g_order.n == n && n >= kSpatialOrderMin) ? g_order.order : nullptr;
+    if (g_order_last < 0)
+        return nullptr;
+    OrderCache const& c = g_order[g_order_last];
+    return (c.pts && c.use && c.n == n && n >= kSpatialOrderMin) ? c.order : nullptr;
 }
 
 int batch_grid_build(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, void* tmp,

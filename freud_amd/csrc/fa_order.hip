@@ -635,9 +635,21 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
 // reduction through wave shuffles and LDS (the same order on every run).
 constexpr int kMaxNm = 41; // 2l + 1 for l <= 20
 constexpr int kSumThreads = 1024;
-__global__ __launch_bounds__(kSumThreads) void sum_partials_kernel(const float2* __restrict__ part, uint32_t nblocks,
-                                                                   int nm, float2* __restrict__ out)
+// One reduction per workgroup: jobs.job[blockIdx.x] sums nblocks rows of nm
+// complex values (the system q_lm of one l).
+constexpr int kSumJobs = 16;
+struct SumJobs
 {
+    const float2* part[kSumJobs];
+    float2* out[kSumJobs];
+    int nm[kSumJobs];
+};
+
+__global__ __launch_bounds__(kSumThreads) void sum_partials_kernel(SumJobs jobs, uint32_t nblocks)
+{
+    const float2* __restrict__ const part = jobs.part[blockIdx.x];
+    float2* __restrict__ const out = jobs.out[blockIdx.x];
+    int const nm = jobs.nm[blockIdx.x];
     __shared__ float2 s_red[kSumThreads / 64][kMaxNm];
     uint32_t const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (int t0 = 0; t0 < nm; t0 += 8)
@@ -1353,13 +1365,14 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
     const uint32_t* sorder = nullptr;
     if (int rc = spatial_order(*box, d_points, n, &sorder))
         return rc;
-    // Partial sums: blocks x (2l + 1) complex per l for the base q_lm (all l
-    // live at once, for the fused pass), one such slab for the averaged q_lm,
-    // and 2l + 1 complex for the result.
+    // Partial sums: blocks x (2l + 1) complex per l for the base and for the
+    // averaged q_lm (every l live at once: the fused pass writes all of them,
+    // and all system sums are reduced and copied back together), then
+    // 2l + 1 complex per l for the system q_lm.
     uint32_t even_mask = 0;
     int lmax = 0;
     bool fuse = n_l >= 2;
-    size_t part_total = 0, nm_max = 1;
+    size_t part_total = 0, nm_total = 0;
     for (int k = 0; k < n_l; ++k)
     {
         int const l = int(ls[k]);
@@ -1369,15 +1382,14 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         even_mask |= 1u << l;
         lmax = std::max(lmax, l);
         part_total += size_t(blocks) * (2 * l + 1);
-        nm_max = std::max<size_t>(nm_max, 2 * l + 1);
+        nm_total += 2 * l + 1;
     }
     void* tmp = nullptr;
-    size_t const pb_max = sizeof(float2) * size_t(blocks) * nm_max;
-    if (int rc = scratch(6, sizeof(float2) * part_total + pb_max + sizeof(float2) * nm_max + 256, &tmp))
+    if (int rc = scratch(6, sizeof(float2) * (2 * part_total + nm_total) + 256, &tmp))
         return rc;
     float2* const part_base = static_cast<float2*>(tmp);
-    float2* const part_ave = part_base + part_total;
-    float2* const sys = reinterpret_cast<float2*>(reinterpret_cast<char*>(part_ave) + pb_max);
+    float2* const part_ave_base = part_base + part_total;
+    float2* const sys_base = part_ave_base + part_total;
     if (fuse && n)
     {
         MultiLOut o{};
@@ -1400,6 +1412,7 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         int const l = int(ls[k]);
         int const nm = 2 * l + 1;
         float2* const part = part_base + poff;
+        float2* const part_ave = part_ave_base + poff;
         poff += size_t(blocks) * nm;
         float2* qlm_l = reinterpret_cast<float2*>(d_qlm) + size_t(n) * moff;
         if (n && !fuse)
@@ -1448,27 +1461,51 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
             }
             FA_HIP(hipGetLastError());
         }
-        // system order: Steinhardt.cc:291-327
-        std::vector<std::complex<float>> hsys(nm, 0.0f);
-        if (n)
+        moff += nm;
+    }
+    // system order (Steinhardt.cc:291-327): every l's system q_lm reduced on
+    // the device, one copy back, one synchronisation
+    std::vector<std::complex<float>> hsys(nm_total, 0.0f);
+    if (n)
+    {
+        size_t po = 0, so = 0;
+        for (int k0 = 0; k0 < n_l; k0 += kSumJobs)
         {
-            FA_KLAUNCH(sum_partials_kernel, dim3(1), dim3(kSumThreads), 0, s, average ? part_ave : part, blocks, nm,
-                               sys);
+            SumJobs jobs{};
+            int const nj = std::min(kSumJobs, n_l - k0);
+            for (int t = 0; t < nj; ++t)
+            {
+                int const nm = 2 * int(ls[k0 + t]) + 1;
+                jobs.part[t] = (average ? part_ave_base : part_base) + po;
+                jobs.out[t] = sys_base + so;
+                jobs.nm[t] = nm;
+                po += size_t(blocks) * nm;
+                so += nm;
+            }
+            FA_KLAUNCH(sum_partials_kernel, dim3(nj), dim3(kSumThreads), 0, s, jobs, blocks);
             FA_HIP(hipGetLastError());
-            FA_HIP(hipMemcpyAsync(hsys.data(), sys, sizeof(float2) * nm, hipMemcpyDeviceToHost, s));
-            FA_HIP(hipStreamSynchronize(s));
         }
+        FA_HIP(hipMemcpyAsync(hsys.data(), sys_base, sizeof(float2) * nm_total, hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+    }
+    size_t so = 0;
+    for (int k = 0; k < n_l; ++k)
+    {
+        int const l = int(ls[k]);
+        int const nm = 2 * l + 1;
+        const std::complex<float>* hs = hsys.data() + so;
+        so += nm;
         float const nf = float(4.0 * M_PI / nm);
         float calc = 0;
         for (int t = 0; t < nm; ++t)
-            calc += std::norm(hsys[t]);
+            calc += std::norm(hs[t]);
         float const ql_sys = std::sqrt(calc * nf);
         if (wl)
         {
             float* w3 = nullptr;
             if (int rc = wigner_table(l, &w3))
                 return rc;
-            float v = reduce_wigner3j_host(hsys.data(), l);
+            float v = reduce_wigner3j_host(hs, l);
             if (wl_norm)
             {
                 float const wn = std::sqrt(nf) / ql_sys;
@@ -1478,7 +1515,6 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         }
         else
             order[k] = ql_sys;
-        moff += nm;
     }
     return FA_OK;
 }
