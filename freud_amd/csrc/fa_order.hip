@@ -581,9 +581,14 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
     if (valid)
     {
         uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
+        // the neighbour index of bond e + 1 is loaded before bond e's row is
+        // gathered and added
+        uint32_t jn = e0 < e1 ? nbr[2 * e0 + 1] : 0u;
         for (uint32_t e = e0; e < e1; ++e)
         {
-            const float2* src = qlm + uint64_t(nbr[2 * e + 1]) * NM;
+            const float2* src = qlm + uint64_t(jn) * NM;
+            if (e + 1 < e1)
+                jn = nbr[2 * (e + 1) + 1];
 #pragma unroll
             for (int t = 0; t < NM; ++t)
             {
@@ -608,35 +613,19 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
         float const nf = float(4.0 * M_PI / NM);
         qlave[uint64_t(i) * ql_stride + ql_col] = sqrtf(norm * nf);
     }
-    __shared__ float2 s_red[kBlock];
-    float const invn = 1.0f / float(n);
-    for (int t = 0; t < NM; ++t)
-    {
-        s_red[threadIdx.x] = valid ? make_float2(acc[t].x * invn, acc[t].y * invn) : make_float2(0.0f, 0.0f);
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1)
-        {
-            if (threadIdx.x < s)
-            {
-                s_red[threadIdx.x].x += s_red[threadIdx.x + s].x;
-                s_red[threadIdx.x].y += s_red[threadIdx.x + s].y;
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0)
-            sys_partial[uint64_t(blockIdx.x) * NM + t] = s_red[0];
-        __syncthreads();
-    }
+    block_qlm_partial<L>(valid, reinterpret_cast<const float*>(acc), 1.0f / float(n),
+                         sys_partial + uint64_t(blockIdx.x) * NM);
 }
 
-// Sum of the blocks' partial q_lm rows (nm <= kMaxNm values each): one block
-// of kSumThreads threads, each adding its rows (tid, tid + kSumThreads, ...)
-// into registers with the loads of four rows in flight together, then a fixed
-// reduction through wave shuffles and LDS (the same order on every run).
+// Sum of the blocks' partial q_lm rows (nm <= kMaxNm values each): one
+// workgroup of kSumThreads threads per sum, each thread adding its rows (tid,
+// tid + kSumThreads, ...) into registers with the loads of four rows in flight
+// together, then a fixed reduction through wave shuffles and LDS (the same
+// order on every run).
 constexpr int kMaxNm = 41; // 2l + 1 for l <= 20
 constexpr int kSumThreads = 1024;
-// One reduction per workgroup: jobs.job[blockIdx.x] sums nblocks rows of nm
-// complex values (the system q_lm of one l).
+// Workgroup b sums the nblocks rows of jobs.nm[b] complex values at
+// jobs.part[b] into jobs.out[b] (the system q_lm of one l).
 constexpr int kSumJobs = 16;
 struct SumJobs
 {
