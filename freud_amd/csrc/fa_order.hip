@@ -617,12 +617,13 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
                          sys_partial + uint64_t(blockIdx.x) * NM);
 }
 
-// Sum of the blocks' partial q_lm rows (nm <= kMaxNm values each): one
-// workgroup of kSumThreads threads per sum, each thread adding its rows (tid,
-// tid + kSumThreads, ...) into registers with the loads of four rows in flight
-// together, then a fixed reduction through wave shuffles and LDS (the same
-// order on every run).
-constexpr int kMaxNm = 41; // 2l + 1 for l <= 20
+// Sum of the blocks' partial q_lm rows (nm = 2l + 1 <= 41 complex values each), one
+// workgroup of kSumThreads threads per sum.  The rows are read as one flat
+// array with a thread stride T that is a multiple of nm, so every thread
+// always adds the same column and consecutive threads read consecutive
+// values (coalesced); four loads are in flight per thread.  The T / nm
+// per-thread sums of each column are then added in thread order through LDS
+// (the same order on every run).
 constexpr int kSumThreads = 1024;
 // Workgroup b sums the nblocks rows of jobs.nm[b] complex values at
 // jobs.part[b] into jobs.out[b] (the system q_lm of one l).
@@ -638,61 +639,43 @@ __global__ __launch_bounds__(kSumThreads) void sum_partials_kernel(SumJobs jobs,
 {
     const float2* __restrict__ const part = jobs.part[blockIdx.x];
     float2* __restrict__ const out = jobs.out[blockIdx.x];
-    int const nm = jobs.nm[blockIdx.x];
-    __shared__ float2 s_red[kSumThreads / 64][kMaxNm];
-    uint32_t const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    for (int t0 = 0; t0 < nm; t0 += 8)
+    uint32_t const nm = uint32_t(jobs.nm[blockIdx.x]);
+    uint32_t const per = kSumThreads / nm; // threads per column
+    uint32_t const T = per * nm;
+    uint32_t const t = threadIdx.x;
+    __shared__ float2 s_col[kSumThreads];
+    float2 acc = make_float2(0.0f, 0.0f);
+    if (t < T)
     {
-        // columns t0 .. t0 + 7 of every row
-        float2 acc[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-            acc[c] = make_float2(0.0f, 0.0f);
-        for (uint32_t bl = threadIdx.x; bl < nblocks; bl += 4 * kSumThreads)
+        uint64_t const total = uint64_t(nblocks) * nm;
+        for (uint64_t e = t; e < total; e += 4ull * T)
         {
-            float2 v[4][8];
+            float2 v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r)
             {
-                uint32_t const row = bl + uint32_t(r) * kSumThreads;
-#pragma unroll
-                for (int c = 0; c < 8; ++c)
-                    v[r][c] = (row < nblocks && t0 + c < nm) ? part[uint64_t(row) * nm + t0 + c]
-                                                             : make_float2(0.0f, 0.0f);
+                uint64_t const x = e + uint64_t(r) * T;
+                v[r] = x < total ? part[x] : make_float2(0.0f, 0.0f);
             }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
-#pragma unroll
-                for (int c = 0; c < 8; ++c)
-                {
-                    acc[c].x += v[r][c].x;
-                    acc[c].y += v[r][c].y;
-                }
-        }
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-        {
-            float x = acc[c].x, y = acc[c].y;
-#pragma unroll
-            for (int d = 32; d > 0; d >>= 1)
             {
-                x += __shfl_xor(x, d);
-                y += __shfl_xor(y, d);
+                acc.x += v[r].x;
+                acc.y += v[r].y;
             }
-            if (lane == 0 && t0 + c < nm)
-                s_red[wave][t0 + c] = make_float2(x, y);
         }
     }
+    s_col[t] = acc;
     __syncthreads();
-    if (threadIdx.x < uint32_t(nm))
+    if (t < nm)
     {
         float2 r = make_float2(0.0f, 0.0f);
-        for (uint32_t w = 0; w < kSumThreads / 64; ++w)
+        for (uint32_t k = 0; k < per; ++k)
         {
-            r.x += s_red[w][threadIdx.x].x;
-            r.y += s_red[w][threadIdx.x].y;
+            r.x += s_col[t + k * nm].x;
+            r.y += s_col[t + k * nm].y;
         }
-        out[threadIdx.x] = r;
+        out[t] = r;
     }
 }
 
