@@ -449,7 +449,11 @@ const uint32_t* cached_spatial_order(uint32_t n);
 
 // Exclusive scan of n uint32 counts into n+1 offsets (offsets[n] = total);
 // returns the 64-bit total through *total (host) when total != nullptr.
-int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total);
+// d_out[0] = 0, d_out[t + 1] = sum of d_in[0..t]; *total (if given) = the 64-bit
+// sum, read back with one synchronisation.  If d_extra is given, the u32 it
+// points to is copied to *h_extra in the same synchronisation.
+int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total,
+                       const uint32_t* d_extra = nullptr, uint32_t* h_extra = nullptr);
 // The same scan for code that may be captured into a HIP graph: no memset, no
 // host synchronisation; `tmp` holds scan_tmp_bytes(n) bytes.
 size_t scan_tmp_bytes(uint32_t n);

@@ -16,7 +16,8 @@ struct widen
 };
 } // namespace
 
-int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total)
+int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total, const uint32_t* d_extra,
+                       uint32_t* h_extra)
 {
     hipStream_t s = stream();
     FA_HIP(hipMemsetAsync(d_out, 0, sizeof(uint32_t), s));
@@ -24,6 +25,11 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
     {
         if (total)
             *total = 0;
+        if (d_extra)
+        {
+            FA_HIP(hipMemcpyAsync(h_extra, d_extra, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            FA_HIP(hipStreamSynchronize(s));
+        }
         return FA_OK;
     }
     size_t bytes_scan = 0, bytes_red = 0;
@@ -44,8 +50,11 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
     {
         FA_HIP(rocprim::reduce(t_red, bytes_red, it, d_total, uint64_t(0), size_t(n), rocprim::plus<uint64_t>(), s));
         FA_HIP(hipMemcpyAsync(total, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        FA_HIP(hipStreamSynchronize(s));
     }
+    if (d_extra)
+        FA_HIP(hipMemcpyAsync(h_extra, d_extra, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (total || d_extra)
+        FA_HIP(hipStreamSynchronize(s));
     return FA_OK;
 }
 

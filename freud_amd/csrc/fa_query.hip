@@ -1086,25 +1086,6 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         }
         if (rc)
             return rc;
-        FA_HIP(hipMemcpyAsync(&h_nflag, nflag, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        FA_HIP(hipStreamSynchronize(s));
-        if (h_nflag)
-        {
-            // bounded work space: at most ~256 MiB of per-query candidate keys at once
-            uint64_t const per_q = uint64_t(nq->n) * sizeof(uint64_t);
-            uint32_t const chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (256ull << 20) / per_q)));
-            for (uint32_t c0 = 0; c0 < h_nflag; c0 += chunk)
-            {
-                uint32_t const nc = std::min(chunk, h_nflag - c0);
-                void* work = nullptr;
-                if (int rc2 = fa::scratch(5, uint64_t(nc) * nq->n * sizeof(uint64_t) + 64, &work))
-                    return rc2;
-                FA_KLAUNCH(knn_exhaustive_kernel, dim3(nc), dim3(kBlock), 0, s, *dbp, nq->d_points, nq->n,
-                                   qpts, flagged + c0, k, rlim * rlim, a.r_min, a.exclude_ii,
-                                   static_cast<uint64_t*>(work), out->counts, sel);
-                FA_HIP(hipGetLastError());
-            }
-        }
     }
     else
     {
@@ -1133,9 +1114,33 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         if (rc)
             return rc;
     }
+    // The number of queries flagged for the exhaustive kernel comes back with
+    // the bond total (one synchronisation when none is flagged).  Flagged
+    // queries hold count 0 until the exhaustive kernel has run; the offsets
+    // are then scanned again.
     uint64_t total = 0;
-    if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
+    if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total,
+                                        nq->kind == FA_NQ_AABB ? nflag : nullptr, &h_nflag))
         return rc;
+    if (h_nflag)
+    {
+        // bounded work space: at most ~256 MiB of per-query candidate keys at once
+        uint64_t const per_q = uint64_t(nq->n) * sizeof(uint64_t);
+        uint32_t const chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (256ull << 20) / per_q)));
+        for (uint32_t c0 = 0; c0 < h_nflag; c0 += chunk)
+        {
+            uint32_t const nc = std::min(chunk, h_nflag - c0);
+            void* work = nullptr;
+            if (int rc2 = fa::scratch(5, uint64_t(nc) * nq->n * sizeof(uint64_t) + 64, &work))
+                return rc2;
+            FA_KLAUNCH(knn_exhaustive_kernel, dim3(nc), dim3(kBlock), 0, s, *dbp, nq->d_points, nq->n, qpts,
+                       flagged + c0, k, rlim * rlim, a.r_min, a.exclude_ii, static_cast<uint64_t*>(work), out->counts,
+                       sel);
+            FA_HIP(hipGetLastError());
+        }
+        if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
+            return rc;
+    }
     if (int rc = fa::nlist_reserve(out, total, n_query))
         return rc;
     out->num_bonds = total;

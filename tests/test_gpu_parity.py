@@ -118,6 +118,23 @@ def test_knn_r_min(oracle, name, box, pts):
         assert np.all(nl.distances >= np.float32(r_min))
 
 
+def test_knn_exhaustive_paths(oracle):
+    """Queries whose k-th neighbour lies beyond half the smallest plane
+    distance go to the exhaustive kernel beside ordinary ones (a dense clump
+    plus isolated points), and k > 64 sends every query there."""
+    rng = np.random.default_rng(9)
+    box = Box.cube(12.0)
+    clump = (rng.standard_normal((300, 3)) * 0.5).astype(np.float32)
+    lone = np.array([[6.0, 6.0, 6.0], [5.5, -5.9, 5.8], [-5.7, 5.6, -5.5], [3.0, 3.0, 3.0]], np.float32)
+    pts = np.concatenate([clump, lone]).astype(np.float32)
+    pts = box.wrap(pts).astype(np.float32)
+    for k in (8, 80):
+        qa = dict(num_neighbors=k, exclude_ii=True)
+        nl = freud_amd.AABBQuery(box, pts).query(pts, qa).toNeighborList()
+        ref = oracle.query(_bt(box), pts, pts, qa, nthreads=8)
+        _assert_nlist_equal(nl, ref)
+
+
 def test_knn_linkcell(oracle):
     name, box, pts = SYSTEMS[1]
     nq = freud_amd.LinkCell(box, pts, cell_width=2.5)
