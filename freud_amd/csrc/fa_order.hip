@@ -118,52 +118,18 @@ __device__ inline void bond_angles(const fa::DevBox& b, float vx, float vy, floa
     }
 }
 
-// Accumulate w * Y_L^m (m = 0..L) of one bond into acc.
-template<int L> __device__ inline void accumulate_ylm(float ct, float st, float cp, float sp, float w, float2* acc)
+// One associated-Legendre sweep up to LMAX for one bond: for m = 0..LMAX the
+// recurrence runs l = m..LMAX and calls sink(l, m, P_lm, cos m phi,
+// sin m phi) for every l >= LMIN (every even l >= LMIN when EVEN); l and m are
+// constants once the loops are unrolled.  Products and sums here and in the
+// sinks are contracted to FMA (the q_lm results are compared within a
+// tolerance; the bond geometry before the sweep is evaluated uncontracted).
+// Every q_lm path shares this code, so a given (l, m) gets the same float
+// operations whatever else is evaluated beside it.
+template<int LMAX, int LMIN, bool EVEN, class Sink>
+__device__ inline void ylm_sweep(float ct, float st, float cp, float sp, Sink&& sink)
 {
-    float pmm = 0.28209479177387814f;
-    float er = 1.0f, ei = 0.0f;
-#pragma unroll
-    for (int m = 0; m <= L; ++m)
-    {
-        if (m > 0)
-        {
-            pmm = kYlm.amm[m] * st * pmm;
-            float const nr = er * cp - ei * sp;
-            float const ni = er * sp + ei * cp;
-            er = nr;
-            ei = ni;
-        }
-        float plm;
-        if (m == L)
-            plm = pmm;
-        else
-        {
-            float p0 = pmm;
-            float p1 = kYlm.bm1[m] * ct * pmm;
-#pragma unroll
-            for (int l = m + 2; l <= L; ++l)
-            {
-                float const p2 = kYlm.a[l][m] * (ct * p1 - kYlm.b[l][m] * p0);
-                p0 = p1;
-                p1 = p2;
-            }
-            plm = p1;
-        }
-        float const wp = w * plm;
-        acc[m].x += wp * er;
-        if (m > 0) // e^{i 0 phi} = 1: the m = 0 imaginary sum stays +0
-            acc[m].y += wp * ei;
-    }
-}
-
-// One associated-Legendre sweep up to LMAX for one bond, with the recurrence
-// and operation order of accumulate_ylm; calls sink(l, m, P_lm, cos m phi,
-// sin m phi) for every even l >= LMIN and 0 <= m <= l (l and m are constants
-// once the loops are unrolled).
-template<int LMAX, int LMIN, class Sink>
-__device__ inline void even_ylm_sweep(float ct, float st, float cp, float sp, Sink&& sink)
-{
+#pragma clang fp contract(fast)
     float pmm = 0.28209479177387814f;
     float er = 1.0f, ei = 0.0f;
 #pragma unroll
@@ -195,10 +161,22 @@ __device__ inline void even_ylm_sweep(float ct, float st, float cp, float sp, Si
                 p0 = p1;
                 p1 = p;
             }
-            if (l >= LMIN && (l & 1) == 0)
+            if (l >= LMIN && (!EVEN || (l & 1) == 0))
                 sink(l, m, p, er, ei);
         }
     }
+}
+
+// Accumulate w * Y_L^m (m = 0..L) of one bond into acc.
+template<int L> __device__ inline void accumulate_ylm(float ct, float st, float cp, float sp, float w, float2* acc)
+{
+    ylm_sweep<L, L, false>(ct, st, cp, sp, [&](int, int m, float p, float er, float ei) {
+#pragma clang fp contract(fast)
+        float const wp = w * p;
+        acc[m].x += wp * er;
+        if (m > 0) // e^{i 0 phi} = 1: the m = 0 imaginary sum stays +0
+            acc[m].y += wp * ei;
+    });
 }
 
 // Steinhardt.cc:120-222: q_lm(i) = sum_b w Y_lm / sum_b w and
@@ -472,7 +450,8 @@ __global__ __launch_bounds__(kBlock) void qlm_even_multi_kernel(fa::DevBox b, co
     if (valid)
         for_each_bond(b, pts, i, src, wts, WEIGHTED, offsets[i], offsets[i + 1],
                       [&](float ct, float st, float cp, float sp, float w) {
-                          even_ylm_sweep<LMAX, 0>(ct, st, cp, sp, [&](int l, int m, float p, float er, float ei) {
+                          ylm_sweep<LMAX, 0, true>(ct, st, cp, sp, [&](int l, int m, float p, float er, float ei) {
+#pragma clang fp contract(fast)
                               // w * p == p exactly for w == 1
                               float const wp = WEIGHTED ? w * p : p;
                               acc[even_base(l) + m].x += wp * er;
@@ -858,7 +837,8 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
         acc[t] = make_float2(0.0f, 0.0f);
     uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
     for_each_bond(b, pts, i, src, nullptr, false, e0, e1, [&](float ct, float st, float cp, float sp, float) {
-        even_ylm_sweep<LM, 4>(ct, st, cp, sp, [&](int l, int m, float p, float er, float ei) {
+        ylm_sweep<LM, 4, true>(ct, st, cp, sp, [&](int l, int m, float p, float er, float ei) {
+#pragma clang fp contract(fast)
             acc[feat_base(l) + m].x += p * er;
             if (m > 0)
                 acc[feat_base(l) + m].y += p * ei;
