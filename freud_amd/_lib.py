@@ -132,6 +132,7 @@ _SIGS = {
                                    ctypes.POINTER(ctypes.c_int)]),
     "fa_tune_set": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "fa_tune_reset": (ctypes.c_int, []),
+    "fa_tune_variants": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fa_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "fa_profile_reset": (ctypes.c_int, []),
     "fa_profile_count": (ctypes.c_int, []),
@@ -271,6 +272,13 @@ def tune_load(table):
 
 def tune_reset():
     check(lib().fa_tune_reset())
+
+
+def tune_variants(analysis, key):
+    """Number of launch variants the autotuner chooses among for (analysis, key)."""
+    n = lib().fa_tune_variants(int(analysis), int(key))
+    check(min(n, 0))
+    return n
 
 
 def synchronize():

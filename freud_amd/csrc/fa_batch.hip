@@ -390,7 +390,7 @@ int tune(fa_batch* b)
     double const density = vol > 0 ? pts / vol : 0.0;
     if (do_ball(b))
     {
-        int const key = fa::tune_key(density, b->plan.ball_r_max) + 1000;
+        int const key = fa::tune_key(density, b->plan.ball_r_max) + fa::kBatchTuneKey;
         int v = fa::tuned_variant(fa::TUNE_BALL, key);
         if (v < 0 && fa::tuning_enabled())
         {
@@ -426,7 +426,7 @@ int tune(fa_batch* b)
     }
     if (do_rdf(b))
     {
-        int const key = fa::tune_key(density, b->plan.rdf_r_max) + 1000;
+        int const key = fa::tune_key(density, b->plan.rdf_r_max) + fa::kBatchTuneKey;
         int v = fa::tuned_variant(fa::TUNE_RDF_SELF, key);
         if (v < 0 && fa::tuning_enabled())
         {
@@ -490,6 +490,11 @@ int capture(fa_batch* b)
 }
 
 } // namespace
+
+int fa::batch_rdf_shape_count()
+{
+    return kRdfShapes;
+}
 
 fa_batch::~fa_batch()
 {

@@ -59,13 +59,19 @@ void prof_end();
 // Autotuner table (fa_tune.hip).  Analyses that the tuner covers:
 enum TunedAnalysis
 {
-    TUNE_RDF_SELF = 1, // variant = cells r_max / R wide, R = 2 + variant
-    TUNE_BALL = 2,     // variant = cells r_max / R wide, R = 1 + variant
+    TUNE_RDF_SELF = 1, // variant = a cell shape (R, F): r_max / R wide, r_max / (R F) along x
+    TUNE_BALL = 2,     // variant = cells r_max / R wide, R = 1 + variant (batched ball query)
     TUNE_KNN = 3,      // variant = cells rk, rk / 1.5, rk / 2 wide (rk: radius holding ~2k points)
 };
+// Keys of the batched analyses are offset by this (the single-frame keys are
+// small integers around 0).
+constexpr int kBatchTuneKey = 1000;
 bool tuning_enabled();
 int tuned_variant(int analysis, int key); // -1 when not tuned yet
 void record_variant(int analysis, int key, int variant);
+// Number of launch variants that the tuner chooses among.
+int rdf_row_variant_count();   // single-frame self RDF
+int batch_rdf_shape_count();   // batched self RDF
 // Elapsed time of the work `launch` enqueues on the library stream (synchronises).
 int time_launch(const std::function<int()>& launch, float* ms);
 // Problem class of a query: expected points in an r^3 volume, in half-octaves.

@@ -923,7 +923,7 @@ struct RowVariant
 {
     int R, F;
 };
-constexpr RowVariant kRowVariant[] = {{2, 1}, {3, 1}, {4, 1}, {2, 2}, {2, 4}, {3, 2}, {2, 8}, {3, 4}};
+constexpr RowVariant kRowVariant[] = {{2, 1}, {3, 1}, {4, 1}, {2, 2}, {2, 4}, {3, 2}, {2, 8}, {3, 4}, {2, 16}};
 constexpr int kRowVariants = int(sizeof(kRowVariant) / sizeof(kRowVariant[0]));
 
 // Self RDF through the row-cluster kernel with cells at least r_max / R wide
@@ -1151,6 +1151,10 @@ int grid_blocks(uint64_t work)
 } // namespace
 
 namespace fa {
+int rdf_row_variant_count()
+{
+    return kRowVariants;
+}
 
 // Batched self RDF with cells at least r_max / R wide (R in 2..4): counts has
 // n_frames x bins entries.  Bins beyond 2048 are not supported here (each wave

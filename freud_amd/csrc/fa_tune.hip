@@ -103,4 +103,19 @@ int fa_tune_reset(void)
     return FA_OK;
 }
 
+int fa_tune_variants(int analysis, int key)
+{
+    switch (analysis)
+    {
+    case fa::TUNE_RDF_SELF:
+        return key >= fa::kBatchTuneKey / 2 ? fa::batch_rdf_shape_count() : fa::rdf_row_variant_count();
+    case fa::TUNE_BALL:
+        return 2;
+    case fa::TUNE_KNN:
+        return 3;
+    default:
+        return fa::fail(FA_ERR_INVALID, "unknown tuned analysis");
+    }
+}
+
 } // extern "C"

@@ -130,6 +130,9 @@ int fa_tune_count(void);
 int fa_tune_get(int idx, int* analysis, int* key, int* variant);
 int fa_tune_set(int analysis, int key, int variant);
 int fa_tune_reset(void);
+/* Number of launch variants the tuner chooses among for (analysis, key):
+ * variants 0 .. count-1 are valid in fa_tune_set. */
+int fa_tune_variants(int analysis, int key);
 
 /* ---------------------------------------------------------------- batched trajectories
  * No counterpart in the reference, which analyses one frame per call (a

@@ -204,16 +204,19 @@ def test_rdf_self_cluster_cases(oracle, L, r_max, n, layout):
 
 
 def _rdf_variants(box, n, r_max):
-    """Presets each of the 8 self-RDF launch variants (cells r_max / R wide,
-    r_max / (R F) along x, for (R, F) = (2, 1), (3, 1), (4, 1), (2, 2), (2, 4),
-    (3, 2), (2, 8), (3, 4)) for this problem class in turn; the autotuner keys a class by the
-    expected number of points in an r_max^3 volume, in half-octaves."""
+    """Presets each self-RDF launch variant (cells r_max / R wide, r_max /
+    (R F) along x, for (R, F) = (2, 1), (3, 1), (4, 1), (2, 2), (2, 4), (3, 2),
+    (2, 8), (3, 4), (2, 16)) for this problem class in turn;
+    the autotuner keys a class by the expected number of points in an r_max^3
+    volume, in half-octaves."""
     from freud_amd import _lib
 
     key = int(np.floor(2.0 * np.log2(n / box.volume * r_max ** 3) + 0.5))
+    count = _lib.tune_variants(1, key)
+    assert count == 9
     before = _lib.tune_table()
     try:
-        for v in range(8):
+        for v in range(count):
             _lib.tune_load({(1, key): v})
             yield v
     finally:
