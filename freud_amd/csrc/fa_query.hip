@@ -131,18 +131,18 @@ template<typename K> __device__ inline void insertion_sort(K* a, uint32_t n)
     }
 }
 
-// Writes bond t of a block given its query index i and key.
+// Writes the bond of query point i (at qx, qy, qz) given its key, at bond
+// index bidx.
 template<int FORMULA, bool BYDIST>
-__device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpts, const float* __restrict__ pts,
-                                  uint32_t i, uint64_t key, uint64_t bidx, uint32_t* __restrict__ nb_out,
-                                  float* __restrict__ d_out, float* __restrict__ w_out, float* __restrict__ v_out)
+__device__ inline void write_bond_at(const DevBox& b, float qx, float qy, float qz, const float* __restrict__ pts,
+                                     uint32_t i, uint64_t key, uint64_t bidx, uint32_t* __restrict__ nb_out,
+                                     float* __restrict__ d_out, float* __restrict__ w_out, float* __restrict__ v_out)
 {
     uint32_t const jc = uint32_t(key);
     uint32_t const j = jc >> 5;
     int const code = int(jc & 31u);
     float rx, ry, rz;
-    fa::bond_vector<FORMULA>(b, qpts[3 * i], qpts[3 * i + 1], qpts[3 * i + 2], pts[3 * j], pts[3 * j + 1],
-                             pts[3 * j + 2], code, rx, ry, rz);
+    fa::bond_vector<FORMULA>(b, qx, qy, qz, pts[3 * j], pts[3 * j + 1], pts[3 * j + 2], code, rx, ry, rz);
     float const r_sq = (rx * rx + ry * ry) + rz * rz;
     reinterpret_cast<uint2*>(nb_out)[bidx] = make_uint2(i, j);
     d_out[bidx] = sqrtf(r_sq);
@@ -152,11 +152,21 @@ __device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpt
     v_out[3 * bidx + 2] = rz;
 }
 
+template<int FORMULA, bool BYDIST>
+__device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpts, const float* __restrict__ pts,
+                                  uint32_t i, uint64_t key, uint64_t bidx, uint32_t* __restrict__ nb_out,
+                                  float* __restrict__ d_out, float* __restrict__ w_out, float* __restrict__ v_out)
+{
+    write_bond_at<FORMULA, BYDIST>(b, qpts[3 * i], qpts[3 * i + 1], qpts[3 * i + 2], pts, i, key, bidx, nb_out, d_out,
+                                   w_out, v_out);
+}
+
 // Block-cooperative store of a block's bonds: keys[t] for t < total belong to
 // the block's query points in processing order (exclusive offsets s_off,
-// point indices s_i); bond t of point lo goes to offsets[s_i[lo]] + t - s_off[lo].
+// point indices s_i, positions s_q); bond t of point lo goes to
+// offsets[s_i[lo]] + t - s_off[lo].
 template<int FORMULA, bool BYDIST, typename K>
-__device__ inline void store_block_bonds(const DevBox& b, const float* __restrict__ qpts, const float* __restrict__ pts,
+__device__ inline void store_block_bonds(const DevBox& b, const float* s_q, const float* __restrict__ pts,
                                          const K* keys, uint32_t total, uint32_t nloc, const uint32_t* s_off,
                                          const uint32_t* s_i, const uint32_t* __restrict__ offsets,
                                          uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
@@ -174,8 +184,8 @@ __device__ inline void store_block_bonds(const DevBox& b, const float* __restric
                 hi = mid;
         }
         uint32_t const i = s_i[lo];
-        write_bond<FORMULA, BYDIST>(b, qpts, pts, i, uint64_t(keys[t]), uint64_t(offsets[i]) + (t - s_off[lo]),
-                                    nb_out, d_out, w_out, v_out);
+        write_bond_at<FORMULA, BYDIST>(b, s_q[3 * lo], s_q[3 * lo + 1], s_q[3 * lo + 2], pts, i, uint64_t(keys[t]),
+                                       uint64_t(offsets[i]) + (t - s_off[lo]), nb_out, d_out, w_out, v_out);
     }
 }
 
@@ -198,6 +208,7 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
+    __shared__ float s_q[3 * kBlock];
     uint32_t const tid = threadIdx.x;
     float qx = 0.0f, qy = 0.0f, qz = 0.0f;
     uint32_t i = 0, cnt = 0;
@@ -206,6 +217,9 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
         i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
         cnt = offsets[i + 1] - offsets[i];
         s_i[tid] = i;
+        s_q[3 * tid] = qx;
+        s_q[3 * tid + 1] = qy;
+        s_q[3 * tid + 2] = qz;
     }
     uint32_t excl;
     uint32_t const total = block_exclusive_scan(cnt, s_tmp, excl);
@@ -243,7 +257,7 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
     }
     __syncthreads();
     if (in_lds)
-        store_block_bonds<FORMULA, BYDIST>(b, qpts, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+        store_block_bonds<FORMULA, BYDIST>(b, s_q, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
                                            w_out, v_out);
 }
 
@@ -514,6 +528,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
+    __shared__ float s_q[3 * kBlock];
     uint32_t const tid = threadIdx.x;
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
@@ -524,6 +539,9 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
         i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
         n = offsets[i + 1] - offsets[i];
         s_i[tid] = i;
+        s_q[3 * tid] = qx;
+        s_q[3 * tid + 1] = qy;
+        s_q[3 * tid + 2] = qz;
     }
     uint32_t excl;
     uint32_t const total = block_exclusive_scan(n, s_tmp, excl);
@@ -556,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     }
     __syncthreads();
     if (in_lds)
-        store_block_bonds<FORMULA, BYDIST>(b, qpts, pts, s_keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+        store_block_bonds<FORMULA, BYDIST>(b, s_q, pts, s_keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
                                            w_out, v_out);
 }
 
