@@ -1036,10 +1036,7 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         double const rk =
             nq->box.is2D ? std::sqrt(2.0 * k / (M_PI * dens)) : std::cbrt(3.0 * 2.0 * k / (4.0 * M_PI * dens));
         static constexpr double kFrac[3] = {1.0, 1.0 / 1.5, 0.5};
-        auto run_variant = [&](int v) -> int {
-            float const wcell = float(std::max(rk * kFrac[v], 1e-6));
-            int dims[3];
-            fa::grid_dims_for(nq->box, nq->n, wcell, 1, dims);
+        auto run_dims = [&](const int dims[3]) -> int {
             if (int rc = fa::ensure_grid(nq, dims))
                 return rc;
             dbp = &nq->cur->dbox;
@@ -1062,8 +1059,37 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
             return FA_KNN_LAUNCH(64);
 #undef FA_KNN_LAUNCH
         };
+        auto run_variant = [&](int v) -> int {
+            float const wcell = float(std::max(rk * kFrac[v], 1e-6));
+            int dims[3];
+            fa::grid_dims_for(nq->box, nq->n, wcell, 1, dims);
+            return run_dims(dims);
+        };
+        // A grid that another query on these points has built serves when its
+        // cells lie within the range of sizes the tuner chooses from (rk / 2 to
+        // rk, with 25 % slack above): every cell size selects the same
+        // neighbours, and the grid build is saved.
+        int reuse = -1;
+        for (int gs = 0; gs < fa_nq::kGrids && reuse < 0; ++gs)
+        {
+            if (!nq->grid_valid[gs])
+                continue;
+            bool fits = true;
+            for (int d = 0; d < (nq->box.is2D ? 2 : 3); ++d)
+            {
+                double const w = double(npd[d]) / double(nq->grid_dims[gs][d]);
+                fits = fits && w >= 0.5 * rk && w <= 1.25 * rk;
+            }
+            if (fits)
+                reuse = gs;
+        }
         int rc = FA_OK;
-        if (k <= 64)
+        if (k <= 64 && reuse >= 0)
+        {
+            int const dims[3] = {nq->grid_dims[reuse][0], nq->grid_dims[reuse][1], nq->grid_dims[reuse][2]};
+            rc = run_dims(dims);
+        }
+        else if (k <= 64)
         {
             int const key = fa::tune_key(dens, rk) * 128 + int(std::min<uint32_t>(k, 127u));
             int variant = fa::tuned_variant(fa::TUNE_KNN, key);

@@ -135,6 +135,21 @@ def test_knn_exhaustive_paths(oracle):
         _assert_nlist_equal(nl, ref)
 
 
+@pytest.mark.parametrize("name,box,pts", SYSTEMS[:2] + SYSTEMS[3:], ids=[s[0] for s in SYSTEMS[:2] + SYSTEMS[3:]])
+def test_knn_after_ball_query(oracle, name, box, pts):
+    """A kNN query on a neighbour-query object whose ball query already built a
+    grid of a suitable cell size runs on that grid; the NeighborList is the
+    same as from a fresh object."""
+    nq = freud_amd.AABBQuery(box, pts)
+    for r_ball in (1.2, 1.5, 2.0):
+        nq.query(pts, dict(r_max=r_ball, exclude_ii=True)).toNeighborList()
+        for k in (6, 12):
+            qa = dict(num_neighbors=k, exclude_ii=True)
+            nl = nq.query(pts, qa).toNeighborList()
+            ref = oracle.query(_bt(box), pts, pts, qa, nthreads=8)
+            _assert_nlist_equal(nl, ref)
+
+
 def test_knn_linkcell(oracle):
     name, box, pts = SYSTEMS[1]
     nq = freud_amd.LinkCell(box, pts, cell_width=2.5)
