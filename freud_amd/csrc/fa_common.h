@@ -61,8 +61,9 @@ enum TunedAnalysis
 {
     TUNE_RDF_SELF = 1, // variant = a cell shape (R, F): r_max / R wide, r_max / (R F) along x
     TUNE_BALL = 2,     // variant = cells r_max / R wide, R = 1 + variant (batched ball query)
-    TUNE_KNN = 3,      // variant = cells rk, rk / 1.5, rk / 2 wide (rk: radius holding ~2k points)
+    TUNE_KNN = 3,      // variant = cells rk, rk / 1.2, rk / 1.5, rk / 2 wide (rk: radius holding ~2k points)
 };
+constexpr int kKnnVariants = 4;
 // Keys of the batched analyses are offset by this (the single-frame keys are
 // small integers around 0).
 constexpr int kBatchTuneKey = 1000;

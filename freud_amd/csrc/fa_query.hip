@@ -1029,13 +1029,13 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
     {
         rlim = knn_radius_limit(nq, a);
         // cells a fraction of the radius rk that holds ~2k points; the
-        // autotuner picks the fraction (rk, rk/1.5, rk/2): smaller cells mean
-        // fewer candidates but more stencil cells per query
+        // autotuner picks the fraction (rk, rk/1.2, rk/1.5, rk/2): smaller
+        // cells mean fewer candidates but more stencil cells per query
         float const vol = fa::box_volume(nq->box);
         double const dens = double(nq->n) / std::max(double(vol), 1e-30);
         double const rk =
             nq->box.is2D ? std::sqrt(2.0 * k / (M_PI * dens)) : std::cbrt(3.0 * 2.0 * k / (4.0 * M_PI * dens));
-        static constexpr double kFrac[3] = {1.0, 1.0 / 1.5, 0.5};
+        static constexpr double kFrac[fa::kKnnVariants] = {1.0, 1.0 / 1.2, 1.0 / 1.5, 0.5};
         auto run_dims = [&](const int dims[3]) -> int {
             if (int rc = fa::ensure_grid(nq, dims))
                 return rc;
@@ -1096,7 +1096,7 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
             if (variant < 0 && fa::tuning_enabled())
             {
                 float best = 0.0f;
-                for (int v = 0; v < 3; ++v)
+                for (int v = 0; v < fa::kKnnVariants; ++v)
                 {
                     float ms = 0.0f;
                     if (int rc2 = run_variant(v)) // builds the grid
@@ -1111,7 +1111,7 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
                 }
                 fa::record_variant(fa::TUNE_KNN, key, variant);
             }
-            rc = run_variant(std::max(variant, 0));
+            rc = run_variant(std::max(variant, 0) % fa::kKnnVariants);
         }
         else
         {

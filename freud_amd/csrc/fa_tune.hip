@@ -112,7 +112,7 @@ int fa_tune_variants(int analysis, int key)
     case fa::TUNE_BALL:
         return 2;
     case fa::TUNE_KNN:
-        return 3;
+        return fa::kKnnVariants;
     default:
         return fa::fail(FA_ERR_INVALID, "unknown tuned analysis");
     }

@@ -192,10 +192,10 @@ def test_modulus_positive_shortcut_is_exact():
 def test_tuner_variant_counts():
     """The autotuner's variant counts per analysis (no GPU work): nine
     single-frame RDF cell shapes, five batched ones (keys offset by 1000),
-    two batched ball-query cell sizes, three kNN cell sizes."""
+    two batched ball-query cell sizes, four kNN cell sizes."""
     assert _lib.tune_variants(1, 0) == 9
     assert _lib.tune_variants(1, 1003) == 5
     assert _lib.tune_variants(2, 1003) == 2
-    assert _lib.tune_variants(3, 4) == 3
+    assert _lib.tune_variants(3, 4) == 4
     with pytest.raises(ValueError):
         _lib.tune_variants(9, 0)
