@@ -1314,9 +1314,16 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
     hipStream_t s = stream();
     DevBox const db = make_devbox(*box);
     unsigned const blocks = (n + kBlock - 1) / kBlock;
+    // The bond passes (q_lm) take the points in index order: a tile's bonds
+    // are then one range of the NeighborList, staged in LDS with coalesced
+    // loads, which beats any processing order with per-thread bond streams
+    // even for points stored in random order.  The averaged q_lm gathers whole
+    // neighbour rows; it takes the cached spatial order where the points are
+    // not stored coherently.
     const uint32_t* sorder = nullptr;
-    if (int rc = spatial_order(*box, d_points, n, &sorder))
-        return rc;
+    if (average)
+        if (int rc = spatial_order(*box, d_points, n, &sorder))
+            return rc;
     // Partial sums: blocks x (2l + 1) complex per l for the base and for the
     // averaged q_lm (every l live at once: the fused pass writes all of them,
     // and all system sums are reduced and copied back together), then
@@ -1355,7 +1362,7 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
             mo += 2 * l + 1;
             po += size_t(blocks) * (2 * l + 1);
         }
-        if (int rc = launch_qlm_even_multi(lmax, db, d_points, n, nl, weighted, even_mask, o, d_ql, n_l, sorder))
+        if (int rc = launch_qlm_even_multi(lmax, db, d_points, n, nl, weighted, even_mask, o, d_ql, n_l, nullptr))
             return rc;
     }
     size_t moff = 0, poff = 0;
@@ -1369,7 +1376,7 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         float2* qlm_l = reinterpret_cast<float2*>(d_qlm) + size_t(n) * moff;
         if (n && !fuse)
         {
-            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part, sorder);
+            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part, nullptr);
             FA_HIP(hipGetLastError());
         }
         float2* src = qlm_l;
@@ -1584,13 +1591,15 @@ int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const f
         return FA_OK;
     fa::DevBox const db = fa::make_devbox(*box);
     unsigned const blocks = (n + kBlock - 1) / kBlock;
+    // the bond pass in index order (bonds staged per tile); the spatial order
+    // is prepared for the fallback bond dots of the classifier's decision step
     const uint32_t* sorder = nullptr;
     if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
         return rc;
-    uint32_t const cap = stage_capacity(nl, n, sorder);
+    uint32_t const cap = stage_capacity(nl, n, nullptr);
     FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, d_points, n,
                nl->neighbors, nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
-               reinterpret_cast<float2*>(d_qlm6), d_ql6, sorder, cap);
+               reinterpret_cast<float2*>(d_qlm6), d_ql6, nullptr, cap);
     FA_HIP(hipGetLastError());
     return FA_OK;
 }

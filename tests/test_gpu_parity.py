@@ -527,9 +527,11 @@ def test_stress_config_shapes(oracle):
 
 
 def test_spatially_ordered_paths(oracle):
-    """Above 65536 points the gather kernels (q_lm, averaged q_lm, SolidLiquid
-    dots, learned features) run in a cached spatial order; per-particle results
-    must not change."""
+    """Above 65536 points the row-gather kernels (averaged q_lm, SolidLiquid
+    dots, the classifier's fallback) run in a cached spatial order when the
+    points are not stored coherently, while the bond passes (q_lm, fused q_lm)
+    stage each tile's bonds in index order; per-particle results must not
+    change."""
     box, pts, _ = freud_amd.data.two_phase(27, seed=8)  # 157,464 points, liquid half in random order
     assert len(pts) >= 65536
     nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
