@@ -441,8 +441,10 @@ struct Grid
 int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, const int dims[3]);
 
 // A processing order of the points that follows space (cells of about four
-// points, counting-sorted), for kernels that gather neighbour data: threads
-// that run together then touch nearby memory.  It is only built when a probe
+// points, counting-sorted), for the averaged q_lm, which gathers whole q_lm
+// rows of the neighbours: threads that run together then reuse each other's
+// rows in cache.  (The bond passes run in index order instead, where a tile's
+// bonds are one range of the NeighborList.)  It is only built when a probe
 // finds the caller's order scattered (random order); a coherent caller order
 // is kept (*order = nullptr, identity), as it also keeps the per-point outputs
 // and bond segments streaming.  Any order gives the same per-point results, so
@@ -450,9 +452,6 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
 // only.  Identity below kSpatialOrderMin points.
 constexpr uint32_t kSpatialOrderMin = 65536;
 int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const uint32_t** order);
-// The last order spatial_order produced if it holds n points (a valid
-// permutation of 0..n-1 whatever the points), else nullptr.
-const uint32_t* cached_spatial_order(uint32_t n);
 
 // Exclusive scan of n uint32 counts into n+1 offsets (offsets[n] = total);
 // returns the 64-bit total through *total (host) when total != nullptr.

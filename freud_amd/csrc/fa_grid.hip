@@ -183,7 +183,6 @@ struct OrderCache
 constexpr int kOrderEntries = 4;
 OrderCache g_order[kOrderEntries];
 uint64_t g_order_clock = 0;
-int g_order_last = -1;
 
 } // namespace
 
@@ -201,7 +200,6 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
         if (c.pts == d_points && c.n == n && std::memcmp(&c.box, &box, sizeof(box)) == 0)
         {
             c.stamp = ++g_order_clock;
-            g_order_last = e;
             *out = c.use ? c.order : nullptr;
             return FA_OK;
         }
@@ -233,7 +231,6 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
             c.box = box;
             c.use = false;
             c.stamp = ++g_order_clock;
-            g_order_last = victim;
             return FA_OK;
         }
     }
@@ -276,18 +273,8 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
     c.box = box;
     c.use = true;
     c.stamp = ++g_order_clock;
-    g_order_last = victim;
     *out = c.order;
     return FA_OK;
-}
-
-// The order that spatial_order returned last, if it covers n points.
-const uint32_t* cached_spatial_order(uint32_t n)
-{
-    if (g_order_last < 0)
-        return nullptr;
-    OrderCache const& c = g_order[g_order_last];
-    return (c.pts && c.use && c.n == n && n >= kSpatialOrderMin) ? c.order : nullptr;
 }
 
 int batch_grid_build(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, void* tmp,

@@ -343,14 +343,12 @@ __global__ __launch_bounds__(kBlock) void qlm_kernel(fa::DevBox b, const float* 
                                                      const float* __restrict__ wts, const uint32_t* __restrict__ offsets,
                                                      int weighted, int want_sys, float2* __restrict__ qlm,
                                                      float* __restrict__ ql, int ql_stride, int ql_col,
-                                                     float2* __restrict__ sys_partial,
-                                                     const uint32_t* __restrict__ order, uint32_t stage_cap)
+                                                     float2* __restrict__ sys_partial, uint32_t stage_cap)
 {
     constexpr int NM = 2 * L + 1;
     uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
-    uint32_t const t0 = tile0 + threadIdx.x;
-    bool const valid = t0 < n;
-    uint32_t const i = (order && valid) ? order[t0] : t0;
+    uint32_t const i = tile0 + threadIdx.x;
+    bool const valid = i < n;
     BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, fa_stage_lds,
                                     reinterpret_cast<float*>(fa_stage_lds + stage_cap), stage_cap);
     float q[2 * NM];
@@ -432,14 +430,13 @@ __global__ __launch_bounds__(kBlock) void qlm_even_multi_kernel(fa::DevBox b, co
                                                                 const uint32_t* __restrict__ nbr,
                                                                 const float* __restrict__ dist,
                                                                 const float* __restrict__ wts,
-                                                                const uint32_t* __restrict__ offsets, uint32_t mask, MultiLOut o, float* __restrict__ ql,
-                                                                int ql_stride, const uint32_t* __restrict__ order,
+                                                                const uint32_t* __restrict__ offsets, uint32_t mask,
+                                                                MultiLOut o, float* __restrict__ ql, int ql_stride,
                                                                 uint32_t stage_cap)
 {
     uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
-    uint32_t const t0 = tile0 + threadIdx.x;
-    bool const valid = t0 < n;
-    uint32_t const i = (order && valid) ? order[t0] : t0;
+    uint32_t const i = tile0 + threadIdx.x;
+    bool const valid = i < n;
     BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, fa_stage_lds,
                                     reinterpret_cast<float*>(fa_stage_lds + stage_cap), stage_cap);
     float2 acc[even_base(LMAX + 2)];
@@ -821,7 +818,7 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
                                                              const float* __restrict__ coef10,
                                                              const float* __restrict__ coef12, float* __restrict__ feat,
                                                              float2* __restrict__ qlm6, float* __restrict__ ql6,
-                                                             const uint32_t* __restrict__ order, uint32_t stage_cap)
+                                                             uint32_t stage_cap)
 {
     constexpr int LM = 12;
     uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
@@ -830,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
                                     reinterpret_cast<float*>(fa_stage_lds + stage_cap), stage_cap);
     if (t0 >= n)
         return;
-    uint32_t const i = order ? order[t0] : t0;
+    uint32_t const i = t0;
     float2 acc[feat_base(14)];
 #pragma unroll
     for (int t = 0; t < feat_base(14); ++t)
@@ -942,13 +939,11 @@ __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, 
                                                              const float2* __restrict__ qlm,
                                                              const float* __restrict__ ql, int normalize_q,
                                                              float q_threshold, float* __restrict__ ql_ij,
-                                                             uint32_t* __restrict__ num_conn,
-                                                             const uint32_t* __restrict__ order)
+                                                             uint32_t* __restrict__ num_conn)
 {
-    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t0 >= n)
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
         return;
-    uint32_t const i = order ? order[t0] : t0;
     num_conn[i] = l == 6 ? solid_bond_count<13>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij)
                          : solid_bond_count<0>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij);
 }
@@ -964,14 +959,12 @@ __global__ __launch_bounds__(kBlock) void sl_decide_kernel(uint32_t n, const flo
                                                            const float* __restrict__ ql6, int normalize_q,
                                                            float q_threshold, uint32_t solid_threshold,
                                                            uint8_t* __restrict__ solid, uint8_t* __restrict__ fallback,
-                                                           uint32_t* __restrict__ n_fallback,
-                                                           const uint32_t* __restrict__ order)
+                                                           uint32_t* __restrict__ n_fallback)
 {
-    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool fb = false;
-    if (t0 < n)
+    if (i < n)
     {
-        uint32_t const i = order ? order[t0] : t0;
         float const y = logits[i];
         fb = !(fabsf(y) >= margin);
         uint8_t lab;
@@ -1143,12 +1136,11 @@ __global__ __launch_bounds__(kBlock) void hexatic_kernel(fa::DevBox b, const flo
                                                          const uint32_t* __restrict__ nbr,
                                                          const float* __restrict__ wts,
                                                          const uint32_t* __restrict__ offsets, float k, int weighted,
-                                                         float2* __restrict__ psi, const uint32_t* __restrict__ order)
+                                                         float2* __restrict__ psi)
 {
-    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (t0 >= n)
+    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n)
         return;
-    uint32_t const i = order ? order[t0] : t0;
     float re = 0.0f, im = 0.0f, total = 0.0f;
     for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e)
     {
@@ -1207,11 +1199,10 @@ float reduce_wigner3j_host(const std::complex<float>* src, int l)
 }
 
 // Bond staging capacity of a kBlock-point tile: 1.3 x the mean bonds per point,
-// rounded to 64, within [512, kStage]; 0 when the points are visited in a
-// processing order (a tile's bonds are then not one range).
-uint32_t stage_capacity(const fa_nlist* nl, uint32_t n, const uint32_t* order)
+// rounded to 64, within [512, kStage].
+uint32_t stage_capacity(const fa_nlist* nl, uint32_t n)
 {
-    if (order || n == 0)
+    if (n == 0)
         return 0;
     double const want = std::ceil(double(nl->num_bonds) / double(n) * kBlock * 1.3);
     uint32_t const cap = uint32_t(std::min(std::max(want, 512.0), double(kStage)));
@@ -1220,12 +1211,12 @@ uint32_t stage_capacity(const fa_nlist* nl, uint32_t n, const uint32_t* order)
 
 template<int L>
 void launch_qlm_l(const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted, int want_sys,
-                  float2* qlm, float* ql, int stride, int col, float2* part, const uint32_t* order)
+                  float2* qlm, float* ql, int stride, int col, float2* part)
 {
     unsigned const blocks = (n + kBlock - 1) / kBlock;
-    uint32_t const cap = stage_capacity(nl, n, order);
+    uint32_t const cap = stage_capacity(nl, n);
     FA_KLAUNCH(qlm_kernel<L>, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, pts, n, nl->neighbors,
-               nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part, order, cap);
+               nl->distances, nl->weights, nl->offsets, weighted, want_sys, qlm, ql, stride, col, part, cap);
 }
 
 template<int L>
@@ -1239,48 +1230,45 @@ void launch_ave_l(uint32_t n, const fa_nlist* nl, const float2* qlm, float2* qlm
 
 template<int L>
 int dispatch_l(int l, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl, int weighted,
-               int want_sys, float2* qlm, float* ql, int stride, int col, float2* part, const uint32_t* order)
+               int want_sys, float2* qlm, float* ql, int stride, int col, float2* part)
 {
     if (l == L)
     {
-        launch_qlm_l<L>(db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part, order);
+        launch_qlm_l<L>(db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part);
         return 1;
     }
     if constexpr (L < kLMax)
-        return dispatch_l<L + 1>(l, db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part, order);
+        return dispatch_l<L + 1>(l, db, pts, n, nl, weighted, want_sys, qlm, ql, stride, col, part);
     return 0;
 }
 
 template<int LMAX>
 int launch_qlm_even_multi(int lmax, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl,
-                          int weighted, uint32_t mask, const MultiLOut& o, float* ql, int stride,
-                          const uint32_t* order)
+                          int weighted, uint32_t mask, const MultiLOut& o, float* ql, int stride)
 {
     if (lmax == LMAX)
     {
         unsigned const blocks = (n + kBlock - 1) / kBlock;
-        uint32_t const cap = stage_capacity(nl, n, order);
+        uint32_t const cap = stage_capacity(nl, n);
         if (weighted)
             FA_KLAUNCH((qlm_even_multi_kernel<LMAX, true>), dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(),
-                       db, pts, n, nl->neighbors, nl->distances, nl->weights, nl->offsets, mask, o, ql, stride, order,
-                       cap);
+                       db, pts, n, nl->neighbors, nl->distances, nl->weights, nl->offsets, mask, o, ql, stride, cap);
         else
             FA_KLAUNCH((qlm_even_multi_kernel<LMAX, false>), dim3(blocks), dim3(kBlock), size_t(cap) * 8,
                        fa::stream(), db, pts, n, nl->neighbors, nl->distances, nl->weights, nl->offsets, mask, o, ql,
-                       stride, order, cap);
+                       stride, cap);
         FA_HIP(hipGetLastError());
         return FA_OK;
     }
     if constexpr (LMAX + 2 <= kFuseLMax)
-        return launch_qlm_even_multi<LMAX + 2>(lmax, db, pts, n, nl, weighted, mask, o, ql, stride, order);
+        return launch_qlm_even_multi<LMAX + 2>(lmax, db, pts, n, nl, weighted, mask, o, ql, stride);
     return fa::fail(FA_ERR_INVALID, "fused q_lm pass: l out of range");
 }
 
 int launch_qlm_even_multi(int lmax, const fa::DevBox& db, const float* pts, uint32_t n, const fa_nlist* nl,
-                          int weighted, uint32_t mask, const MultiLOut& o, float* ql, int stride,
-                          const uint32_t* order)
+                          int weighted, uint32_t mask, const MultiLOut& o, float* ql, int stride)
 {
-    return launch_qlm_even_multi<2>(lmax, db, pts, n, nl, weighted, mask, o, ql, stride, order);
+    return launch_qlm_even_multi<2>(lmax, db, pts, n, nl, weighted, mask, o, ql, stride);
 }
 
 template<int L>
@@ -1362,7 +1350,7 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
             mo += 2 * l + 1;
             po += size_t(blocks) * (2 * l + 1);
         }
-        if (int rc = launch_qlm_even_multi(lmax, db, d_points, n, nl, weighted, even_mask, o, d_ql, n_l, nullptr))
+        if (int rc = launch_qlm_even_multi(lmax, db, d_points, n, nl, weighted, even_mask, o, d_ql, n_l))
             return rc;
     }
     size_t moff = 0, poff = 0;
@@ -1376,7 +1364,7 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
         float2* qlm_l = reinterpret_cast<float2*>(d_qlm) + size_t(n) * moff;
         if (n && !fuse)
         {
-            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part, nullptr);
+            dispatch_l<0>(l, db, d_points, n, nl, weighted, !average, qlm_l, d_ql, n_l, k, part);
             FA_HIP(hipGetLastError());
         }
         float2* src = qlm_l;
@@ -1591,15 +1579,10 @@ int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const f
         return FA_OK;
     fa::DevBox const db = fa::make_devbox(*box);
     unsigned const blocks = (n + kBlock - 1) / kBlock;
-    // the bond pass in index order (bonds staged per tile); the spatial order
-    // is prepared for the fallback bond dots of the classifier's decision step
-    const uint32_t* sorder = nullptr;
-    if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
-        return rc;
-    uint32_t const cap = stage_capacity(nl, n, nullptr);
+    uint32_t const cap = stage_capacity(nl, n);
     FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, d_points, n,
                nl->neighbors, nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
-               reinterpret_cast<float2*>(d_qlm6), d_ql6, nullptr, cap);
+               reinterpret_cast<float2*>(d_qlm6), d_ql6, cap);
     FA_HIP(hipGetLastError());
     return FA_OK;
 }
@@ -1620,7 +1603,7 @@ int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float ma
     if (n)
         FA_KLAUNCH(sl_decide_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), n, d_logits,
                    margin, nl->neighbors, nl->offsets, reinterpret_cast<const float2*>(d_qlm6), d_ql6, normalize_q,
-                   q_threshold, solid_threshold, d_solid, d_fallback, cnt, fa::cached_spatial_order(n));
+                   q_threshold, solid_threshold, d_solid, d_fallback, cnt);
     FA_HIP(hipGetLastError());
     if (n_fallback)
     {
@@ -1694,13 +1677,10 @@ int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const f
         return rc;
     hipStream_t s = fa::stream();
     unsigned const blocks = (n + kBlock - 1) / kBlock;
-    const uint32_t* sorder = nullptr;
-    if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
-        return rc;
     if (n)
         FA_KLAUNCH(solid_bonds_kernel, dim3(blocks), dim3(kBlock), 0, s, n, int(l), nl->neighbors, nl->offsets,
                    reinterpret_cast<const float2*>(d_qlm), d_ql, normalize_q, q_threshold, d_ql_ij,
-                   d_num_connections, sorder);
+                   d_num_connections);
     FA_HIP(hipGetLastError());
     void* keep = nullptr;
     if (int rc = fa::scratch(8, std::max<uint64_t>(nl->num_bonds, 1) + 64, &keep))
@@ -1724,11 +1704,8 @@ int fa_hexatic(const fa_box* box, const float* d_points, uint32_t n, const fa_nl
     if (n == 0)
         return FA_OK;
     fa::DevBox const db = fa::make_devbox(*box);
-    const uint32_t* sorder = nullptr;
-    if (int rc = fa::spatial_order(*box, d_points, n, &sorder))
-        return rc;
     FA_KLAUNCH(hexatic_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), db, d_points, n,
-               nl->neighbors, nl->weights, nl->offsets, float(k), weighted, reinterpret_cast<float2*>(d_psi), sorder);
+               nl->neighbors, nl->weights, nl->offsets, float(k), weighted, reinterpret_cast<float2*>(d_psi));
     FA_HIP(hipGetLastError());
     return FA_OK;
 }
