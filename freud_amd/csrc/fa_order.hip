@@ -884,11 +884,22 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
 // Solid-like bond count of particle i; writes each bond's q_ij when ql_ij is
 // not null.  NM = 2l + 1 known at compile time keeps q_lm(i) in registers and
 // issues the 2l + 1 loads of a neighbour's q_lm together; NM = 0: runtime l.
+// With `tile` staged, bond e's neighbour index comes from tile.sj[e - base]
+// and its ql_ij goes to tile.so[e - base] (the caller stores the tile's values
+// to ql_ij with coalesced writes).
+struct SolidTile
+{
+    const uint32_t* sj = nullptr;
+    float* so = nullptr;
+    uint32_t base = 0;
+    bool staged = false;
+};
+
 template<int NM>
 __device__ inline uint32_t solid_bond_count(uint32_t i, int l, const uint32_t* __restrict__ nbr,
                                             const uint32_t* __restrict__ offsets, const float2* __restrict__ qlm,
                                             const float* __restrict__ ql, int normalize_q, float q_threshold,
-                                            float* __restrict__ ql_ij)
+                                            float* __restrict__ ql_ij, const SolidTile& tile = SolidTile())
 {
     int const nm = NM ? NM : 2 * l + 1;
     float const nf = float(4.0 * M_PI / nm);
@@ -904,7 +915,7 @@ __device__ inline uint32_t solid_bond_count(uint32_t i, int l, const uint32_t* _
     }
     for (uint32_t e = e0; e < e1; ++e)
     {
-        uint32_t const j = nbr[2 * e + 1];
+        uint32_t const j = tile.staged ? tile.sj[e - tile.base] : nbr[2 * e + 1];
         const float2* qj = qlm + uint64_t(j) * nm;
         float re = 0.0f;
         if (NM)
@@ -927,7 +938,9 @@ __device__ inline uint32_t solid_bond_count(uint32_t i, int l, const uint32_t* _
         }
         if (normalize_q)
             re *= nf / (ql[i] * ql[j]);
-        if (ql_ij)
+        if (tile.staged)
+            tile.so[e - tile.base] = re;
+        else if (ql_ij)
             ql_ij[e] = re;
         cnt += (re > q_threshold) ? 1u : 0u;
     }
@@ -939,13 +952,41 @@ __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, 
                                                              const float2* __restrict__ qlm,
                                                              const float* __restrict__ ql, int normalize_q,
                                                              float q_threshold, float* __restrict__ ql_ij,
-                                                             uint32_t* __restrict__ num_conn)
+                                                             uint32_t* __restrict__ num_conn, uint32_t stage_cap)
 {
-    uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    num_conn[i] = l == 6 ? solid_bond_count<13>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij)
-                         : solid_bond_count<0>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij);
+    // A tile's bonds are one range of the NeighborList: when it fits in
+    // stage_cap bonds, the neighbour indices are read into LDS and the ql_ij
+    // values written back from LDS, both with coalesced accesses.
+    uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    uint32_t const i = tile0 + threadIdx.x;
+    uint32_t const nloc = min(blockDim.x, n - tile0);
+    SolidTile tile;
+    uint32_t b0 = 0, nb = 0;
+    if (stage_cap && ql_ij)
+    {
+        b0 = offsets[tile0];
+        nb = offsets[tile0 + nloc] - b0;
+        if (nb <= stage_cap)
+        {
+            tile.sj = fa_stage_lds;
+            tile.so = reinterpret_cast<float*>(fa_stage_lds + stage_cap);
+            tile.base = b0;
+            tile.staged = true;
+            for (uint32_t t = threadIdx.x; t < nb; t += blockDim.x)
+                fa_stage_lds[t] = reinterpret_cast<const uint2*>(nbr)[b0 + t].y;
+            __syncthreads();
+        }
+    }
+    if (i < n)
+        num_conn[i] = l == 6
+                          ? solid_bond_count<13>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij, tile)
+                          : solid_bond_count<0>(i, l, nbr, offsets, qlm, ql, normalize_q, q_threshold, ql_ij, tile);
+    if (tile.staged)
+    {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nb; t += blockDim.x)
+            ql_ij[b0 + t] = tile.so[t];
+    }
 }
 
 // Learned SolidLiquid labels: a particle whose classifier logit is at least
@@ -1678,9 +1719,12 @@ int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const f
     hipStream_t s = fa::stream();
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     if (n)
-        FA_KLAUNCH(solid_bonds_kernel, dim3(blocks), dim3(kBlock), 0, s, n, int(l), nl->neighbors, nl->offsets,
-                   reinterpret_cast<const float2*>(d_qlm), d_ql, normalize_q, q_threshold, d_ql_ij,
-                   d_num_connections);
+    {
+        uint32_t const cap = stage_capacity(nl, n);
+        FA_KLAUNCH(solid_bonds_kernel, dim3(blocks), dim3(kBlock), size_t(cap) * 8, s, n, int(l), nl->neighbors,
+                   nl->offsets, reinterpret_cast<const float2*>(d_qlm), d_ql, normalize_q, q_threshold, d_ql_ij,
+                   d_num_connections, cap);
+    }
     FA_HIP(hipGetLastError());
     void* keep = nullptr;
     if (int rc = fa::scratch(8, std::max<uint64_t>(nl->num_bonds, 1) + 64, &keep))
