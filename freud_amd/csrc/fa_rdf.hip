@@ -310,7 +310,12 @@ struct LutBins
     __device__ __forceinline__ void add2(float x0, bool ok0, float x1, bool ok1, uint32_t w) const
     {
         bool const f0 = ok0 && x0 >= x_fast, f1 = ok1 && x1 >= x_fast;
-        uint32_t const a0 = fast_slot(x0), a1 = fast_slot(x1);
+        // fast_slot of both values, the two table reads of each issued together
+        uint32_t const e0 = *(lds_cu32*) (uintptr_t) (lut + 4u * min(uint32_t(x0 * inv_delta), kLut - 1u));
+        uint32_t const e1 = *(lds_cu32*) (uintptr_t) (lut + 4u * min(uint32_t(x1 * inv_delta), kLut - 1u));
+        float const t0v = *(lds_cf32*) (uintptr_t) e0;
+        float const t1v = *(lds_cf32*) (uintptr_t) e1;
+        uint32_t const a0 = e0 + (x0 >= t0v ? 4u : 0u) + hist_c, a1 = e1 + (x1 >= t1v ? 4u : 0u) + hist_c;
         asm volatile("" ::"v"(a0), "v"(a1)); // both lookups ahead of the masked atomics
         if (f0)
             atomic(a0, w);
