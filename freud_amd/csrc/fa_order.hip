@@ -179,15 +179,12 @@ template<int L> __device__ inline void accumulate_ylm(float ct, float st, float 
     });
 }
 
-// Steinhardt.cc:120-222: q_lm(i) = sum_b w Y_lm / sum_b w and
-// q_l(i) = sqrt(4 pi / (2l+1) sum_m |q_lm|^2); also the block partial sums of
-// q_lm(i) / N for the system-wide order.
-// Where a kernel reads a bond's neighbour index and distance.  When a block's
-// points are consecutive (no processing order), their bonds form one
-// contiguous range of the NeighborList, which the block copies into LDS with
-// coalesced loads; each thread then walks its own bonds in LDS instead of
-// keeping a private strided stream through L2 (thousands of such streams per
-// CU thrash the cache).  Ranges longer than kStage bonds read global memory.
+// Where a kernel reads a bond's neighbour index and distance.  A block's
+// points are consecutive, so their bonds form one contiguous range of the
+// NeighborList, which the block copies into LDS with coalesced loads; each
+// thread then walks its own bonds in LDS instead of keeping a private strided
+// stream through L2 (thousands of such streams per CU thrash the cache).
+// Ranges longer than the staging capacity read global memory.
 constexpr uint32_t kStage = 4096;
 struct BondSrc
 {
@@ -264,10 +261,11 @@ __device__ inline void for_each_bond(const fa::DevBox& b, const float* __restric
 
 // Dynamic LDS of the kernels over one NeighborList that stage bonds: cap
 // neighbour indices followed by cap distances, cap sized by the host from the
-// mean bond count (stage_capacity); 0 bytes when the points are visited in a
-// processing order, where a block's bonds are not one range.
+// mean bond count (stage_capacity).
 extern __shared__ uint32_t fa_stage_lds[];
 
+// Steinhardt.cc:120-222: q_lm(i) = sum_b w Y_lm / sum_b w into q (freud's m
+// order, interleaved re/im), returns q_l(i) = sqrt(4 pi / (2l+1) sum_m |q_lm|^2).
 template<int L>
 __device__ inline float qlm_point(const fa::DevBox& b, const float* __restrict__ pts, uint32_t i, const BondSrc& src,
                                   const float* __restrict__ wts, const uint32_t* __restrict__ offsets, int weighted,
