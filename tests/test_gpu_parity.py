@@ -329,6 +329,29 @@ def test_steinhardt_weighted_and_fused_l(oracle):
                 assert st.order[k] == one.order
 
 
+def test_steinhardt_particles_without_bonds(oracle):
+    """Particles without neighbours: q_lm = 0 / 0 (Steinhardt.cc:208) is NaN,
+    so are their q_l and the system order that sums over all particles; the
+    others match the oracle.  Single-l and fused passes, plain and averaged."""
+    rng = np.random.default_rng(21)
+    box = Box.cube(20.0)
+    pts = ((rng.random((600, 3)) - 0.5) * 20).astype(np.float32)
+    nl = freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.2, exclude_ii=True)).toNeighborList()
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.2, exclude_ii=True), nthreads=8)
+    lonely = np.bincount(ref_nl["ij"][:, 0], minlength=len(pts)) == 0
+    assert lonely.any() and not lonely.all()
+    for ls in ([6], [4, 6, 8]):
+        for kw in (dict(), dict(average=True)):
+            st = freud_amd.order.Steinhardt(ls, **kw).compute((box, pts), neighbors=nl)
+            ref = oracle.steinhardt(_bt(box), pts, ref_nl, ls, nthreads=8, **kw)
+            po = np.asarray(st.particle_order).reshape(len(pts), -1)
+            rpo = np.asarray(ref["particle_order"]).reshape(len(pts), -1)
+            npt.assert_array_equal(np.isnan(po), np.isnan(rpo))
+            assert np.isnan(po[lonely]).all()
+            npt.assert_allclose(po[~np.isnan(rpo)], rpo[~np.isnan(rpo)], atol=1e-5, rtol=0)
+            assert np.isnan(np.ravel(st.order)).all() and np.isnan(np.ravel(ref["order"])).all()
+
+
 def test_steinhardt_gc_golden():
     # tests/validation/test_steinhardt_average.py:82-109 of the reference
     fr = open_gsd(os.path.join(GOLDEN, "steinhardt_average", "Test_Configuration.gsd"))[-1]
