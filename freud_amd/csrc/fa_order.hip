@@ -222,6 +222,24 @@ __device__ inline BondSrc stage_bonds(uint32_t p0, uint32_t nloc, const uint32_t
     return src;
 }
 
+// stage_bonds for kernels that read neighbour indices only.
+__device__ inline BondSrc stage_neighbors(uint32_t p0, uint32_t nloc, const uint32_t* __restrict__ offsets,
+                                          const uint32_t* __restrict__ nbr, uint32_t* s_j, uint32_t cap)
+{
+    BondSrc src{nbr, nullptr, s_j, nullptr, 0u, false};
+    if (cap == 0 || nloc == 0)
+        return src;
+    uint32_t const b0 = offsets[p0], b1 = offsets[p0 + nloc];
+    if (b1 - b0 > cap)
+        return src;
+    for (uint32_t t = threadIdx.x; t < b1 - b0; t += blockDim.x)
+        s_j[t] = reinterpret_cast<const uint2*>(nbr)[b0 + t].y;
+    __syncthreads();
+    src.base = b0;
+    src.staged = true;
+    return src;
+}
+
 // Calls f(ct, st, cp, sp, w) for the bonds e0 <= e < e1 of point i, in order.
 // The neighbour position, distance and weight of bond e + 1 and the neighbour
 // index of bond e + 2 are loaded before bond e is evaluated, so that the
@@ -542,12 +560,16 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
                                                          const float2* __restrict__ qlm, float2* __restrict__ qlmave,
                                                          float* __restrict__ qlave, int ql_stride, int ql_col,
                                                          float2* __restrict__ sys_partial,
-                                                         const uint32_t* __restrict__ order)
+                                                         const uint32_t* __restrict__ order, uint32_t stage_cap)
 {
     constexpr int NM = 2 * L + 1;
-    uint32_t const t0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    uint32_t const t0 = tile0 + threadIdx.x;
     bool const valid = t0 < n;
     uint32_t const i = (order && valid) ? order[t0] : t0;
+    // in index order a tile's neighbour indices are staged in LDS
+    BondSrc const src =
+        stage_neighbors(tile0, min(blockDim.x, n - tile0), offsets, nbr, fa_stage_lds, order ? 0u : stage_cap);
     float2 acc[NM];
 #pragma unroll
     for (int t = 0; t < NM; ++t)
@@ -557,16 +579,16 @@ __global__ __launch_bounds__(kBlock) void qlm_ave_kernel(uint32_t n, const uint3
         uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
         // the neighbour index of bond e + 1 is loaded before bond e's row is
         // gathered and added
-        uint32_t jn = e0 < e1 ? nbr[2 * e0 + 1] : 0u;
+        uint32_t jn = e0 < e1 ? src.j(e0) : 0u;
         for (uint32_t e = e0; e < e1; ++e)
         {
-            const float2* src = qlm + uint64_t(jn) * NM;
+            const float2* row = qlm + uint64_t(jn) * NM;
             if (e + 1 < e1)
-                jn = nbr[2 * (e + 1) + 1];
+                jn = src.j(e + 1);
 #pragma unroll
             for (int t = 0; t < NM; ++t)
             {
-                float2 const v = src[t];
+                float2 const v = row[t];
                 acc[t].x += v.x;
                 acc[t].y += v.y;
             }
@@ -1263,8 +1285,9 @@ void launch_ave_l(uint32_t n, const fa_nlist* nl, const float2* qlm, float2* qlm
                   float2* part, const uint32_t* order)
 {
     unsigned const blocks = (n + kBlock - 1) / kBlock;
-    FA_KLAUNCH(qlm_ave_kernel<L>, dim3(blocks), dim3(kBlock), 0, fa::stream(), n, nl->neighbors, nl->offsets,
-               qlm, qlmave, qlave, stride, col, part, order);
+    uint32_t const cap = order ? 0u : stage_capacity(nl, n);
+    FA_KLAUNCH(qlm_ave_kernel<L>, dim3(blocks), dim3(kBlock), size_t(cap) * 4, fa::stream(), n, nl->neighbors,
+               nl->offsets, qlm, qlmave, qlave, stride, col, part, order, cap);
 }
 
 template<int L>
