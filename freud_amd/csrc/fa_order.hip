@@ -1083,6 +1083,24 @@ __global__ void uf_init_kernel(uint32_t n, uint32_t* parent, uint32_t* size)
     }
 }
 
+// Before the union pass: every point hooks to the smallest of itself and its
+// (kept) neighbours.  Each link is a bond and points to a smaller index, so
+// this is a valid forest whose roots are the smallest members of their trees;
+// the union pass then mostly finds both ends of a bond in one tree already
+// instead of merging single points one CAS at a time.
+__global__ void uf_hook_min_kernel(uint32_t n, const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ offsets,
+                                   const uint8_t* __restrict__ keep, uint32_t* __restrict__ parent)
+{
+    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    uint32_t m = i;
+    for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e)
+        if (!keep || keep[e])
+            m = min(m, nbr[2 * e + 1]);
+    parent[i] = m;
+}
+
 __global__ void uf_union_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ keep,
                                 uint32_t* parent)
 {
@@ -1166,10 +1184,19 @@ __global__ void uf_keys_kernel(uint32_t n, const uint32_t* __restrict__ parent, 
                                uint32_t* __restrict__ counter, uint64_t* __restrict__ keys)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || parent[i] != i)
+    bool const is_root = i < n && parent[i] == i;
+    // one counter atomic per wave (most points of a liquid are roots)
+    uint64_t const roots = __ballot(is_root);
+    if (roots == 0)
         return;
-    uint32_t const slot = atomicAdd(counter, 1u);
-    keys[slot] = (uint64_t(n - size[i]) << 32) | i;
+    uint32_t const lane = threadIdx.x & 63u;
+    uint32_t const leader = uint32_t(__ffsll(static_cast<unsigned long long>(roots)) - 1);
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(counter, uint32_t(__popcll(roots)));
+    base = uint32_t(__shfl(int(base), int(leader)));
+    if (is_root)
+        keys[base + uint32_t(__popcll(roots & ((uint64_t(1) << lane) - 1u)))] = (uint64_t(n - size[i]) << 32) | i;
 }
 
 __global__ void uf_rank_kernel(uint32_t n, uint32_t ncl, const uint64_t* __restrict__ keys,
@@ -1698,6 +1725,9 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
     uint64_t* keys = reinterpret_cast<uint64_t*>((reinterpret_cast<uintptr_t>(counter + 4) + 15) & ~uintptr_t(15));
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     FA_KLAUNCH(uf_init_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size);
+    if (nl->num_bonds && nl->num_query_points == n)
+        FA_KLAUNCH(uf_hook_min_kernel, dim3(blocks), dim3(kBlock), 0, s, n, nl->neighbors, nl->offsets, d_keep,
+                   parent);
     if (nl->num_bonds)
         FA_KLAUNCH(uf_union_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            nl->num_bonds, nl->neighbors, d_keep, parent);
