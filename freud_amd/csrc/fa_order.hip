@@ -1088,17 +1088,41 @@ __global__ void uf_init_kernel(uint32_t n, uint32_t* parent, uint32_t* size)
 // this is a valid forest whose roots are the smallest members of their trees;
 // the union pass then mostly finds both ends of a bond in one tree already
 // instead of merging single points one CAS at a time.
-__global__ void uf_hook_min_kernel(uint32_t n, const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ offsets,
-                                   const uint8_t* __restrict__ keep, uint32_t* __restrict__ parent)
+// A tile's neighbour indices and keep flags are read into LDS with coalesced
+// loads when they fit in stage_cap bonds.
+__global__ __launch_bounds__(kBlock) void uf_hook_min_kernel(uint32_t n, const uint32_t* __restrict__ nbr,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const uint8_t* __restrict__ keep,
+                                                             uint32_t* __restrict__ parent, uint32_t stage_cap)
 {
-    uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    uint32_t m = i;
-    for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e)
-        if (!keep || keep[e])
-            m = min(m, nbr[2 * e + 1]);
-    parent[i] = m;
+    uint32_t const tile0 = blockIdx.x * blockDim.x;
+    uint32_t const i = tile0 + threadIdx.x;
+    uint32_t const nloc = min(blockDim.x, n - tile0);
+    uint32_t* const sj = fa_stage_lds;
+    uint8_t* const sk = reinterpret_cast<uint8_t*>(fa_stage_lds + stage_cap);
+    uint32_t const b0 = offsets[tile0], nb = offsets[tile0 + nloc] - b0;
+    bool const staged = stage_cap && nb <= stage_cap;
+    if (staged)
+    {
+        for (uint32_t t = threadIdx.x; t < nb; t += blockDim.x)
+        {
+            sj[t] = reinterpret_cast<const uint2*>(nbr)[b0 + t].y;
+            if (keep)
+                sk[t] = keep[b0 + t];
+        }
+        __syncthreads();
+    }
+    if (i < n)
+    {
+        uint32_t m = i;
+        for (uint32_t e = offsets[i]; e < offsets[i + 1]; ++e)
+        {
+            bool const k = !keep || (staged ? sk[e - b0] : keep[e]);
+            if (k)
+                m = min(m, staged ? sj[e - b0] : nbr[2 * e + 1]);
+        }
+        parent[i] = m;
+    }
 }
 
 __global__ void uf_union_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, const uint8_t* __restrict__ keep,
@@ -1726,8 +1750,11 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     FA_KLAUNCH(uf_init_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size);
     if (nl->num_bonds && nl->num_query_points == n)
-        FA_KLAUNCH(uf_hook_min_kernel, dim3(blocks), dim3(kBlock), 0, s, n, nl->neighbors, nl->offsets, d_keep,
-                   parent);
+    {
+        uint32_t const cap = stage_capacity(nl, n);
+        FA_KLAUNCH(uf_hook_min_kernel, dim3(blocks), dim3(kBlock), (size_t(cap) * 5 + 7) & ~size_t(7), s, n,
+                   nl->neighbors, nl->offsets, d_keep, parent, cap);
+    }
     if (nl->num_bonds)
         FA_KLAUNCH(uf_union_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            nl->num_bonds, nl->neighbors, d_keep, parent);
