@@ -1203,24 +1203,39 @@ __global__ void uf_flatten_kernel(uint32_t n, uint32_t* parent, uint32_t* __rest
 }
 
 // key = (n - size) << 32 | root sorts clusters by size descending, then by
-// smallest member ascending (Cluster.cc:110-128)
-__global__ void uf_keys_kernel(uint32_t n, const uint32_t* __restrict__ parent, const uint32_t* __restrict__ size,
-                               uint32_t* __restrict__ counter, uint64_t* __restrict__ keys)
+// smallest member ascending (Cluster.cc:110-128).  Slots come from one counter
+// atomic per workgroup (most points of a liquid are roots; a device-wide
+// counter serialises its atomics).
+__global__ __launch_bounds__(kBlock) void uf_keys_kernel(uint32_t n, const uint32_t* __restrict__ parent,
+                                                         const uint32_t* __restrict__ size,
+                                                         uint32_t* __restrict__ counter, uint64_t* __restrict__ keys)
 {
+    __shared__ uint32_t s_wave[kBlock / 64 + 1];
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     bool const is_root = i < n && parent[i] == i;
-    // one counter atomic per wave (most points of a liquid are roots)
     uint64_t const roots = __ballot(is_root);
-    if (roots == 0)
-        return;
-    uint32_t const lane = threadIdx.x & 63u;
-    uint32_t const leader = uint32_t(__ffsll(static_cast<unsigned long long>(roots)) - 1);
-    uint32_t base = 0;
-    if (lane == leader)
-        base = atomicAdd(counter, uint32_t(__popcll(roots)));
-    base = uint32_t(__shfl(int(base), int(leader)));
+    uint32_t const lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        s_wave[wave] = uint32_t(__popcll(roots));
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kBlock / 64; ++w)
+        {
+            uint32_t const c = s_wave[w];
+            s_wave[w] = total;
+            total += c;
+        }
+        s_wave[kBlock / 64] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
     if (is_root)
-        keys[base + uint32_t(__popcll(roots & ((uint64_t(1) << lane) - 1u)))] = (uint64_t(n - size[i]) << 32) | i;
+    {
+        uint32_t const slot =
+            s_wave[kBlock / 64] + s_wave[wave] + uint32_t(__popcll(roots & ((uint64_t(1) << lane) - 1u)));
+        keys[slot] = (uint64_t(n - size[i]) << 32) | i;
+    }
 }
 
 __global__ void uf_rank_kernel(uint32_t n, uint32_t ncl, const uint64_t* __restrict__ keys,
