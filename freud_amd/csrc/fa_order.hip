@@ -1037,10 +1037,18 @@ __global__ __launch_bounds__(kBlock) void sl_decide_kernel(uint32_t n, const flo
         solid[i] = lab;
         fallback[i] = fb;
     }
-    // one atomic per wave
+    // one atomic per workgroup that has fallbacks (a device-wide counter
+    // serialises its atomics)
+    __shared__ uint32_t s_fb;
+    if (threadIdx.x == 0)
+        s_fb = 0;
+    __syncthreads();
     uint64_t const ball = __ballot(fb);
     if ((threadIdx.x & 63u) == 0 && ball)
-        atomicAdd(n_fallback, uint32_t(__popcll(ball)));
+        atomicAdd(&s_fb, uint32_t(__popcll(ball)));
+    __syncthreads();
+    if (threadIdx.x == 0 && s_fb)
+        atomicAdd(n_fallback, s_fb);
 }
 
 // SolidLiquid.cc:100-110: solid-like bonds between particles with at least
