@@ -1161,22 +1161,22 @@ __global__ void uf_union_kernel(uint64_t nb, const uint32_t* __restrict__ nbr, c
     }
 }
 
-// Roots into a separate array with a read-only walk: path-halving writes here
-// could overwrite another thread's finished entry with a non-root ancestor.
-// One round of pointer jumping, parent <- grandparent.  Parallel hooking can
-// leave long chains (bonds (i, i+1) processed together hook i+1 under i for
-// every i); ceil(log2 n) + 1 rounds bring every point next to its root, so the
-// flatten below walks at most a step or two.  Parents only move to ancestors,
-// so the in-place updates are safe.
+// One round of pointer jumping, parent <- its third ancestor (three hops), so
+// that the distance to a point's parent grows at least fourfold per round.
+// Parallel hooking can leave long chains (bonds (i, i+1) processed together
+// hook i+1 under i for every i); ceil(log4 n) + 1 rounds bring every point
+// next to its root, so the flatten below walks at most a step or two.
+// Parents only move to ancestors, so the in-place updates are safe.
 __global__ void uf_jump_kernel(uint32_t n, uint32_t* parent)
 {
     uint32_t const i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     uint32_t const p = parent[i];
-    uint32_t const gp = parent[p];
-    if (gp != p)
-        parent[i] = gp;
+    uint32_t const p2 = parent[p];
+    uint32_t const p3 = parent[p2];
+    if (p3 != p)
+        parent[i] = p3;
 }
 
 // Root of every point, compressing paths in place as it goes: a node's parent
@@ -1781,7 +1781,7 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
     if (nl->num_bonds)
         FA_KLAUNCH(uf_union_kernel, dim3(unsigned((nl->num_bonds + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
                            nl->num_bonds, nl->neighbors, d_keep, parent);
-    for (uint32_t span = 1; span < n; span *= 2)
+    for (uint64_t span = 1; span < n; span *= 4)
         FA_KLAUNCH(uf_jump_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent);
     FA_KLAUNCH(uf_flatten_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, root, size);
     FA_HIP(hipMemsetAsync(counter, 0, sizeof(uint32_t), s));
