@@ -1335,12 +1335,12 @@ float reduce_wigner3j_host(const std::complex<float>* src, int l)
 
 // Bond staging capacity of a kBlock-point tile: 1.3 x the mean bonds per point,
 // rounded to 64, within [512, kStage].
-uint32_t stage_capacity(const fa_nlist* nl, uint32_t n)
+uint32_t stage_capacity(const fa_nlist* nl, uint32_t n, uint32_t max_cap = kStage)
 {
     if (n == 0)
         return 0;
     double const want = std::ceil(double(nl->num_bonds) / double(n) * kBlock * 1.3);
-    uint32_t const cap = uint32_t(std::min(std::max(want, 512.0), double(kStage)));
+    uint32_t const cap = uint32_t(std::min(std::max(want, 512.0), double(max_cap)));
     return (cap + 63u) & ~63u;
 }
 
@@ -1385,7 +1385,7 @@ int launch_qlm_even_multi(int lmax, const fa::DevBox& db, const float* pts, uint
     if (lmax == LMAX)
     {
         unsigned const blocks = (n + kBlock - 1) / kBlock;
-        uint32_t const cap = stage_capacity(nl, n);
+        uint32_t const cap = stage_capacity(nl, n, 2 * kStage);
         if (weighted)
             FA_KLAUNCH((qlm_even_multi_kernel<LMAX, true>), dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(),
                        db, pts, n, nl->neighbors, nl->distances, nl->weights, nl->offsets, mask, o, ql, stride, cap);
