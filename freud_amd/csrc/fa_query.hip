@@ -385,14 +385,14 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
     if (extended)
     {
         uint32_t const slot = atomicAdd(nflag, 1u);
-        flagged[slot] = i;
+        flagged[slot] = p;
         counts[i] = 0;
         return;
     }
     uint32_t const n = min(found, k);
     counts[i] = n;
     for (uint32_t m = 0; m < n; ++m)
-        sel[uint64_t(i) * k + m] = best.at(m);
+        sel[uint64_t(m) * nq + p] = best.at(m);
 }
 
 // LinkCell formula kNN (LinkCell.cc:575-679): every point in cells within
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void knn_linkcell_kernel(DevBox b, DevGrid 
     uint32_t const n = min(found, k);
     counts[i] = n;
     for (uint32_t m = 0; m < n; ++m)
-        sel[uint64_t(i) * k + m] = best.at(m);
+        sel[uint64_t(m) * nq + i] = best.at(m);
 }
 
 // Exhaustive kNN for flagged queries (one block per query): every point and
@@ -431,14 +431,18 @@ __global__ __launch_bounds__(kBlock) void knn_linkcell_kernel(DevBox b, DevGrid 
 // reference's minimum-distance map, AABBQuery.cc:188-205); a point whose
 // closest image is nearer than r_min is dropped (AABBQuery.cc:197-201, 235-244).
 __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const float* __restrict__ pts, uint32_t np,
-                                                                const float* __restrict__ qpts,
+                                                                const float* __restrict__ qpts, uint32_t nq,
+                                                                const float4* __restrict__ cpos,
                                                                 const uint32_t* __restrict__ flagged, uint32_t k,
                                                                 float rlim_sq, float r_min, int exclude_ii,
                                                                 uint64_t* __restrict__ work,
                                                                 uint32_t* __restrict__ counts,
                                                                 uint64_t* __restrict__ sel)
 {
-    uint32_t const i = flagged[blockIdx.x];
+    // flagged holds processing positions: with cpos (queries in cell order)
+    // the point index is the cell-ordered entry's, else the position itself
+    uint32_t const p = flagged[blockIdx.x];
+    uint32_t const i = cpos ? __float_as_uint(cpos[p].w) : p;
     uint64_t* w = work + uint64_t(blockIdx.x) * np;
     float const qx = qpts[3 * i], qy = qpts[3 * i + 1];
     float const qz = b.is2D ? 0.0f : qpts[3 * i + 2];
@@ -496,7 +500,7 @@ __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const 
             break;
         if (threadIdx.x == 0)
         {
-            sel[uint64_t(i) * k + m] = mn;
+            sel[uint64_t(m) * nq + p] = mn;
             s_cnt = m + 1;
         }
         uint32_t const jm = uint32_t(mn) >> 5;
@@ -556,7 +560,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
         K* keys = in_lds ? s_keys + excl : reinterpret_cast<K*>(tmp + uint64_t(i) * k);
         for (uint32_t m = 0; m < n; ++m)
         {
-            uint64_t const s = sel[uint64_t(i) * k + m];
+            uint64_t const s = sel[uint64_t(m) * nq + p0 + tid];
             uint32_t const jc = uint32_t(s);
             if (BYDIST)
             {
@@ -1177,9 +1181,12 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
             void* work = nullptr;
             if (int rc2 = fa::scratch(5, uint64_t(nc) * nq->n * sizeof(uint64_t) + 64, &work))
                 return rc2;
-            FA_KLAUNCH(knn_exhaustive_kernel, dim3(nc), dim3(kBlock), 0, s, *dbp, nq->d_points, nq->n, qpts,
-                       flagged + c0, k, rlim * rlim, a.r_min, a.exclude_ii, static_cast<uint64_t*>(work), out->counts,
-                       sel);
+            // flagged positions are in the grid's cell order when the queries
+            // are the points themselves (the AABB kernel's processing order)
+            bool const cell_order = (qpts == nq->d_points) && (n_query == nq->n);
+            FA_KLAUNCH(knn_exhaustive_kernel, dim3(nc), dim3(kBlock), 0, s, *dbp, nq->d_points, nq->n, qpts, n_query,
+                       cell_order ? nq->cur->cpos : nullptr, flagged + c0, k, rlim * rlim, a.r_min, a.exclude_ii,
+                       static_cast<uint64_t*>(work), out->counts, sel);
             FA_HIP(hipGetLastError());
         }
         if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
@@ -1212,12 +1219,10 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
     }
     else
     {
-        if (self)
-            by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, true>)
-                        : launch(knn_write_kernel<LINKCELL_FORMULA, false, true>);
-        else
-            by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, false>)
-                        : launch(knn_write_kernel<LINKCELL_FORMULA, false, false>);
+        // the LinkCell search runs in index order and stores its selections by
+        // point index: the write pass takes the same order
+        by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, false>)
+                    : launch(knn_write_kernel<LINKCELL_FORMULA, false, false>);
     }
     FA_HIP(hipGetLastError());
     return fa::nlist_finish_counts(out);
