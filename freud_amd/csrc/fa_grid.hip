@@ -46,9 +46,13 @@ __device__ inline uint32_t wave_aggregated_increment(uint32_t* arr, uint32_t idx
     return uint32_t(__shfl(int(base), int(leader))) + rank;
 }
 
+// rank_of_point (nullable) receives each point's rank among the points of its
+// cell (the value its counter increment returned), which places it in the
+// scatter without a second round of atomics.
 __global__ void grid_assign_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n, int d0, int d1, int d2,
                                    uint32_t* __restrict__ cell_of_point, int* __restrict__ wrap_of_point,
-                                   uint32_t* __restrict__ counts, int* __restrict__ any_outside)
+                                   uint32_t* __restrict__ counts, int* __restrict__ any_outside,
+                                   uint32_t* __restrict__ rank_of_point)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool const valid = i < n;
@@ -65,21 +69,20 @@ __global__ void grid_assign_kernel(fa::DevBox b, const float* __restrict__ pts, 
         if (w[0] != 0 || w[1] != 0 || w[2] != 0)
             atomicOr(any_outside, 1);
     }
-    wave_aggregated_increment(counts, cell, valid);
+    uint32_t const rank = wave_aggregated_increment(counts, cell, valid);
+    if (valid && rank_of_point)
+        rank_of_point[i] = rank;
 }
 
 __global__ void grid_scatter_kernel(const float* __restrict__ pts, uint32_t n,
                                     const uint32_t* __restrict__ cell_of_point, const int* __restrict__ wrap_of_point,
-                                    const uint32_t* __restrict__ cstart, uint32_t* __restrict__ fill,
+                                    const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ rank_of_point,
                                     float4* __restrict__ cpos, int* __restrict__ wrapc)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool const valid = i < n;
-    uint32_t const cell = valid ? cell_of_point[i] : 0u;
-    uint32_t const rank = wave_aggregated_increment(fill, cell, valid);
-    if (!valid)
+    if (i >= n)
         return;
-    uint32_t const slot = cstart[cell] + rank;
+    uint32_t const slot = cstart[cell_of_point[i]] + rank_of_point[i];
     cpos[slot] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], __uint_as_float(i));
     wrapc[slot] = wrap_of_point[i];
 }
@@ -263,7 +266,7 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
     DevBox const db = make_devbox(box);
     unsigned const blocks = (n + 255) / 256;
     FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, db, d_points, n, dims[0], dims[1], dims[2],
-               cell_of_point, nullptr, counts, d_any);
+               cell_of_point, nullptr, counts, d_any, nullptr);
     if (int rc = exclusive_scan_u32(counts, cstart, ncell, nullptr))
         return rc;
     FA_KLAUNCH(order_scatter_kernel, dim3(blocks), dim3(256), 0, s, n, cell_of_point, cstart, fill, c.order);
@@ -349,26 +352,27 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
         for (int k = 0; k < 3; ++k)
             g.cw[k] = (k == 2 && box.is2D) ? 0.0f : npd[k] / float(dims[k]);
     }
-    // scratch: counts (ncell) + fill (ncell) + each point's packed image offset;
-    // the outside flag lives with the grid, read by the kernels (no host round trip)
+    // scratch: counts (ncell) + each point's rank in its cell + each point's
+    // packed image offset; the outside flag lives with the grid, read by the
+    // kernels (no host round trip)
     void* tmp = nullptr;
     size_t const cb = sizeof(uint32_t) * size_t(ncell);
-    if (int rc = scratch(2, 2 * cb + sizeof(int) * size_t(n) + 256, &tmp))
+    if (int rc = scratch(2, cb + sizeof(uint32_t) * size_t(n) + sizeof(int) * size_t(n) + 256, &tmp))
         return rc;
     uint32_t* counts = static_cast<uint32_t*>(tmp);
-    uint32_t* fill = counts + ncell;
-    int* wrap_of_point = reinterpret_cast<int*>(fill + ncell);
+    uint32_t* rank_of_point = counts + ncell;
+    int* wrap_of_point = reinterpret_cast<int*>(rank_of_point + n);
     if (!g.outside)
         if (int rc = device_alloc((void**) &g.outside, 256))
             return rc;
     int* d_any = g.outside;
-    FA_HIP(hipMemsetAsync(tmp, 0, 2 * cb, s));
+    FA_HIP(hipMemsetAsync(counts, 0, cb, s));
     FA_HIP(hipMemsetAsync(d_any, 0, sizeof(int), s));
     unsigned const blocks = (n + 255) / 256;
     if (n)
     {
         FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, g.dbox, d_points, n, dims[0], dims[1],
-                           dims[2], g.cell_of_point, wrap_of_point, counts, d_any);
+                           dims[2], g.cell_of_point, wrap_of_point, counts, d_any, rank_of_point);
         FA_HIP(hipGetLastError());
     }
     if (int rc = exclusive_scan_u32(counts, g.cstart, ncell, nullptr))
@@ -376,7 +380,7 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
     if (n)
     {
         FA_KLAUNCH(grid_scatter_kernel, dim3(blocks), dim3(256), 0, s, d_points, n, g.cell_of_point, wrap_of_point,
-                   g.cstart, fill, g.cpos, g.wrapc);
+                   g.cstart, rank_of_point, g.cpos, g.wrapc);
         FA_HIP(hipGetLastError());
     }
     return FA_OK;
