@@ -148,6 +148,7 @@ void free_grid(BatchGrid& g)
     fa::device_free(g.wrapc);
     fa::device_free(g.cstart);
     fa::device_free(g.cell_of_point);
+    fa::device_free(g.rank_of_point);
     fa::device_free(g.fill);
     g = BatchGrid{};
 }
@@ -219,6 +220,8 @@ int make_grid(fa_batch* b, float r, int R, BatchGrid& g, int Fx = 1)
     if (int rc = fa::device_alloc((void**) &g.wrapc, sizeof(int) * np))
         return rc;
     if (int rc = fa::device_alloc((void**) &g.cell_of_point, sizeof(uint32_t) * np))
+        return rc;
+    if (int rc = fa::device_alloc((void**) &g.rank_of_point, sizeof(uint32_t) * np))
         return rc;
     if (int rc = fa::device_alloc((void**) &g.cstart, sizeof(uint32_t) * (cells + 1)))
         return rc;

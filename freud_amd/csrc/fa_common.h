@@ -496,7 +496,8 @@ struct BatchGrid
     int* wrapc = nullptr;
     uint32_t* cstart = nullptr; // total_cells + 1
     uint32_t* cell_of_point = nullptr;
-    uint32_t* fill = nullptr; // total_cells + 1 (counts, then scatter cursors)
+    uint32_t* rank_of_point = nullptr; // each point's rank among its cell's points
+    uint32_t* fill = nullptr;          // total_cells + 1 (cell counts)
     uint32_t n_frames = 0, total_points = 0, total_cells = 0, total_rows = 0;
 };
 

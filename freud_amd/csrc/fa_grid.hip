@@ -88,8 +88,9 @@ __global__ void grid_scatter_kernel(const float* __restrict__ pts, uint32_t n,
 }
 
 // ---------------------------------------------------------------- batches
-// Per tile (one frame per workgroup): the global cell of each point and the
-// cell counts; flags the frame when a point lies outside its box.
+// Per tile (one frame per workgroup): the global cell of each point, its rank
+// among its cell's points and the cell counts; flags the frame when a point
+// lies outside its box.
 __global__ __launch_bounds__(256) void bgrid_assign_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
                                                            const float* __restrict__ pts)
 {
@@ -108,7 +109,9 @@ __global__ __launch_bounds__(256) void bgrid_assign_kernel(fa::BatchGrid G, cons
         if (w[0] != 0 || w[1] != 0 || w[2] != 0)
             atomicOr(&G.frames[tile_frame[t]].any_outside, 1);
     }
-    wave_aggregated_increment(G.fill, cell, valid);
+    uint32_t const rank = wave_aggregated_increment(G.fill, cell, valid);
+    if (valid)
+        G.rank_of_point[F.pt_base + local] = rank;
 }
 
 __global__ __launch_bounds__(256) void bgrid_scatter_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
@@ -118,12 +121,10 @@ __global__ __launch_bounds__(256) void bgrid_scatter_kernel(fa::BatchGrid G, con
     const fa::BatchFrame& F = G.frames[tile_frame[t]];
     uint32_t const local = (t - F.tile_base) * fa::kTile + threadIdx.x;
     bool const valid = local < F.n;
-    uint32_t const i = F.pt_base + local;
-    uint32_t const cell = valid ? G.cell_of_point[i] : 0u;
-    uint32_t const rank = wave_aggregated_increment(G.fill, cell, valid);
     if (!valid)
         return;
-    uint32_t const slot = G.cstart[cell] + rank;
+    uint32_t const i = F.pt_base + local;
+    uint32_t const slot = G.cstart[G.cell_of_point[i]] + G.rank_of_point[i];
     float const x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
     G.cpos[slot] = make_float4(x, y, z, __uint_as_float(local));
     int c[3], w[3];
@@ -290,8 +291,6 @@ int batch_grid_build(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t 
     if (ntiles)
         FA_KLAUNCH(bgrid_assign_kernel, dim3(ntiles), dim3(kTile), 0, s, G, d_tile_frame, d_pts);
     if (int rc = exclusive_scan_u32_async(G.fill, G.cstart, G.total_cells, tmp, tmp_bytes))
-        return rc;
-    if (int rc = zero_u32(G.fill, G.total_cells))
         return rc;
     if (ntiles)
         FA_KLAUNCH(bgrid_scatter_kernel, dim3(ntiles), dim3(kTile), 0, s, G, d_tile_frame, d_pts);
