@@ -35,6 +35,9 @@ int fail(int code, const std::string& msg);
     } while (0)
 
 hipStream_t stream();
+// Eight pinned host words for small read-backs on the library stream: copies
+// into them are queued asynchronously, so one synchronisation covers several.
+int readback_stage(uint64_t** h);
 
 // Per-kernel event timing on the library stream (off unless enabled through
 // fa_profile_enable); each FA_PROF_BEGIN/END pair brackets one launch.

@@ -47,14 +47,24 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
     void* t_red = base + 256 + ((bytes_scan + 255) / 256) * 256;
     FA_HIP(rocprim::inclusive_scan(t_scan, bytes_scan, d_in, d_out + 1, size_t(n), rocprim::plus<uint32_t>(), s));
     if (total)
-    {
         FA_HIP(rocprim::reduce(t_red, bytes_red, it, d_total, uint64_t(0), size_t(n), rocprim::plus<uint64_t>(), s));
-        FA_HIP(hipMemcpyAsync(total, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    }
+    if (!total && !d_extra)
+        return FA_OK;
+    // Both values come back through pinned host memory: the two copies are
+    // queued without host staging and one synchronisation covers them (a
+    // pageable copy would synchronise by itself).
+    uint64_t* h_stage = nullptr;
+    if (int rc = readback_stage(&h_stage))
+        return rc;
+    if (total)
+        FA_HIP(hipMemcpyAsync(h_stage, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     if (d_extra)
-        FA_HIP(hipMemcpyAsync(h_extra, d_extra, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    if (total || d_extra)
-        FA_HIP(hipStreamSynchronize(s));
+        FA_HIP(hipMemcpyAsync(h_stage + 1, d_extra, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (total)
+        *total = h_stage[0];
+    if (d_extra)
+        std::memcpy(h_extra, h_stage + 1, sizeof(uint32_t));
     return FA_OK;
 }
 

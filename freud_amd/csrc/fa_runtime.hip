@@ -15,6 +15,8 @@ namespace {
 thread_local std::string g_err;
 std::mutex g_mutex;
 hipStream_t g_stream = nullptr;
+// pinned host words for small device-to-host read-backs (allocated once)
+uint64_t* g_stage = nullptr;
 bool g_init = false;
 std::vector<std::pair<void*, size_t>> g_scratch;
 hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
@@ -118,11 +120,20 @@ static int ensure_init()
     FA_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     FA_HIP(hipEventCreate(&g_t0));
     FA_HIP(hipEventCreate(&g_t1));
+    FA_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_stage), 8 * sizeof(uint64_t), hipHostMallocDefault));
     g_init = true;
     return FA_OK;
 }
 
 hipStream_t stream() { return g_stream; }
+
+int readback_stage(uint64_t** h)
+{
+    if (!g_stage)
+        return fail(FA_ERR_DEVICE, "library not initialised (no pinned read-back buffer)");
+    *h = g_stage;
+    return FA_OK;
+}
 
 // Caching device allocator.  hipFree synchronises the whole device and
 // hipMalloc is slow, and a frame-by-frame analysis (a new NeighborQuery with its
