@@ -35,9 +35,14 @@ int fail(int code, const std::string& msg);
     } while (0)
 
 hipStream_t stream();
-// Eight pinned host words for small read-backs on the library stream: copies
-// into them are queued asynchronously, so one synchronisation covers several.
-int readback_stage(uint64_t** h);
+// Pinned host memory for small read-backs on the library stream: copies into
+// it are queued asynchronously, so one synchronisation covers several.  *h is
+// null when `bytes` exceeds its kReadbackBytes (the caller then copies into
+// pageable memory).
+constexpr size_t kReadbackBytes = 64 * 1024;
+int readback_stage(size_t bytes, void** h);
+// One device word read back through the pinned buffer (synchronises the stream).
+int read_u32(const uint32_t* d, uint32_t* h);
 
 // Per-kernel event timing on the library stream (off unless enabled through
 // fa_profile_enable); each FA_PROF_BEGIN/END pair brackets one launch.

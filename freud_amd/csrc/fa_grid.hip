@@ -226,8 +226,8 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
         FA_KLAUNCH(order_probe_kernel, dim3(kProbe / 256), dim3(256), 0, s, make_devbox(box), d_points, n, far * far,
                    cnt);
         uint32_t h = 0;
-        FA_HIP(hipMemcpyAsync(&h, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        FA_HIP(hipStreamSynchronize(s));
+        if (int rc = read_u32(cnt, &h))
+            return rc;
         if (h <= kProbe / 4)
         {
             c.pts = d_points;

@@ -1568,8 +1568,14 @@ int steinhardt_run(const fa_box* box, const float* d_points, uint32_t n, const f
             FA_KLAUNCH(sum_partials_kernel, dim3(nj), dim3(kSumThreads), 0, s, jobs, blocks);
             FA_HIP(hipGetLastError());
         }
-        FA_HIP(hipMemcpyAsync(hsys.data(), sys_base, sizeof(float2) * nm_total, hipMemcpyDeviceToHost, s));
+        size_t const sys_bytes = sizeof(float2) * nm_total;
+        void* stage = nullptr;
+        if (int rc = fa::readback_stage(sys_bytes, &stage))
+            return rc;
+        FA_HIP(hipMemcpyAsync(stage ? stage : hsys.data(), sys_base, sys_bytes, hipMemcpyDeviceToHost, s));
         FA_HIP(hipStreamSynchronize(s));
+        if (stage)
+            std::memcpy(hsys.data(), stage, sys_bytes);
     }
     size_t so = 0;
     for (int k = 0; k < n_l; ++k)
@@ -1742,10 +1748,7 @@ int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float ma
                    q_threshold, solid_threshold, d_solid, d_fallback, cnt);
     FA_HIP(hipGetLastError());
     if (n_fallback)
-    {
-        FA_HIP(hipMemcpyAsync(n_fallback, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, fa::stream()));
-        FA_HIP(hipStreamSynchronize(fa::stream()));
-    }
+        return fa::read_u32(cnt, n_fallback);
     return FA_OK;
 }
 
@@ -1788,8 +1791,8 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
     FA_KLAUNCH(uf_keys_kernel, dim3(blocks), dim3(kBlock), 0, s, n, parent, size, counter, keys);
     FA_HIP(hipGetLastError());
     uint32_t ncl = 0;
-    FA_HIP(hipMemcpyAsync(&ncl, counter, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FA_HIP(hipStreamSynchronize(s));
+    if (int rc = fa::read_u32(counter, &ncl))
+        return rc;
     if (ncl == 0)
         return fa::fail(FA_ERR_RUNTIME, "cluster labelling found no roots");
     if (int rc = fa::sort_u64(keys, ncl))

@@ -53,9 +53,10 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
     // Both values come back through pinned host memory: the two copies are
     // queued without host staging and one synchronisation covers them (a
     // pageable copy would synchronise by itself).
-    uint64_t* h_stage = nullptr;
-    if (int rc = readback_stage(&h_stage))
+    void* stage = nullptr;
+    if (int rc = readback_stage(2 * sizeof(uint64_t), &stage))
         return rc;
+    uint64_t* h_stage = static_cast<uint64_t*>(stage);
     if (total)
         FA_HIP(hipMemcpyAsync(h_stage, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     if (d_extra)

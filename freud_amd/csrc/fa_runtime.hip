@@ -120,18 +120,29 @@ static int ensure_init()
     FA_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     FA_HIP(hipEventCreate(&g_t0));
     FA_HIP(hipEventCreate(&g_t1));
-    FA_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_stage), 8 * sizeof(uint64_t), hipHostMallocDefault));
+    FA_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_stage), kReadbackBytes, hipHostMallocDefault));
     g_init = true;
     return FA_OK;
 }
 
 hipStream_t stream() { return g_stream; }
 
-int readback_stage(uint64_t** h)
+int read_u32(const uint32_t* d, uint32_t* h)
+{
+    void* stage = nullptr;
+    if (int rc = readback_stage(sizeof(uint32_t), &stage))
+        return rc;
+    FA_HIP(hipMemcpyAsync(stage, d, sizeof(uint32_t), hipMemcpyDeviceToHost, g_stream));
+    FA_HIP(hipStreamSynchronize(g_stream));
+    std::memcpy(h, stage, sizeof(uint32_t));
+    return FA_OK;
+}
+
+int readback_stage(size_t bytes, void** h)
 {
     if (!g_stage)
         return fail(FA_ERR_DEVICE, "library not initialised (no pinned read-back buffer)");
-    *h = g_stage;
+    *h = bytes <= kReadbackBytes ? g_stage : nullptr;
     return FA_OK;
 }
 
