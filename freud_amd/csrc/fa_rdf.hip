@@ -1092,7 +1092,7 @@ __device__ inline void flush_wave_hist(uint32_t* wh, uint32_t nbins, uint32_t* _
 // frame) when the frame changes and at exit.  Frames with points outside their
 // box take the generic per-point search for the cluster's points (weight 1 per
 // ordered pair).
-constexpr uint32_t kChunk = 4;
+constexpr uint32_t kChunk = 2;
 
 template<int R>
 __global__ __launch_bounds__(kBlock) void brdf_rows_kernel(fa::BatchGrid G, const uint4* __restrict__ table,
