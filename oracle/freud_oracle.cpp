@@ -15,14 +15,24 @@
 //                                    tests/test_box_box.py
 //   ball query (AABB, LinkCell) .... numpy brute force with the same per-formula
 //                                    float32 arithmetic; hand-built known answers
-//   kNN ............................ brute-force k-smallest; known answers; ties
-//                                    are broken by point index (reference order is
-//                                    an unstable sort: tie order is unpinned)
+//   kNN ............................ brute-force k-smallest (also with r_min);
+//                                    known answers incl. the reference's r_min
+//                                    case; ties are broken by point index
+//                                    (reference order is an unstable sort: tie
+//                                    order is unpinned)
+//   all_pairs ...................... the reference's all_pairs test (counts,
+//                                    index order, distances) and hand-built
+//                                    vectors wrap(q_i - p_j)
 //   RDF counts / g(r) / N(r) ....... golden tests/integration/files/lj/rdf.npz;
 //                                    np.histogram_bin_edges; n_r closed form
 //   Y_lm / q_lm .................... scipy.special.sph_harm_y (closed form)
 //   q_l, w_l, averages ............. GC validation files (atol 2e-5); perfect FCC
 //                                    constants; Wigner-3j vs sympy exact values
+//   weighted Steinhardt ............ the reference's re-weighted FCC test, plus
+//                                    invariants tying it to the unweighted
+//                                    branch (uniform weight, weight 0 = removed
+//                                    bond, weight 2 = doubled bond, dominant
+//                                    weight -> q_l = 1)
 //   SolidLiquid, Cluster, Hexatic .. FCC known answer; scipy connected components;
 //                                    closed forms
 //   system `order` ................. parity unpinned bitwise (the reference's value

@@ -72,18 +72,21 @@ def ball_linkcell(box, points, query_points, r_max, r_min=0.0, exclude_ii=False)
     return _to_nlist(bonds)
 
 
-def knn_aabb(box, points, query_points, k, exclude_ii=False, r_max=np.inf):
-    """k nearest by the AABB formula, closest image per point, ties by index."""
+def knn_aabb(box, points, query_points, k, exclude_ii=False, r_max=np.inf, r_min=0.0):
+    """k nearest by the AABB formula, closest image per point, ties by index.
+    A point whose closest image lies nearer than r_min (d < r_min) is left
+    out (AABBQuery.cc:188-205, 235-244)."""
     full = ball_aabb(box, points, query_points, r_max=min(r_max, 1e30), exclude_ii=exclude_ii)
+    return _select_k(full, k, r_min)
+
+
+def knn_linkcell(box, points, query_points, k, exclude_ii=False, r_max=np.inf, r_min=0.0):
+    """k nearest by the LinkCell formula; r_min as r_sq >= r_min^2 (LinkCell.cc:616-630)."""
+    full = ball_linkcell(box, points, query_points, r_max=min(r_max, 1e30), r_min=r_min, exclude_ii=exclude_ii)
     return _select_k(full, k)
 
 
-def knn_linkcell(box, points, query_points, k, exclude_ii=False, r_max=np.inf):
-    full = ball_linkcell(box, points, query_points, r_max=min(r_max, 1e30), exclude_ii=exclude_ii)
-    return _select_k(full, k)
-
-
-def _select_k(full, k):
+def _select_k(full, k, r_min=0.0):
     ij, d, v = full["ij"], full["distances"], full["vectors"]
     best = {}
     for t in range(len(ij)):
@@ -92,6 +95,8 @@ def _select_k(full, k):
             best[key] = t
     per_i = {}
     for (i, j), t in best.items():
+        if not d[t] >= f32(r_min):
+            continue
         per_i.setdefault(i, []).append((d[t], j, t))
     keep = []
     for i, lst in per_i.items():

@@ -339,3 +339,159 @@ def test_hexatic_hexagon(oracle):
     pr = (pts @ rot.T).astype(np.float32)
     psi_r = oracle.hexatic(box, pr, oracle.query(box, pr, pr, dict(num_neighbors=6, exclude_ii=True)), k=6)
     npt.assert_allclose(psi_r[0], psi[0] * np.exp(6j * th), atol=1e-5)
+
+
+# --------------------------------------------------------------------- r_min, weighted bonds, all_pairs
+def test_r_min_known_answer(oracle):
+    # tests/test_locality_neighbor_query.py:218-251 of the reference: four
+    # points on a line, ball and nearest-neighbour queries with r_min = 1.1
+    box = (10, 10, 10, 0, 0, 0, False)
+    pts = np.array([[0, 0, 0], [1, 0, 0], [3, 0, 0], [2, 0, 0]], np.float32)
+    for kind in ("aabb", "linkcell"):
+        nl = oracle.query(box, pts, pts, dict(mode="ball", r_max=2.9, r_min=1.1, exclude_ii=True), nq_type=kind,
+                          cell_width=1.0)
+        assert _neighbor_sets(nl, 4) == [{3}, {2}, {1}, {0}]
+        nl = oracle.query(box, pts, pts, dict(mode="nearest", num_neighbors=3, r_min=1.1, exclude_ii=True),
+                          nq_type=kind, cell_width=1.0)
+        assert _neighbor_sets(nl, 4) == [{2, 3}, {2}, {0, 1}, {0}]
+        assert np.all(nl["distances"] >= np.float32(1.1))
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_knn_r_min_bruteforce(oracle, seed):
+    """kNN with r_min against numpy brute force: the k nearest points whose
+    closest image lies at d >= r_min, by (d, j)."""
+    rng = np.random.default_rng(20 + seed)
+    box = Box(8.0, 8.5, 9.0, 0.2, -0.1, 0.15)
+    pts = box.make_absolute(rng.random((150, 3)).astype(np.float32))
+    base = brute.knn_aabb(box, pts, pts, 4, exclude_ii=True)
+    for r_min in (0.35, float(np.sort(base["distances"])[len(base["distances"]) // 3])):
+        for k in (1, 6, 10):
+            got = oracle.query(_box_tuple(box), pts, pts, dict(num_neighbors=k, r_min=r_min, exclude_ii=True))
+            ref = brute.knn_aabb(box, pts, pts, k, exclude_ii=True, r_min=r_min)
+            npt.assert_array_equal(got["ij"], ref["ij"])
+            npt.assert_array_equal(got["distances"], ref["distances"])
+            npt.assert_array_equal(got["vectors"], ref["vectors"])
+            got = oracle.query(_box_tuple(box), pts, pts, dict(num_neighbors=k, r_min=r_min, exclude_ii=True),
+                               nq_type="linkcell", cell_width=2.0)
+            ref = brute.knn_linkcell(box, pts, pts, k, exclude_ii=True, r_min=r_min)
+            npt.assert_array_equal(got["ij"], ref["ij"])
+            npt.assert_array_equal(got["distances"], ref["distances"])
+
+
+PERFECT_FCC_Q6 = 0.57452416
+PERFECT_FCC_W6 = -0.00262604
+
+
+def _fcc12(oracle):
+    # freud.data.UnitCell.fcc().generate_system(4) of the reference: a = 1, L = 4
+    box, p = _fcc(4, 1.0)
+    nl = oracle.query(box, p, p, dict(num_neighbors=12, r_guess=1.5, exclude_ii=True))
+    return box, p, nl
+
+
+def _segments(ij, n):
+    return np.searchsorted(ij[:, 0], np.arange(n))
+
+
+@pytest.mark.parametrize("wt", [0, 0.1, 0.9, 1.1, 10, 1e6])
+def test_steinhardt_weighted_fcc(oracle, wt):
+    # tests/test_order_steinhardt.py:217-260 of the reference: the first bond of
+    # every particle re-weighted; all q6 stay equal and exceed the perfect FCC
+    # value, and w6 moves away from the perfect value
+    box, p, nl = _fcc12(oracle)
+    assert len(nl["ij"]) == 12 * len(p)
+    w = nl["weights"].copy()
+    w[_segments(nl["ij"], len(p))] = wt
+    wnl = dict(nl, weights=w)
+    r = oracle.steinhardt(box, p, wnl, [6], weighted=True)
+    q6 = r["ql"][:, 0]
+    assert np.all(q6 > PERFECT_FCC_Q6)
+    npt.assert_allclose(q6, q6[0], atol=1e-5)
+    assert r["order"][0] > PERFECT_FCC_Q6
+    r = oracle.steinhardt(box, p, wnl, [6], wl=True, weighted=True)
+    assert not np.allclose(np.average(r["wl"][:, 0]), PERFECT_FCC_W6, rtol=1e-5, atol=0)
+    assert not np.allclose(r["order"][0], PERFECT_FCC_W6, rtol=1e-5, atol=0)
+    # unweighted, the same list gives the perfect values
+    r = oracle.steinhardt(box, p, wnl, [6], wl=True)
+    npt.assert_allclose(r["wl"][:, 0], PERFECT_FCC_W6, atol=1e-5)
+
+
+def test_steinhardt_weighted_invariants(oracle):
+    """The weighted branch (q_lm = sum w Y / sum w, Steinhardt.cc:150-205) tied
+    to the unweighted branch, which the GC goldens pin: a uniform weight
+    changes nothing, weight 0 removes a bond, weight 2 doubles it, and one
+    dominant weight leaves a single bond direction, whose q_l is 1
+    (sum_m |Y_lm|^2 = (2l+1)/4pi)."""
+    rng = np.random.default_rng(4)
+    box = Box(7.0, 7.5, 8.0, 0.1, -0.2, 0.05)
+    pts = box.make_absolute(rng.random((200, 3)).astype(np.float32))
+    bt = _box_tuple(box)
+    nl = oracle.query(bt, pts, pts, dict(r_max=1.8, exclude_ii=True))
+    ls = [4, 6]
+    base = oracle.steinhardt(bt, pts, nl, ls)
+    ok = np.isfinite(base["ql"][:, 0])
+    for flags in (dict(), dict(wl=True), dict(average=True)):
+        u = oracle.steinhardt(bt, pts, dict(nl, weights=np.full(len(nl["ij"]), 3.7, np.float32)), ls, weighted=True,
+                              **flags)
+        b = oracle.steinhardt(bt, pts, nl, ls, **flags)
+        npt.assert_allclose(u["particle_order"][ok], b["particle_order"][ok], atol=1e-6)
+    # weight 0 on every third bond == those bonds removed
+    drop = np.arange(len(nl["ij"])) % 3 == 0
+    w = np.ones(len(nl["ij"]), np.float32)
+    w[drop] = 0
+    z = oracle.steinhardt(bt, pts, dict(nl, weights=w), ls, weighted=True, wl=True)
+    kept = {key: v[~drop] for key, v in nl.items()}
+    r = oracle.steinhardt(bt, pts, kept, ls, wl=True)
+    ok2 = np.isfinite(r["ql"][:, 0])
+    npt.assert_allclose(z["ql"][ok2], r["ql"][ok2], atol=1e-6)
+    npt.assert_allclose(z["wl"][ok2], r["wl"][ok2], atol=1e-6)
+    # weight 2 on every fifth bond == those bonds listed twice
+    dup = np.arange(len(nl["ij"])) % 5 == 0
+    w = np.ones(len(nl["ij"]), np.float32)
+    w[dup] = 2
+    t = oracle.steinhardt(bt, pts, dict(nl, weights=w), ls, weighted=True)
+    rep = np.where(dup, 2, 1)
+    doubled = {key: np.repeat(v, rep, axis=0) for key, v in nl.items()}
+    r = oracle.steinhardt(bt, pts, doubled, ls)
+    for a, c in zip(t["qlm"], r["qlm"]):
+        npt.assert_allclose(a[ok], c[ok], atol=1e-6)
+    npt.assert_allclose(t["ql"][ok], r["ql"][ok], atol=1e-6)
+    # one dominant bond per particle: q_l -> 1
+    seg = _segments(nl["ij"], len(pts))[ok]
+    w = np.ones(len(nl["ij"]), np.float32)
+    w[seg] = 1e7
+    big = oracle.steinhardt(bt, pts, dict(nl, weights=w), ls, weighted=True)
+    npt.assert_allclose(big["ql"][ok], 1.0, atol=1e-4)
+
+
+def test_all_pairs_reference_cases(oracle):
+    # tests/test_locality_neighbor_list.py:220-251 of the reference
+    rng = np.random.default_rng(1)
+    N, M, L = 100, 50, 10.0
+    box = Box.cube(L)
+    pts = ((rng.random((N, 3)) - 0.5) * L).astype(np.float32)
+    qp = ((rng.random((M, 3)) - 0.5) * L).astype(np.float32)
+    nl = oracle.all_pairs(_box_tuple(box), pts, pts, exclude_ii=True)
+    assert len(nl["ij"]) == N * (N - 1)
+    assert not np.any(nl["ij"][:, 0] == nl["ij"][:, 1])
+    nl = oracle.all_pairs(_box_tuple(box), pts, qp, exclude_ii=False)
+    npt.assert_array_equal(nl["ij"][:, 0], np.arange(M).repeat(N))
+    npt.assert_array_equal(nl["ij"][:, 1], np.tile(np.arange(N), M))
+    d = box.wrap((qp[nl["ij"][:, 0]] - pts[nl["ij"][:, 1]]).astype(np.float32))
+    # the vector is wrap(q_i - p_j) (NeighborList.cc:121), the opposite sign of
+    # a query's bond vector p_j - q_i
+    npt.assert_array_equal(nl["vectors"], d)
+    npt.assert_allclose(nl["distances"], np.linalg.norm(d, axis=-1), rtol=5e-7)
+    npt.assert_array_equal(nl["weights"], 1.0)
+    # hand-built: q = (1, 0, 0), p = (0, 0, 0) -> (+1, 0, 0); across the edge
+    # q = (4.5, 0, 0), p = (-4.5, 0, 0): 9 wraps to -1
+    two = oracle.all_pairs(_box_tuple(box), np.zeros((1, 3), np.float32), np.array([[1, 0, 0]], np.float32),
+                           exclude_ii=False)
+    npt.assert_array_equal(two["vectors"], [[1, 0, 0]])
+    edge = oracle.all_pairs(_box_tuple(box), np.array([[-4.5, 0, 0]], np.float32),
+                            np.array([[4.5, 0, 0]], np.float32), exclude_ii=False)
+    # (the fractional round trip of the wrap rounds 9 to -1.0000002)
+    npt.assert_allclose(edge["vectors"], [[-1, 0, 0]], atol=1e-6)
+    assert edge["vectors"][0, 0] < 0
+    npt.assert_allclose(edge["distances"], [1.0], atol=1e-6)
