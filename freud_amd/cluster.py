@@ -8,7 +8,7 @@ import numpy as np
 from freud_amd import _lib
 from freud_amd._lib import check, lib
 from freud_amd.locality import _PairCompute
-from freud_amd.util import _Compute
+from freud_amd.util import _Compute, computed_property
 
 
 class Cluster(_PairCompute):
@@ -31,15 +31,15 @@ class Cluster(_PairCompute):
         self._keys = None if keys is None else np.asarray(keys)
         return self
 
-    @_Compute._computed_property
+    @computed_property
     def num_clusters(self):
         return self._num_clusters
 
-    @_Compute._computed_property
+    @computed_property
     def cluster_idx(self):
         return self._d_cidx.numpy()
 
-    @_Compute._computed_property
+    @computed_property
     def cluster_keys(self):
         idx = self._d_cidx.numpy()
         keys = np.arange(len(idx)) if self._keys is None else self._keys

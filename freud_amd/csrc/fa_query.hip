@@ -10,8 +10,8 @@
 // lane, so the NeighborList stores are coalesced.
 //
 // kNN: one thread per query point walks Chebyshev rings of cells, keeping the
-// k best (r_sq, j) keys in registers, and stops once the k-th distance is
-// inside the completely searched radius.
+// k best (d, j) keys in registers, and stops once the k-th distance is inside
+// the completely searched radius.
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -318,6 +318,33 @@ template<int KMAX> struct TopK
     }
 };
 
+// kNN selection keys order candidates by (d, j, image) with d = sqrtf(r_sq),
+// the reference's distance (AABBQuery.cc:227,246 and LinkCell.cc:656 sort the
+// bonds by their float distance; equal distances go to the lower index here).
+// Two different r_sq can share one d, so the key holds the bits of d itself
+// (monotone as an unsigned integer for d >= 0), not those of r_sq.
+__device__ inline uint64_t knn_key(float r_sq, uint32_t jc)
+{
+    return (uint64_t(__float_as_uint(sqrtf(r_sq))) << 32) | jc;
+}
+__device__ inline float knn_key_distance(uint64_t key)
+{
+    return __uint_as_float(uint32_t(key >> 32));
+}
+// The square root is only taken for candidates that can enter the selection:
+// once it is full, a candidate needs d <= d_last, which implies r_sq below
+// d_last^2 (1 + 2^-22) for the correctly rounded sqrtf; the bound below leaves
+// a margin of 4x that plus an absolute term for subnormal squares.  A bound
+// that admits too much only costs a square root, never a different result.
+template<int KMAX> __device__ inline float knn_rsq_cut(const TopK<KMAX>& best, float r_sq_limit)
+{
+    uint64_t const last = best.v[KMAX - 1];
+    if (last == ~0ull)
+        return r_sq_limit;
+    float const d = knn_key_distance(last);
+    return fminf(r_sq_limit, d * d * (1.0f + 0x1p-20f) + 1e-37f);
+}
+
 // AABB formula kNN (AABBQuery.cc:152-281).  The candidate set is every (j,
 // image) with r_sq < rlim_sq (rlim = the last ball radius of the reference's
 // expanding search) and sqrt(r_sq) >= r_min; the k smallest by (r_sq, j) are
@@ -341,6 +368,7 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
     best.init();
     uint32_t found = 0;
     bool extended = false;
+    float cut = rlim_sq; // candidates above it cannot enter the selection
     for (int R = 0; R <= rmax_ring; ++R)
     {
         fa::for_each_candidate<AABB_FORMULA>(b, g, qx, qy, qz, R, true,
@@ -354,13 +382,15 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
                                                  if (!(r_sq >= rmin_thr))
                                                      return;
                                                  ++found;
-                                                 best.insert((uint64_t(__float_as_uint(r_sq)) << 32)
-                                                             | ((j << 5) | uint32_t(code)));
+                                                 if (!(r_sq <= cut))
+                                                     return;
+                                                 best.insert(knn_key(r_sq, (j << 5) | uint32_t(code)));
+                                                 cut = knn_rsq_cut(best, rlim_sq);
                                              });
         float const complete = float(R) * wmin * (1.0f - 1e-4f);
         if (found >= k)
         {
-            float const kth = sqrtf(__uint_as_float(uint32_t(best.at(k - 1) >> 32)));
+            float const kth = knn_key_distance(best.at(k - 1));
             if (kth < complete)
                 break;
         }
@@ -378,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
     uint32_t const nsel = min(found, k);
     if (!extended && rlim > half_min_plane && nsel > 0)
     {
-        float const last = sqrtf(__uint_as_float(uint32_t(best.at(nsel - 1) >> 32)));
+        float const last = knn_key_distance(best.at(nsel - 1));
         if (last >= half_min_plane)
             extended = true;
     }
@@ -411,13 +441,16 @@ __global__ __launch_bounds__(kBlock) void knn_linkcell_kernel(DevBox b, DevGrid 
     TopK<KMAX> best;
     best.init();
     uint32_t found = 0;
+    float cut = r_max_sq;
     fa::for_each_candidate<LINKCELL_FORMULA>(b, g, qx, qy, qz, R, false,
                                              [&](uint32_t j, float, float, float, float r_sq, int) {
                                                  if (r_sq < r_max_sq && r_sq >= r_min_sq && !(exclude_ii && j == i))
                                                  {
                                                      ++found;
-                                                     best.insert((uint64_t(__float_as_uint(r_sq)) << 32)
-                                                                 | ((j << 5) | 13u));
+                                                     if (!(r_sq <= cut))
+                                                         return;
+                                                     best.insert(knn_key(r_sq, (j << 5) | 13u));
+                                                     cut = knn_rsq_cut(best, r_max_sq);
                                                  }
                                              });
     uint32_t const n = min(found, k);
@@ -464,12 +497,9 @@ __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const 
                         float const rz = pz - (qz + b.img[code][2]);
                         float const r_sq = (rx * rx + ry * ry) + rz * rz;
                         if (r_sq < rlim_sq)
-                        {
-                            uint64_t const key = (uint64_t(__float_as_uint(r_sq)) << 32) | ((j << 5) | uint32_t(code));
-                            bestk = min(bestk, key);
-                        }
+                            bestk = min(bestk, knn_key(r_sq, (j << 5) | uint32_t(code)));
                     }
-            if (bestk != ~0ull && !(sqrtf(__uint_as_float(uint32_t(bestk >> 32))) >= r_min))
+            if (bestk != ~0ull && !(knn_key_distance(bestk) >= r_min))
                 bestk = ~0ull;
         }
         w[j] = bestk;
@@ -563,10 +593,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
             uint64_t const s = sel[uint64_t(m) * nq + p0 + tid];
             uint32_t const jc = uint32_t(s);
             if (BYDIST)
-            {
-                float const d = sqrtf(__uint_as_float(uint32_t(s >> 32)));
-                keys[m] = K((uint64_t(float_order_bits(d)) << 32) | jc);
-            }
+                keys[m] = K((uint64_t(float_order_bits(knn_key_distance(s))) << 32) | jc);
             else
                 keys[m] = K(jc);
         }

@@ -14,7 +14,7 @@ import numpy as np
 from freud_amd import _lib
 from freud_amd._lib import DeviceArray, check, lib
 from freud_amd.locality import _SpatialHistogram1D
-from freud_amd.util import _Compute
+from freud_amd.util import _Compute, computed_property
 
 
 class RDF(_SpatialHistogram1D):
@@ -100,19 +100,19 @@ class RDF(_SpatialHistogram1D):
             self._reduced = (counts, rdf, n_r)
         return self._reduced
 
-    @_Compute._computed_property
+    @computed_property
     def rdf(self):
         return self._reduce()[1].copy()
 
-    @_Compute._computed_property
+    @computed_property
     def n_r(self):
         return self._reduce()[2].copy()
 
-    @_Compute._computed_property
+    @computed_property
     def bin_counts(self):
         return self._reduce()[0].copy()
 
-    @_Compute._computed_property
+    @computed_property
     def box(self):
         import freud_amd.box
 

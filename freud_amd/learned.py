@@ -19,7 +19,7 @@ import numpy as np
 from freud_amd import _lib
 from freud_amd._lib import DeviceArray, check, lib
 from freud_amd.locality import AABBQuery, _PairCompute
-from freud_amd.util import _Compute
+from freud_amd.util import _Compute, computed_property
 
 
 class LearnedSolidLiquid(_PairCompute):
@@ -190,29 +190,29 @@ class LearnedSolidLiquid(_PairCompute):
         self._nlist = nlist
         return self
 
-    @_Compute._computed_property
+    @computed_property
     def solid(self):
         """(N,) bool: solid-like (classifier, or exact where it fell back)."""
         return self._d["solid"].numpy()[: self._n].astype(bool)
 
-    @_Compute._computed_property
+    @computed_property
     def fallback(self):
         """(N,) bool: particles labelled by the exact bond-dot path."""
         return self._d["fallback"].numpy()[: self._n].astype(bool)
 
-    @_Compute._computed_property
+    @computed_property
     def num_fallback(self):
         return self._n_fallback
 
-    @_Compute._computed_property
+    @computed_property
     def logits(self):
         return self._d["logits"].numpy()
 
-    @_Compute._computed_property
+    @computed_property
     def particle_features(self):
         return self._d["feat"].numpy()
 
-    @_Compute._computed_property
+    @computed_property
     def nlist(self):
         return self._nlist
 

@@ -11,7 +11,7 @@ import numpy as np
 from freud_amd import _lib
 from freud_amd._lib import check, lib
 from freud_amd.locality import _PairCompute
-from freud_amd.util import _Compute
+from freud_amd.util import _Compute, computed_property
 
 
 class Steinhardt(_PairCompute):
@@ -99,11 +99,11 @@ class Steinhardt(_PairCompute):
                 self._cache[name] = out
         return self._cache[name]
 
-    @_Compute._computed_property
+    @computed_property
     def order(self):
         return float(self._order[0]) if len(self._order) == 1 else self._order.copy()
 
-    @_Compute._computed_property
+    @computed_property
     def particle_order(self):
         if self._wl:
             p = self._get("wl")
@@ -113,12 +113,12 @@ class Steinhardt(_PairCompute):
             p = self._get("ql")
         return np.ravel(p).copy() if p.shape[1] == 1 else p.copy()
 
-    @_Compute._computed_property
+    @computed_property
     def ql(self):
         q = self._get("ql")
         return np.ravel(q).copy() if q.shape[1] == 1 else q.copy()
 
-    @_Compute._computed_property
+    @computed_property
     def particle_harmonics(self):
         q = self._get("qlm")
         return q if len(q) > 1 else q[0]
@@ -189,33 +189,33 @@ class SolidLiquid(_PairCompute):
     def normalize_q(self):
         return self._normalize_q
 
-    @_Compute._computed_property
+    @computed_property
     def cluster_idx(self):
         return self._d_cidx.numpy()
 
-    @_Compute._computed_property
+    @computed_property
     def ql_ij(self):
         return self._d_qlij.numpy()[: self._nb]
 
-    @_Compute._computed_property
+    @computed_property
     def particle_harmonics(self):
         q = self._d_qlm.numpy()
         return (q[..., 0] + 1j * q[..., 1]).astype(np.complex64)
 
-    @_Compute._computed_property
+    @computed_property
     def cluster_sizes(self):
         return self._d_sizes.numpy()[: self._num_clusters]
 
-    @_Compute._computed_property
+    @computed_property
     def largest_cluster_size(self):
         s = self.cluster_sizes
         return int(s[0]) if len(s) else 0
 
-    @_Compute._computed_property
+    @computed_property
     def nlist(self):
         return self._nlist
 
-    @_Compute._computed_property
+    @computed_property
     def num_connections(self):
         return self._d_nc.numpy()
 
@@ -250,7 +250,7 @@ class Hexatic(_PairCompute):
     def default_query_args(self):
         return dict(mode="nearest", num_neighbors=self._k)
 
-    @_Compute._computed_property
+    @computed_property
     def particle_order(self):
         p = self._d_psi.numpy()
         return (p[:, 0] + 1j * p[:, 1]).astype(np.complex64)

@@ -1,82 +1,111 @@
-"""Compute-class plumbing mirroring freud/util.py:11-168."""
+"""Host-side helpers shared by the compute classes.
 
-from functools import wraps
+Behaviour follows freud's compute classes (freud/util.py:11-168 of freud
+3.5.0): result properties raise ``AttributeError`` until ``compute`` has
+returned once; array arguments are coerced to C-contiguous arrays of a given
+dtype, copying only when needed, and their shapes are checked; box-like
+arguments become :class:`freud_amd.box.Box`.
+"""
+
+import functools
 
 import numpy as np
 
 import freud_amd.box
 
+# Text of the error raised by compute classes that need explicit neighbours
+# (freud/errors.py:15-18).
 NO_DEFAULT_QUERY_ARGS_MESSAGE = (
     "The {} class does not provide default query arguments. Query arguments "
     "or a neighbor list must be provided to this compute method."
 )
 
 
+class computed_property:
+    """A read-only property whose getter runs only after ``compute``.
+
+    The owning object records a completed ``compute`` call in
+    ``_called_compute`` (see :class:`_Compute`); reading the property before
+    that raises ``AttributeError``, as freud's properties do.
+    """
+
+    def __init__(self, getter):
+        self._getter = getter
+        functools.update_wrapper(self, getter)
+
+    def __set_name__(self, owner, name):
+        self._name = name
+
+    def __get__(self, obj, objtype=None):
+        if obj is None:
+            return self
+        if not obj.__dict__.get("_called_compute", False):
+            raise AttributeError(f"Property {getattr(self, '_name', '')!s} not computed. Call compute first.")
+        return self._getter(obj)
+
+    def __set__(self, obj, value):
+        raise AttributeError("computed properties are read-only")
+
+
 class _Compute:
-    """Guards computed properties until ``compute`` has run (freud/util.py:11-83)."""
+    """Base of the compute classes.
+
+    Every ``compute`` method defined by a subclass is wrapped when the class is
+    created: after it returns without raising, the instance is marked as
+    computed, which unlocks its :class:`computed_property` attributes.
+    """
 
     def __init__(self):
         self._called_compute = False
 
-    def __getattribute__(self, attr):
-        attribute = object.__getattribute__(self, attr)
-        if attr == "compute":
-            compute = attribute
+    def __init_subclass__(cls, **kwargs):
+        super().__init_subclass__(**kwargs)
+        fn = cls.__dict__.get("compute")
+        if fn is None or getattr(fn, "_marks_computed", False):
+            return
 
-            @wraps(compute)
-            def compute_wrapper(*args, **kwargs):
-                return_value = compute(*args, **kwargs)
-                self._called_compute = True
-                return return_value
+        @functools.wraps(fn)
+        def compute(self, *args, **kw):
+            out = fn(self, *args, **kw)
+            self.__dict__["_called_compute"] = True
+            return out
 
-            return compute_wrapper
-        return attribute
-
-    @staticmethod
-    def _computed_property(prop):
-        @property
-        @wraps(prop)
-        def wrapper(self, *args, **kwargs):
-            if not getattr(self, "_called_compute", False):
-                msg = "Property not computed. Call compute first."
-                raise AttributeError(msg)
-            return prop(self, *args, **kwargs)
-
-        return wrapper
+        compute._marks_computed = True
+        cls.compute = compute
 
     def __str__(self):
         return repr(self)
 
 
+def _check_shape(arr, shape):
+    if len(shape) != arr.ndim:
+        raise ValueError(f"expected an array with {len(shape)} dimension(s), got shape {arr.shape}")
+    for axis, (want, have) in enumerate(zip(shape, arr.shape)):
+        if want is not None and want != have:
+            expect = "(" + ", ".join("..." if s is None else str(s) for s in shape) + ")"
+            raise ValueError(f"axis {axis} of an array of shape {arr.shape} does not match the expected {expect}")
+
+
 def _convert_array(array, shape=None, dtype=np.float32, requirements=("C",), allow_copy=True):
-    """freud/util.py:86-141."""
+    """``array`` as an ndarray of ``dtype`` with the given flags (a copy only
+    when the input does not already qualify); ``shape`` entries of None match
+    any length.  None gives an uninitialised array of ``shape``.  With
+    ``allow_copy=False`` an input that would need a copy is rejected."""
     if array is None:
         return np.empty(shape, dtype=dtype)
-    array = np.asarray(array)
-    return_arr = np.require(array, dtype=dtype, requirements=requirements)
-    if not allow_copy and return_arr is not array:
-        msg = (
-            "The provided output array must have dtype "
-            f"{dtype}, and have the following array flags: {', '.join(requirements)}."
-        )
-        raise ValueError(msg)
+    src = np.asarray(array)
+    out = np.require(src, dtype=dtype, requirements=requirements)
+    if out is not src and not allow_copy:
+        raise ValueError(f"the output array must already be {np.dtype(dtype).name} with flags {', '.join(requirements)}")
     if shape is not None:
-        if return_arr.ndim != len(shape):
-            msg = f"array.ndim = {return_arr.ndim}; expected ndim = {len(shape)}"
-            raise ValueError(msg)
-        for i, s in enumerate(shape):
-            if s is not None and return_arr.shape[i] != s:
-                shape_str = "(" + ", ".join(str(i) if i is not None else "..." for i in shape) + ")"
-                msg = f"array.shape= {return_arr.shape}; expected shape = {shape_str}"
-                raise ValueError(msg)
-    return return_arr
+        _check_shape(out, shape)
+    return out
 
 
 def _convert_box(box, dimensions=None):
-    """freud/util.py:144-168."""
-    if not isinstance(box, freud_amd.box.Box):
-        box = freud_amd.box.Box.from_box(box)
-    if dimensions is not None and box.dimensions != dimensions:
-        msg = f"The box must be {dimensions}-dimensional."
-        raise ValueError(msg)
-    return box
+    """A :class:`freud_amd.box.Box` from any box-like value; with
+    ``dimensions`` (2 or 3) the box must have that many."""
+    out = box if isinstance(box, freud_amd.box.Box) else freud_amd.box.Box.from_box(box)
+    if dimensions is not None and out.dimensions != dimensions:
+        raise ValueError(f"The box must be {dimensions}-dimensional.")
+    return out

@@ -115,3 +115,51 @@ def _to_nlist(bonds):
         "distances": np.array([b[2] for b in bonds], dtype=f32),
         "vectors": np.array([b[3] for b in bonds], dtype=f32),
     }
+
+
+def distance_tie_points(box, formula="aabb", radius=1.3, seed=0, tries=200000):
+    """Two points A, B around a query point at the origin whose bond r_sq
+    differ by one float step while sqrtf gives both the same distance d: a
+    selection by (r_sq, j) and one by (d, j) then keep different points when
+    B, the farther in r_sq, has the lower index.  Returns (A, B, r_sq_A,
+    r_sq_B, d) with A, B float32 (3,) positions; the bond vector of each is
+    p - 0 (AABB formula, image 0) or box.wrap(p) (LinkCell formula)."""
+    rng = np.random.default_rng(seed)
+
+    def rsq_of(p):
+        v = p[None, :] if formula == "aabb" else box.wrap(p[None, :])
+        return dot3(v.astype(f32))[0]
+
+    ang = rng.random(tries) * 2 * np.pi
+    cand = np.zeros((tries, 3), f32)
+    cand[:, 0] = (radius * np.cos(ang)).astype(f32)
+    cand[:, 1] = (radius * np.sin(ang)).astype(f32)
+    r = np.array([rsq_of(c) for c in cand[:4000]], f32)
+    by_rsq = {}
+    for c, x in zip(cand[:4000], r):
+        by_rsq.setdefault(x.view(np.uint32).item(), c)
+    for bits, a in sorted(by_rsq.items()):
+        nb = bits + 1
+        if nb in by_rsq:
+            ra, rb = np.uint32(bits).view(f32), np.uint32(nb).view(f32)
+            if np.sqrt(ra) == np.sqrt(rb):
+                return a, by_rsq[nb], ra, rb, np.sqrt(ra)
+    raise RuntimeError("no distance tie found")
+
+
+def knn_tie_system(box, formula="aabb", n_fill=300, seed=0):
+    """Points for a kNN distance tie at the k-th slot of query point 0 (at the
+    origin): 1 = B and 2 = A of distance_tie_points (equal d, r_sq of B one
+    step above A's), 3 and 4 nearer (d = 0.5 and 0.8), then n_fill points
+    farther than 4 from the origin.  Query 0 keeps {B, 3, 4} for k = 3 under
+    the (d, j) rule; an (r_sq, j) rule would keep A instead of B."""
+    a, b, ra, rb, d = distance_tie_points(box, formula, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    fill = []
+    while len(fill) < n_fill:
+        p = ((rng.random(3) - 0.5) * np.array([box.Lx, box.Ly, box.Lz], f32)).astype(f32)
+        if np.linalg.norm(p) > 4.0:
+            fill.append(p)
+    pts = np.concatenate([np.zeros((1, 3), f32), b[None], a[None], np.array([[0, 0, 0.5], [0, 0, -0.8]], f32),
+                          np.array(fill, f32)]).astype(f32)
+    return pts

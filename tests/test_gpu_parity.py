@@ -105,6 +105,26 @@ def test_knn(oracle, name, box, pts, k):
     _assert_nlist_equal(nl, ref)
 
 
+@pytest.mark.parametrize("kind", ["aabb", "linkcell"])
+def test_knn_distance_tie(oracle, kind):
+    """A constructed tie at the k-th slot: two candidates with r_sq one float
+    step apart and the same sqrtf distance.  The selection follows (d, j), as
+    the reference's sort by distance does, so the lower index is kept."""
+    import brute
+
+    box = Box.cube(20.0)
+    pts = brute.knn_tie_system(box, kind)
+    nq = freud_amd.AABBQuery(box, pts) if kind == "aabb" else freud_amd.LinkCell(box, pts, cell_width=2.0)
+    for k in (3, 4):
+        for by_d in (False, True):
+            qa = dict(num_neighbors=k, exclude_ii=True)
+            nl = nq.query(pts, qa).toNeighborList(sort_by_distance=by_d)
+            ref = oracle.query(_bt(box), pts, pts, qa, nq_type=kind, cell_width=2.0, sort_by_distance=by_d)
+            _assert_nlist_equal(nl, ref)
+    nl = nq.query(pts, dict(num_neighbors=3, exclude_ii=True)).toNeighborList()
+    assert sorted(nl.point_indices[nl.query_point_indices == 0].tolist()) == [1, 3, 4]
+
+
 @pytest.mark.parametrize("name,box,pts", SYSTEMS[:2], ids=[s[0] for s in SYSTEMS[:2]])
 def test_knn_r_min(oracle, name, box, pts):
     """kNN with r_min > 0, including r_min equal to an existing bond length

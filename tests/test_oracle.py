@@ -495,3 +495,26 @@ def test_all_pairs_reference_cases(oracle):
     npt.assert_allclose(edge["vectors"], [[-1, 0, 0]], atol=1e-6)
     assert edge["vectors"][0, 0] < 0
     npt.assert_allclose(edge["distances"], [1.0], atol=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["aabb", "linkcell"])
+def test_knn_distance_tie(oracle, kind):
+    """Two candidates whose r_sq differ by one float step but whose sqrtf
+    distances are equal: the kNN selection follows the distance (the
+    reference sorts bonds by d, AABBQuery.cc:227,246, LinkCell.cc:656), so
+    the lower index wins the k-th slot although its r_sq is larger."""
+    box = Box.cube(20.0)
+    pts = brute.knn_tie_system(box, kind)
+    nl3 = oracle.query(_box_tuple(box), pts, pts, dict(num_neighbors=3, exclude_ii=True), nq_type=kind,
+                       cell_width=2.0)
+    row = nl3["ij"][:, 0] == 0
+    assert sorted(nl3["ij"][row, 1].tolist()) == [1, 3, 4]
+    # the premise: bond 0 -> 2 has the smaller r_sq and the same distance
+    full = oracle.query(_box_tuple(box), pts, pts, dict(r_max=1.5, exclude_ii=True), nq_type=kind, cell_width=2.0)
+    sel = full["ij"][:, 0] == 0
+    byj = dict(zip(full["ij"][sel, 1].tolist(), range(int(sel.sum()))))
+    r = full["vectors"][sel]
+    rsq = brute.dot3(r)
+    d = full["distances"][sel]
+    assert rsq[byj[2]] < rsq[byj[1]] and d[byj[2]] == d[byj[1]]
+    npt.assert_array_equal(nl3["vectors"][row][0], r[byj[1]])
