@@ -99,7 +99,9 @@ class Frame:
 
 def algorithmic_bytes(name, n, b_ball, b_knn, k=12, l=6, bins=200):
     """Compulsory HBM bytes per launch of each kernel (inputs read once,
-    outputs written once)."""
+    outputs written once).  The averaged q_lm also gathers 8(2l+1) bytes of
+    neighbour q_lm per bond, mostly from cache; those re-reads are not
+    compulsory and are not counted here."""
     nm = 2 * l + 1
     table = {
         "ball_count_kernel": 12 * n + 12 * n + 4 * n,
@@ -109,7 +111,7 @@ def algorithmic_bytes(name, n, b_ball, b_knn, k=12, l=6, bins=200):
         "rdf_self_kernel": 16 * n + 4 * bins,
         "rdf_cluster_kernel": 16 * n + 4 * bins,
         "qlm_kernel": 24 * b_ball + 8 * nm * n + 4 * n,
-        "qlm_ave_kernel": 8 * b_ball + 8 * nm * b_ball + 8 * nm * n + 4 * n,
+        "qlm_ave_kernel": 4 * b_ball + 4 * (n + 1) + 8 * nm * n + 8 * nm * n + 4 * n,
         "wl_kernel": 8 * nm * n + 4 * n + 4 * n,
         "grid_assign_kernel": 12 * n + 4 * n,
         "grid_scatter_kernel": 12 * n + 4 * n + 16 * n + 4 * n,
@@ -196,32 +198,46 @@ def qlm_roofline(kernel, avg_ms, bonds, n, ls):
             "compulsory_hbm_gbs": round(hbm / (avg_ms * 1e-3) / 1e9, 1)}
 
 
+def _oracle_frame(oracle, bt, pts, q, cores):
+    """The frame's pipeline on the oracle for query points q."""
+    nl = oracle.query(bt, pts, q, dict(r_max=1.5, exclude_ii=True), nthreads=cores)
+    knn = oracle.query(bt, pts, q, dict(num_neighbors=12, exclude_ii=True), nthreads=cores)
+    counts = oracle.rdf_accumulate(bt, pts, q, 200, 5.0, exclude_ii=True, nthreads=cores)
+    st_w = oracle.steinhardt(bt, pts, nl, [6], wl=True, nthreads=cores)
+    st_a = oracle.steinhardt(bt, pts, nl, [6], average=True, nthreads=cores)
+    return nl, knn, counts, st_w, st_a
+
+
 def cpu_baseline(box, pts, budget_s=15.0):
     """The oracle (C++ restatement of the reference's AABB tree, thread-local
-    histograms and Steinhardt loops) on a bounded sample of the query points,
-    on all host cores."""
+    histograms and Steinhardt loops) on the frame: on all host cores over the
+    whole frame when it finishes within the budget (else over a prefix of the
+    query points), and on one thread over a prefix of the query points.
+    Returns the baseline record and, when the whole frame ran, the oracle's
+    outputs (the checker of the GPU frame)."""
     import oracle
 
     oracle.build()
     n = len(pts)
     cores = os.cpu_count() or 1
     bt = (box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D)
-    s = 2048
-    elapsed = 0.0
-    while True:
-        q = pts[:s]
-        t0 = time.perf_counter()
-        nl = oracle.query(bt, pts, q, dict(r_max=1.5, exclude_ii=True), nthreads=cores)
-        oracle.query(bt, pts, q, dict(num_neighbors=12, exclude_ii=True), nthreads=cores)
-        counts = oracle.rdf_accumulate(bt, pts, q, 200, 5.0, exclude_ii=True, nthreads=cores)
-        oracle.steinhardt(bt, pts, nl, [6], wl=True, nthreads=cores)
-        oracle.steinhardt(bt, pts, nl, [6], average=True, nthreads=cores)
-        elapsed = time.perf_counter() - t0
-        if elapsed > budget_s / 4 or s * 2 > n:
-            break
-        s *= 2
-    pairs = len(nl["ij"]) + int(counts.astype(np.int64).sum())
-    return {
+
+    def timed(threads, budget):
+        s = 2048
+        while True:
+            q = pts[:s]
+            t0 = time.perf_counter()
+            res = _oracle_frame(oracle, bt, pts, q, threads)
+            elapsed = time.perf_counter() - t0
+            if elapsed > budget / 4 or s >= n:
+                return s, elapsed, res
+            s = min(s * 2, n)
+
+    s, elapsed, res = timed(cores, budget_s)
+    pairs = len(res[0]["ij"]) + int(res[2].astype(np.int64).sum())
+    s1, elapsed1, res1 = timed(1, budget_s)
+    pairs1 = len(res1[0]["ij"]) + int(res1[2].astype(np.int64).sum())
+    rec = {
         "value": pairs / elapsed,
         "unit": "pairs/s",
         "cores": cores,
@@ -229,7 +245,58 @@ def cpu_baseline(box, pts, budget_s=15.0):
         "sample": (f"{s} of {n} query points of the same frame (all {n} points searched), full pipeline "
                    f"(AABB tree build + ball r=1.5 + kNN k=12 + RDF r=5 + q6/w6/avg-q6) in {elapsed:.2f} s on "
                    f"{cores} threads of {_cpu_model()}"),
+        "single_thread": {"value": pairs1 / elapsed1, "unit": "pairs/s", "cores": 1,
+                          "sample": f"{s1} of {n} query points, same pipeline, {elapsed1:.2f} s on 1 thread"},
     }
+    return rec, (res if s == n else None)
+
+
+def frame_parity(fr, box, pts, ref):
+    """The timed frame's GPU outputs against the oracle's run of the same
+    frame: NeighborLists and RDF counts bit for bit, q6 / w6 / averaged q6 as
+    max |error| against the oracle's float32 and float64 evaluations."""
+    import oracle
+    from freud_amd._lib import copy_d2h
+
+    nl, knn, counts, st_w, st_a = ref
+    cores = os.cpu_count() or 1
+    bt = (box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D)
+    n = len(pts)
+
+    def same_nlist(c, r):
+        b = int(c.num_bonds)
+        if b != len(r["ij"]):
+            return False
+        return bool(np.array_equal(copy_d2h(c.neighbors, (b, 2), np.uint32), r["ij"])
+                    and np.array_equal(copy_d2h(c.distances, (b,), np.float32), r["distances"])
+                    and np.array_equal(copy_d2h(c.vectors, (b, 3), np.float32), r["vectors"]))
+
+    ql = fr.d_ql.numpy().astype(np.float64)
+    qlave = fr.d_qlave.numpy().astype(np.float64)
+    wl = fr.d_wl.numpy().astype(np.float64)
+    w64 = oracle.steinhardt(bt, pts, nl, [6], wl=True, f64=True, nthreads=cores)
+    a64 = oracle.steinhardt(bt, pts, nl, [6], average=True, f64=True, nthreads=cores)
+
+    def err(a, b):
+        return float(np.max(np.abs(a - b)))
+
+    out = {
+        "ball_nlist_bit_exact": same_nlist(fr.nl_ball, nl),
+        "knn_nlist_bit_exact": same_nlist(fr.nl_knn, knn),
+        "rdf_counts_bit_exact": bool(np.array_equal(fr.counts.numpy(), counts)),
+        "q6_max_err_vs_f32": err(ql, st_a["ql"][:, 0]),
+        "q6_max_err_vs_f64": err(ql, a64["ql"][:, 0]),
+        "w6_max_err_vs_f32": err(wl, st_w["wl"][:, 0]),
+        "w6_max_err_vs_f64": err(wl, w64["wl"][:, 0]),
+        "avg_q6_max_err_vs_f32": err(qlave, st_a["ql_ave"][:, 0]),
+        "avg_q6_max_err_vs_f64": err(qlave, a64["ql_ave"][:, 0]),
+        "oracle_f32_q6_err_vs_f64": err(st_a["ql"][:, 0], a64["ql"][:, 0]),
+        "n_points": n,
+    }
+    out["pass"] = bool(out["ball_nlist_bit_exact"] and out["knn_nlist_bit_exact"] and out["rdf_counts_bit_exact"]
+                       and max(out["q6_max_err_vs_f64"], out["w6_max_err_vs_f64"],
+                               out["avg_q6_max_err_vs_f64"]) <= 1e-5)
+    return out
 
 
 class FrameLoop:
@@ -665,12 +732,16 @@ def main():
     for name, (cnt, tot) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
         avg = tot / cnt
         ab = algorithmic_bytes(name, n, b_ball, b_knn)
+        mb = measured_traffic(name)
         kernels[name] = {
             "launches_per_step": cnt / args.steps,
             "avg_ms": round(avg, 4),
             "share": round(tot / (ms_per_step * args.steps), 4),
             "algorithmic_bytes": ab,
-            "hbm_frac": (None if ab is None else round(ab / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+            "algorithmic_frac": (None if ab is None else round(ab / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
+            # HBM bytes from the FETCH_SIZE + WRITE_SIZE counters of a profiled run of this workload
+            "measured_bytes": mb,
+            "hbm_frac": (None if mb is None else round(mb / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
         }
     # dominant kernel by time
     dom = max(stats.items(), key=lambda kv: kv[1][1])[0]
@@ -686,9 +757,14 @@ def main():
                 "avg_ms": round(avg_ms, 4)}
 
     cpu = None
+    parity = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(box, pts)
+        cpu, ref = cpu_baseline(box, pts)
+        if ref is not None:
+            print("[bench] parity of the timed frame against the oracle ...", file=sys.stderr, flush=True)
+            parity = frame_parity(fr, box, pts, ref)
+            print(f"[bench] parity: {parity}", file=sys.stderr, flush=True)
 
     out = {
         "metric": "pairs/s (query+RDF) & particles/s (SolidLiquid), plain vs batched/learned",
@@ -715,6 +791,7 @@ def main():
         "roofline": roof,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "parity": parity,
     }
     print(json.dumps(out))
 

@@ -268,6 +268,26 @@ def test_rdf_lj_golden():
     npt.assert_allclose(rdf.rdf, gold, rtol=1e-5, atol=1e-5)
 
 
+def test_reader_integration_dcd(oracle):
+    # tests/integration/test_reader_integrations.py:33-40,50-53 of the reference on
+    # lj.dcd: RDF over all frames (reset=False) equals the golden, and
+    # Steinhardt(6) over the 6 nearest neighbours runs on every frame
+    from freud_amd.dcd import open_dcd
+
+    traj = open_dcd(os.path.join(GOLDEN, "lj.dcd"))
+    rdf = freud_amd.density.RDF(bins=100, r_max=5)
+    ql = freud_amd.order.Steinhardt(6)
+    for fr in traj:
+        rdf.compute(fr, reset=False)
+        ql.compute(fr, neighbors={"num_neighbors": 6})
+    npt.assert_allclose(rdf.rdf, np.load(os.path.join(GOLDEN, "rdf.npz"))["rdf"], rtol=1e-5, atol=1e-5)
+    b = fr.box
+    bt = (b.Lx, b.Ly, b.Lz, b.xy, b.xz, b.yz, b.is2D)
+    ref_nl = oracle.query(bt, fr.positions, fr.positions, dict(num_neighbors=6, exclude_ii=True), nthreads=8)
+    ref = oracle.steinhardt(bt, fr.positions, ref_nl, [6], nthreads=8)
+    npt.assert_allclose(ql.particle_order, ref["particle_order"][:, 0], atol=1e-5)
+
+
 def test_rdf_nlist_and_query_points(oracle):
     rng = np.random.default_rng(7)
     box = Box.cube(12.0)

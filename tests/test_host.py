@@ -140,6 +140,37 @@ def test_gsd_reader():
     assert fr.positions.shape == (3288, 3)
 
 
+def test_dcd_reader():
+    """lj.dcd and lj.gsd hold the same HOOMD run (tests/integration/files/lj/lj.py
+    of the reference dumps both every 1000 steps): every DCD frame must equal
+    the GSD frame bit for bit, with the 20 x 20 x 20 cubic box."""
+    from freud_amd.dcd import open_dcd
+    from freud_amd.gsd import open_gsd
+
+    from conftest import GOLDEN
+
+    d = open_dcd(os.path.join(GOLDEN, "lj.dcd"))
+    g = open_gsd(os.path.join(GOLDEN, "lj.gsd"))
+    assert len(d) == len(g) == 10 and d.n_atoms == 1000
+    assert d.title[0] == "Created by HOOMD"
+    for f in range(len(d)):
+        a, b = d[f], g[f]
+        assert a.step == b.step == 10000 + 1000 * f
+        npt.assert_array_equal(a.positions, b.positions)
+        assert a.cell == (20.0, 20.0, 20.0, 90.0, 90.0, 90.0)
+        assert (a.box.Lx, a.box.Ly, a.box.Lz, a.box.xy, a.box.xz, a.box.yz, a.box.is2D) == (20, 20, 20, 0, 0, 0,
+                                                                                            False)
+    with pytest.raises(IndexError):
+        d[10]
+    # a triclinic cell from lengths and angles: |a| = A, |b| = B, |c| = C, and the angles between them
+    from freud_amd.dcd import cell_vectors
+
+    m = cell_vectors(10.0, 11.0, 12.0, 80.0, 85.0, 95.0)
+    npt.assert_allclose(np.linalg.norm(m, axis=1), [10, 11, 12])
+    cos = lambda u, v: np.dot(u, v) / np.linalg.norm(u) / np.linalg.norm(v)  # noqa: E731
+    npt.assert_allclose(np.degrees(np.arccos([cos(m[1], m[2]), cos(m[0], m[2]), cos(m[0], m[1])])), [80, 85, 95])
+
+
 def test_rdf_bin_table_is_exact(oracle):
     """The RDF kernels bin r_sq through a host-built table (thresholds + lookup);
     emulate the device arithmetic on many r_sq values and compare with the

@@ -166,6 +166,20 @@ def test_rdf_lj_golden(oracle):
     npt.assert_allclose(n_r, np.cumsum(counts.astype(np.float64)) / (1000 * len(traj)), rtol=1e-5)
 
 
+def test_rdf_lj_golden_dcd(oracle):
+    # the same golden through the DCD file of the run (test_reader_integrations.py:28,50)
+    from freud_amd.dcd import open_dcd
+
+    counts = np.zeros(100, np.uint32)
+    traj = open_dcd(os.path.join(GOLDEN, "lj.dcd"))
+    for fr in traj:
+        b = fr.box
+        box = (b.Lx, b.Ly, b.Lz, b.xy, b.xz, b.yz, b.is2D)
+        oracle.rdf_accumulate(box, fr.positions, fr.positions, 100, 5.0, exclude_ii=True, counts=counts, nthreads=4)
+    rdf, _ = oracle.rdf_reduce(box, counts, 100, 0.0, 5.0, 1000, 1000, len(traj))
+    npt.assert_allclose(rdf, np.load(os.path.join(GOLDEN, "rdf.npz"))["rdf"], rtol=1e-5, atol=1e-5)
+
+
 def test_rdf_bin_edges(oracle):
     # tests/test_density_rdf.py:242-251 of the reference
     for bins, r_min, r_max in [(100, 0.0, 4.0), (37, 0.5, 3.3), (200, 0.0, 5.0)]:
