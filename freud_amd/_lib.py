@@ -159,6 +159,9 @@ _SIGS = {
     "fa_rdf_accumulate_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
                                                ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_float,
                                                ctypes.c_float, ctypes.c_void_p]),
+    "fa_rdf_accumulate_self_noalloc": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32,
+                                                      ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                                      ctypes.c_float, ctypes.c_float, ctypes.c_void_p]),
     "fa_rdf_bin_table": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
                                         ctypes.c_uint64]),
     "fa_rdf_accumulate_nlist": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_uint32, ctypes.c_float,

@@ -448,6 +448,31 @@ struct Grid
 // wide along each plane normal (dims clamped to >= 1, z = 1 in 2D).
 int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, const int dims[3]);
 
+// The same build into device buffers the caller provides (nothing is
+// allocated; counters are cleared by kernels, not hipMemsetAsync).  Sizes:
+// cpos n + kCposPad, wrapc / cell_of_point / rank_of_point / wrap_of_point n,
+// cstart ncell + 1, counts ncell, outside one int, scan_tmp
+// scan_tmp_bytes(ncell) bytes.
+struct GridBuffers
+{
+    float4* cpos;
+    int* wrapc;
+    uint32_t* cell_of_point;
+    uint32_t* cstart;
+    int* outside;
+    uint32_t* counts;
+    uint32_t* rank_of_point;
+    int* wrap_of_point;
+    void* scan_tmp;
+    size_t scan_bytes;
+};
+int build_grid_into(const GridBuffers& B, const DevBox& db, const float* d_points, uint32_t n, const int dims[3]);
+
+// Looks up, for the current device, the address of the statically allocated
+// scratch array of the no-allocation RDF entry point (called once per device
+// by the library's initialisation, with that device current).
+int noalloc_setup();
+
 // A processing order of the points that follows space (cells of about four
 // points, counting-sorted), for the averaged q_lm, which gathers whole q_lm
 // rows of the neighbours: threads that run together then reuse each other's

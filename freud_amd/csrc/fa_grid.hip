@@ -385,4 +385,30 @@ int build_grid(Grid& g, const fa_box& box, const float* d_points, uint32_t n, co
     return FA_OK;
 }
 
+int build_grid_into(const GridBuffers& B, const DevBox& db, const float* d_points, uint32_t n, const int dims[3])
+{
+    hipStream_t s = stream();
+    uint32_t const ncell = uint32_t(dims[0]) * uint32_t(dims[1]) * uint32_t(dims[2]);
+    if (int rc = zero_u32(B.counts, ncell))
+        return rc;
+    if (int rc = zero_u32(reinterpret_cast<uint32_t*>(B.outside), 1))
+        return rc;
+    unsigned const blocks = (n + 255) / 256;
+    if (n)
+    {
+        FA_KLAUNCH(grid_assign_kernel, dim3(blocks), dim3(256), 0, s, db, d_points, n, dims[0], dims[1], dims[2],
+                   B.cell_of_point, B.wrap_of_point, B.counts, B.outside, B.rank_of_point);
+        FA_HIP(hipGetLastError());
+    }
+    if (int rc = exclusive_scan_u32_async(B.counts, B.cstart, ncell, B.scan_tmp, B.scan_bytes))
+        return rc;
+    if (n)
+    {
+        FA_KLAUNCH(grid_scatter_kernel, dim3(blocks), dim3(256), 0, s, d_points, n, B.cell_of_point,
+                   B.wrap_of_point, B.cstart, B.rank_of_point, B.cpos, B.wrapc);
+        FA_HIP(hipGetLastError());
+    }
+    return FA_OK;
+}
+
 } // namespace fa

@@ -931,60 +931,68 @@ struct RowVariant
 constexpr RowVariant kRowVariant[] = {{2, 1}, {3, 1}, {4, 1}, {2, 2}, {2, 4}, {3, 2}, {2, 8}, {3, 4}, {2, 16}};
 constexpr int kRowVariants = int(sizeof(kRowVariant) / sizeof(kRowVariant[0]));
 
-// Self RDF through the row-cluster kernel with cells at least r_max / R wide
-// (r_max / (R F) along x).  Sets *handled = false (and does nothing) when the
-// kernel does not apply: points outside the box, or rows too long for the
-// cluster encoding.
-int run_rows(const RowsArgs& a, RowVariant rv, uint32_t* d_counts, bool* handled)
+// Grid shape of the row-cluster self RDF for variant rv: cells at least
+// r_max / R wide across the rows, F times narrower along x (F falls back to 1
+// when the refined grid would hold more than 8 cells per point).  Returns
+// false when the kernel does not apply (rows too long for the cluster
+// encoding, or too many points for 32-bit chunk offsets).
+bool rows_grid_dims(const fa_box& box, uint32_t n, float q_r_max, RowVariant rv, int dims[3], int* F_out)
 {
-    *handled = false;
-    fa_nq* nq = a.nq;
-    hipStream_t s = fa::stream();
-    int const R = rv.R;
-    int dims[3];
-    fa::grid_dims_for(nq->box, nq->n, a.q_r_max, R, dims);
+    fa::grid_dims_for(box, n, q_r_max, rv.R, dims);
     int F = rv.F;
-    if (dims[0] * F > 65535 || double(dims[0]) * F * dims[1] * dims[2] > std::max<double>(8.0 * nq->n, 64.0))
+    if (dims[0] * F > 65535 || double(dims[0]) * F * dims[1] * dims[2] > std::max<double>(8.0 * n, 64.0))
         F = 1;
     dims[0] *= F;
+    *F_out = F;
     // cluster entries hold cell x in 16 bits; chunk loads use 32-bit byte offsets
-    if (dims[0] > 65535 || uint64_t(nq->n) + fa::kCposPad > (1ull << 28))
-        return FA_OK;
-    if (int rc = fa::ensure_grid(nq, dims))
-        return rc;
-    *handled = true;
-    DevGrid const g = nq->cur->dev();
-    DevBox const& db = nq->cur->dbox;
+    return !(dims[0] > 65535 || uint64_t(n) + fa::kCposPad > (1ull << 28));
+}
+
+// Row-cluster buffers of a grid with nrows rows over n points: cluster counts
+// and offsets (nrows + 1 each), the cluster table (max_clusters uint4) and
+// the scan's temporary storage.
+struct RowsBuffers
+{
+    uint32_t* nclus;
+    uint32_t* cl_off;
+    uint4* table;
+    void* scan_tmp; // nullptr: the library's scan scratch
+    size_t scan_bytes;
+};
+inline uint64_t rows_max_clusters(uint32_t n, uint32_t nrows) { return uint64_t(n) / 8 + nrows; }
+
+// The cluster table of a built grid, then the row-cluster kernel.
+int launch_rows(const DevBox& db, const DevGrid& g, const int dims[3], int R, int F, uint32_t n,
+                const RowsBuffers& B, float qmax_sq, float qmin_sq, const float* T, uint32_t bins, float r_min,
+                float inv_width, uint32_t* d_counts)
+{
+    hipStream_t s = fa::stream();
     uint32_t const nrows = uint32_t(dims[1]) * uint32_t(dims[2]);
-    uint64_t const max_clusters = uint64_t(nq->n) / 8 + nrows;
-    void* tmp = nullptr;
-    size_t const rb = sizeof(uint32_t) * (size_t(nrows) + 1);
-    if (int rc = fa::scratch(13, 2 * rb + sizeof(uint4) * max_clusters + 256, &tmp))
-        return rc;
-    uint32_t* nclus = static_cast<uint32_t*>(tmp);
-    uint32_t* cl_off = nclus + (nrows + 1);
-    uint4* table = reinterpret_cast<uint4*>(static_cast<char*>(tmp) + ((2 * rb + 15) & ~size_t(15)));
+    uint64_t const max_clusters = rows_max_clusters(n, nrows);
     unsigned const rblocks = (nrows + kBlock - 1) / kBlock;
     FA_KLAUNCH(row_cluster_count_kernel, dim3(rblocks), dim3(kBlock), 0, s, g.cstart, uint32_t(dims[0]), nrows,
-               nclus);
-    if (int rc = fa::exclusive_scan_u32(nclus, cl_off, nrows, nullptr))
+               B.nclus);
+    if (B.scan_tmp)
+    {
+        if (int rc = fa::exclusive_scan_u32_async(B.nclus, B.cl_off, nrows, B.scan_tmp, B.scan_bytes))
+            return rc;
+    }
+    else if (int rc = fa::exclusive_scan_u32(B.nclus, B.cl_off, nrows, nullptr))
         return rc;
     unsigned const tblocks = unsigned((max_clusters + kBlock - 1) / kBlock);
     FA_KLAUNCH(row_cluster_table_kernel, dim3(tblocks), dim3(kBlock), 0, s, g.cstart, uint32_t(dims[0]), nrows,
-               cl_off, uint32_t(max_clusters), table);
+               B.cl_off, uint32_t(max_clusters), B.table);
     unsigned const wblocks = unsigned((max_clusters + kBlock / 64 - 1) / (kBlock / 64));
-    bool const lds = a.bins <= kMaxLdsBins;
-    size_t const shmem = lds ? sizeof(float) * lds_table_floats(a.bins) + sizeof(uint32_t) * a.bins : 0;
+    bool const lds = bins <= kMaxLdsBins;
+    size_t const shmem = lds ? sizeof(float) * lds_table_floats(bins) + sizeof(uint32_t) * bins : 0;
 #define FA_ROWS(RR)                                                                                                 \
     case RR:                                                                                                        \
         if (lds)                                                                                                    \
-            FA_KLAUNCH((rdf_rows_kernel<RR, true>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,            \
-                       cl_off + nrows, R * F, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,  \
-                       a.inv_width, d_counts);                                                                      \
+            FA_KLAUNCH((rdf_rows_kernel<RR, true>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, B.table,          \
+                       B.cl_off + nrows, R * F, qmax_sq, qmin_sq, T, bins, r_min, inv_width, d_counts);            \
         else                                                                                                        \
-            FA_KLAUNCH((rdf_rows_kernel<RR, false>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, table,           \
-                       cl_off + nrows, R * F, a.qmax_sq, a.qmin_sq, a.T, a.bins, a.r_min,  \
-                       a.inv_width, d_counts);                                                                      \
+            FA_KLAUNCH((rdf_rows_kernel<RR, false>), dim3(wblocks), dim3(kBlock), shmem, s, db, g, B.table,         \
+                       B.cl_off + nrows, R * F, qmax_sq, qmin_sq, T, bins, r_min, inv_width, d_counts);            \
         break;
     switch (R)
     {
@@ -997,6 +1005,34 @@ int run_rows(const RowsArgs& a, RowVariant rv, uint32_t* d_counts, bool* handled
 #undef FA_ROWS
     FA_HIP(hipGetLastError());
     return FA_OK;
+}
+
+// Self RDF through the row-cluster kernel with cells at least r_max / R wide
+// (r_max / (R F) along x).  Sets *handled = false (and does nothing) when the
+// kernel does not apply: rows too long for the cluster encoding, or too many
+// points (points outside the box are handled inside the kernel).
+int run_rows(const RowsArgs& a, RowVariant rv, uint32_t* d_counts, bool* handled)
+{
+    *handled = false;
+    fa_nq* nq = a.nq;
+    int dims[3], F = 1;
+    if (!rows_grid_dims(nq->box, nq->n, a.q_r_max, rv, dims, &F))
+        return FA_OK;
+    if (int rc = fa::ensure_grid(nq, dims))
+        return rc;
+    *handled = true;
+    uint32_t const nrows = uint32_t(dims[1]) * uint32_t(dims[2]);
+    uint64_t const max_clusters = rows_max_clusters(nq->n, nrows);
+    void* tmp = nullptr;
+    size_t const rb = sizeof(uint32_t) * (size_t(nrows) + 1);
+    if (int rc = fa::scratch(13, 2 * rb + sizeof(uint4) * max_clusters + 256, &tmp))
+        return rc;
+    RowsBuffers B{};
+    B.nclus = static_cast<uint32_t*>(tmp);
+    B.cl_off = B.nclus + (nrows + 1);
+    B.table = reinterpret_cast<uint4*>(static_cast<char*>(tmp) + ((2 * rb + 15) & ~size_t(15)));
+    return launch_rows(nq->cur->dbox, nq->cur->dev(), dims, rv.R, F, nq->n, B, a.qmax_sq, a.qmin_sq, a.T, a.bins,
+                       a.r_min, a.inv_width, d_counts);
 }
 
 // ------------------------------------------------------------------ batches
@@ -1228,6 +1264,59 @@ size_t rdf_table_floats(uint32_t bins)
 
 } // namespace fa
 
+// ------------------------------------------------------------------ no-allocation self RDF
+// fa_rdf_accumulate_self_noalloc keeps every device buffer it needs (bin
+// table, cell grid, cluster table, scan storage) in the statically allocated
+// array fa_noalloc_scratch: each GPU the library runs on holds its own copy,
+// from module load to the end of the process.  Host code never passes the
+// array's name to a kernel (in host code it names a host-side stand-in); it
+// uses the device address that noalloc_setup looks up once per device.
+namespace {
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr uint32_t kNoAllocMaxPoints = 1u << 20;
+constexpr size_t kNoAllocTableBytes = size_t(32) << 10; // table_floats(kMaxLdsBins) floats
+static_assert(sizeof(float) * table_floats(kMaxLdsBins) <= kNoAllocTableBytes, "bin table region too small");
+constexpr int kNoAllocMaxDevices = 64;
+} // namespace
+
+// External linkage on purpose: this library is built without relocatable
+// device code, and a `static` device variable then has a local symbol in the
+// code object, which the HIP runtime cannot resolve by name (the lookup
+// aborts).  `used` keeps the array in the code object although no kernel
+// names it (the kernels receive its address as an argument).
+__device__ __attribute__((used)) __align__(256) unsigned char fa_noalloc_scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocDevice
+{
+    char* base = nullptr; // device address of fa_noalloc_scratch on this device
+    bool table_valid = false;
+    uint32_t bins = 0;
+    float r_min = 0.0f, r_max = 0.0f;
+};
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+} // namespace
+
+namespace fa {
+int noalloc_setup()
+{
+    int dev = 0;
+    FA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(FA_ERR_RUNTIME, "device index beyond the no-allocation scratch table");
+    void* ptr = nullptr;
+    // qualified: inside namespace fa the unqualified name would find other declarations first
+    hipError_t const e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::fa_noalloc_scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(FA_ERR_DEVICE, std::string("hipGetSymbolAddress(fa_noalloc_scratch): ") + hipGetErrorString(e));
+    g_noalloc[dev].base = static_cast<char*>(ptr);
+    g_noalloc[dev].table_valid = false;
+    return FA_OK;
+}
+} // namespace fa
+
 extern "C" {
 
 int fa_rdf_bin_table(uint32_t bins, float r_min, float r_max, float* out, uint64_t cap)
@@ -1351,6 +1440,127 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
 #undef FA_RDF_LAUNCH
     FA_HIP(hipGetLastError());
     return FA_OK;
+}
+
+int fa_rdf_accumulate_self_noalloc(const fa_box* box, const float* d_points, uint32_t n, float q_r_max,
+                                   float q_r_min, uint32_t bins, float r_min, float r_max, uint32_t* d_counts)
+{
+    if (int rc = fa_init())
+        return rc;
+    int dev = 0;
+    FA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices || !g_noalloc[dev].base)
+        return fa::fail(FA_ERR_RUNTIME, "the no-allocation scratch is not set up on the current device");
+    NoAllocDevice& D = g_noalloc[dev];
+    if (!box || !d_counts || !d_points)
+        return fa::fail(FA_ERR_INVALID, "box, points and counts must not be null");
+    // NeighborQuery.h:97-100 and the AABBQuery ball-query checks
+    if (n == 0)
+        return fa::fail(FA_ERR_INVALID, "Cannot create a NeighborQuery with 0 particles.");
+    if (n > kNoAllocMaxPoints)
+        return fa::fail(FA_ERR_INVALID, "fa_rdf_accumulate_self_noalloc takes at most 1048576 points");
+    if (bins == 0 || bins > kMaxLdsBins || !(r_max > r_min))
+        return fa::fail(FA_ERR_INVALID, "fa_rdf_accumulate_self_noalloc needs 1 <= bins <= 4096 and r_max > r_min");
+    fa_nq shape{};
+    shape.box = *box;
+    shape.d_points = d_points;
+    shape.n = n;
+    shape.kind = FA_NQ_AABB;
+    fa_query_args a{FA_MODE_BALL, 0xffffffffu, q_r_max, q_r_min, -1.0f, -1.0f, 1};
+    if (int rc = fa::validate_query_args(&shape, a, n))
+        return rc;
+
+    // launch shape: the tuner's recorded cell shape for this problem class, else (R, F) = (2, 8)
+    double const density = double(n) / double(fa::box_volume(*box));
+    int variant = fa::tuned_variant(fa::TUNE_RDF_SELF, fa::tune_key(density, q_r_max));
+    if (variant < 0)
+        variant = 6;
+    RowVariant const rv = kRowVariant[variant % kRowVariants];
+    int dims[3], F = 1;
+    if (!rows_grid_dims(*box, n, q_r_max, rv, dims, &F))
+        return fa::fail(FA_ERR_INVALID, "cell grid rows too long for fa_rdf_accumulate_self_noalloc");
+    uint64_t const ncell = uint64_t(dims[0]) * uint64_t(dims[1]) * uint64_t(dims[2]);
+    uint32_t const nrows = uint32_t(dims[1]) * uint32_t(dims[2]);
+    uint64_t const max_clusters = rows_max_clusters(n, nrows);
+
+    // carve the scratch: table | outside flag | cpos | wrapc | cell_of_point |
+    // rank_of_point | wrap_of_point | cstart | counts | nclus | cl_off | cluster table | scan storage
+    size_t const scan_bytes = std::max(fa::scan_tmp_bytes(uint32_t(ncell)), fa::scan_tmp_bytes(nrows));
+    if (scan_bytes == 0)
+        return fa::fail(FA_ERR_DEVICE, "scan storage query failed");
+    size_t off[13];
+    size_t const sizes[13] = {kNoAllocTableBytes,
+                              256,
+                              sizeof(float4) * (size_t(n) + fa::kCposPad),
+                              sizeof(int) * size_t(n),
+                              sizeof(uint32_t) * size_t(n),
+                              sizeof(uint32_t) * size_t(n),
+                              sizeof(int) * size_t(n),
+                              sizeof(uint32_t) * size_t(ncell + 1),
+                              sizeof(uint32_t) * size_t(ncell),
+                              sizeof(uint32_t) * (size_t(nrows) + 1),
+                              sizeof(uint32_t) * (size_t(nrows) + 1),
+                              sizeof(uint4) * size_t(max_clusters),
+                              scan_bytes};
+    size_t total = 0;
+    for (int k = 0; k < 13; ++k)
+    {
+        off[k] = total;
+        total += align256(sizes[k]);
+    }
+    if (total > kNoAllocBytes)
+        return fa::fail(FA_ERR_INVALID, "fa_rdf_accumulate_self_noalloc: the cell grid for these points and r_max "
+                                        "needs " + std::to_string(total) + " bytes of scratch, more than its " +
+                                            std::to_string(kNoAllocBytes));
+    char* const base = D.base;
+    hipStream_t s = fa::stream();
+
+    float* T = reinterpret_cast<float*>(base + off[0]);
+    if (!D.table_valid || D.bins != bins || D.r_min != r_min || D.r_max != r_max)
+    {
+        std::vector<float> const tab = build_table(bins, r_min, r_max);
+        FA_HIP(hipMemcpyAsync(T, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice, s));
+        D.table_valid = true;
+        D.bins = bins;
+        D.r_min = r_min;
+        D.r_max = r_max;
+    }
+    fa::GridBuffers G{};
+    G.outside = reinterpret_cast<int*>(base + off[1]);
+    G.cpos = reinterpret_cast<float4*>(base + off[2]);
+    G.wrapc = reinterpret_cast<int*>(base + off[3]);
+    G.cell_of_point = reinterpret_cast<uint32_t*>(base + off[4]);
+    G.rank_of_point = reinterpret_cast<uint32_t*>(base + off[5]);
+    G.wrap_of_point = reinterpret_cast<int*>(base + off[6]);
+    G.cstart = reinterpret_cast<uint32_t*>(base + off[7]);
+    G.counts = reinterpret_cast<uint32_t*>(base + off[8]);
+    G.scan_tmp = base + off[12];
+    G.scan_bytes = scan_bytes;
+    DevBox const db = fa::make_devbox(*box);
+    if (int rc = fa::build_grid_into(G, db, d_points, n, dims))
+        return rc;
+    DevGrid g{};
+    g.cpos = G.cpos;
+    g.wrapc = G.wrapc;
+    g.cstart = G.cstart;
+    g.outside = G.outside;
+    float npd[3];
+    fa::nearest_plane_distance(*box, npd);
+    for (int k = 0; k < 3; ++k)
+    {
+        g.dim[k] = dims[k];
+        g.cw[k] = (k == 2 && box->is2D) ? 0.0f : npd[k] / float(dims[k]);
+    }
+    RowsBuffers B{};
+    B.nclus = reinterpret_cast<uint32_t*>(base + off[9]);
+    B.cl_off = reinterpret_cast<uint32_t*>(base + off[10]);
+    B.table = reinterpret_cast<uint4*>(base + off[11]);
+    B.scan_tmp = base + off[12];
+    B.scan_bytes = scan_bytes;
+    float const width = (r_max - r_min) / float(bins);
+    float const inv_width = 1.0f / width;
+    return launch_rows(db, g, dims, rv.R, F, n, B, q_r_max * q_r_max, q_r_min * q_r_min, T, bins, r_min, inv_width,
+                       d_counts);
 }
 
 int fa_rdf_accumulate_nlist(const fa_nlist* nl, uint32_t bins, float r_min, float r_max, uint32_t* d_counts)

@@ -121,6 +121,8 @@ static int ensure_init()
     FA_HIP(hipEventCreate(&g_t0));
     FA_HIP(hipEventCreate(&g_t1));
     FA_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_stage), kReadbackBytes, hipHostMallocDefault));
+    if (int rc = noalloc_setup())
+        return rc;
     g_init = true;
     return FA_OK;
 }

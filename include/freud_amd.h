@@ -233,6 +233,20 @@ int fa_nlist_copy(const fa_nlist* src, fa_nlist* dst);
 int fa_rdf_bin_table(uint32_t bins, float r_min, float r_max, float* out, uint64_t cap);
 int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, float q_r_max, float q_r_min,
                             int exclude_ii, uint32_t bins, float r_min, float r_max, uint32_t* d_counts);
+/* No counterpart in the reference: a variant of fa_rdf_accumulate_query for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It computes what
+ * fa_rdf_accumulate_query computes for an AABBQuery over d_points with the
+ * query points equal to the points and exclude_ii set (the default of
+ * RDF.compute((box, points))), adding into d_counts (bins uint32), and calls
+ * no hipMalloc, hipFree or allocating library routine: the cell grid, bin
+ * table, cluster table and scan storage live in a 64 MiB array that each GPU
+ * holds from module load to process exit.  Limits: n <= 1,048,576 points,
+ * 1 <= bins <= 4096, and a cell grid that fits the array (an error says when
+ * it does not).  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (from several host threads) race. */
+int fa_rdf_accumulate_self_noalloc(const fa_box* box, const float* d_points, uint32_t n, float q_r_max,
+                                   float q_r_min, uint32_t bins, float r_min, float r_max, uint32_t* d_counts);
 /* Same, binning the distances of an existing NeighborList. */
 int fa_rdf_accumulate_nlist(const fa_nlist* nl, uint32_t bins, float r_min, float r_max, uint32_t* d_counts);
 /* RDF::reduce (freud/density/RDF.cc:73-99) and the shell volumes of the RDF
