@@ -2,12 +2,13 @@
 SolidLiquid, Hexatic, Cluster) on one AMD Instinct MI355X.
 
 The Python API mirrors freud 3.5.0's ``freud.box``, ``freud.locality``,
-``freud.density``, ``freud.order`` and ``freud.cluster`` for this path; the work
+``freud.density``, ``freud.order``, ``freud.pmft`` (PMFTXY) and ``freud.cluster``
+for this path; the work
 runs in hand-written HIP kernels (``freud_amd/csrc``) behind the C-ABI declared
 in ``include/freud_amd.h``.
 """
 
-from freud_amd import batch, box, cluster, data, density, learned, locality, order, parallel
+from freud_amd import batch, box, cluster, data, dcd, density, gsd, learned, locality, order, parallel, pmft
 from freud_amd.box import Box
 from freud_amd.locality import AABBQuery, LinkCell, NeighborList
 
@@ -27,4 +28,5 @@ __all__ = [
     "locality",
     "order",
     "parallel",
+    "pmft",
 ]

@@ -162,6 +162,13 @@ _SIGS = {
     "fa_rdf_accumulate_self_noalloc": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32,
                                                       ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
                                                       ctypes.c_float, ctypes.c_float, ctypes.c_void_p]),
+    "fa_local_density": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_void_p]),
+    "fa_pmftxy_accumulate": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_pmftxy_reduce": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_uint32, ctypes.c_void_p]),
     "fa_rdf_bin_table": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
                                         ctypes.c_uint64]),
     "fa_rdf_accumulate_nlist": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_uint32, ctypes.c_float,

@@ -1,5 +1,6 @@
-"""Radial distribution function: the API of ``freud.density.RDF``
-(freud/density.py:535-696) on the GPU.
+"""Radial distribution function and local density: the APIs of
+``freud.density.RDF`` (freud/density.py:535-696) and
+``freud.density.LocalDensity`` (freud/density.py:418-515) on the GPU.
 
 Without a NeighborList the bins are filled by the fused query + histogram
 kernel (``fa_rdf_accumulate_query``); a given NeighborList is binned from its
@@ -13,7 +14,7 @@ import numpy as np
 
 from freud_amd import _lib
 from freud_amd._lib import DeviceArray, check, lib
-from freud_amd.locality import _SpatialHistogram1D
+from freud_amd.locality import _PairCompute, _SpatialHistogram1D
 from freud_amd.util import _Compute, computed_property
 
 
@@ -139,3 +140,69 @@ class RDF(_SpatialHistogram1D):
 
     def __repr__(self):
         return f"freud_amd.density.RDF(bins={self._bins}, r_max={self.bounds[1]}, r_min={self.bounds[0]})"
+
+
+class LocalDensity(_PairCompute):
+    """Local number density around each query point (freud/density.py:418-515,
+    freud/density/LocalDensity.cc): points within r_max - diameter/2 count
+    fully, points whose centres lie within diameter/2 of r_max count linearly
+    in between.  Sums run in NeighborList bond order on the device."""
+
+    def __init__(self, r_max, diameter):
+        # LocalDensity.cc:26-37
+        if r_max <= 0:
+            msg = "LocalDensity requires r_max to be positive."
+            raise ValueError(msg)
+        if diameter < 0:
+            msg = "LocalDensity requires diameter to be non-negative."
+            raise ValueError(msg)
+        self._r_max32 = np.float32(r_max)
+        self._diameter32 = np.float32(diameter)
+
+    @property
+    def r_max(self):
+        return float(self._r_max32)
+
+    @property
+    def diameter(self):
+        return float(self._diameter32)
+
+    @property
+    def default_query_args(self):
+        """``{'mode': 'ball', 'r_max': r_max + 0.5 * diameter}`` (density.py:510-515)."""
+        return dict(mode="ball", r_max=self.r_max + 0.5 * self.diameter)
+
+    def compute(self, system, query_points=None, neighbors=None):
+        nq, nlist, qargs, d_q, num_query_points = self._preprocess_arguments(system, query_points, neighbors)
+        nlist = self._neighbor_list(nq, nlist, qargs, d_q, num_query_points)
+        n = int(num_query_points)
+        dens = _lib.reuse(self, "_d_density", (max(n, 1),), np.float32)
+        num = _lib.reuse(self, "_d_num", (max(n, 1),), np.float32)
+        check(lib().fa_local_density(ctypes.byref(nlist._c), float(self._r_max32), float(self._diameter32),
+                                     int(bool(nq._box.is2D)), dens.ptr, num.ptr))
+        self._n = n
+        self._box = nq._box
+        self._host = None
+        return self
+
+    def _results(self):
+        if self._host is None:
+            self._host = (self._d_density.numpy()[: self._n], self._d_num.numpy()[: self._n])
+        return self._host
+
+    @computed_property
+    def density(self):
+        return self._results()[0].copy()
+
+    @computed_property
+    def num_neighbors(self):
+        return self._results()[1].copy()
+
+    @computed_property
+    def box(self):
+        import freud_amd.box
+
+        return freud_amd.box.Box.from_box(self._box)
+
+    def __repr__(self):
+        return f"freud_amd.density.LocalDensity(r_max={self.r_max}, diameter={self.diameter})"

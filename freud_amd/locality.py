@@ -157,8 +157,15 @@ class NeighborQueryResult:
             yield (int(i), int(j), float(d))
 
     def toNeighborList(self, sort_by_distance=False):
-        d_q = DeviceArray.from_numpy(self._points.reshape(-1, 3))
-        return self._nq._run_query(d_q, len(self._points), self._query_args, sort_by_distance)
+        q = self._points.reshape(-1, 3)
+        host = self._nq._host_points
+        if host is not None and host.shape == q.shape and np.array_equal(host.view(np.uint32), q.view(np.uint32)):
+            # the query points are the NeighborQuery's own points: query with its
+            # device copy (no upload; the library then runs its self-query kernels)
+            d_q = self._nq._d_points
+        else:
+            d_q = DeviceArray.from_numpy(q)
+        return self._nq._run_query(d_q, len(q), self._query_args, sort_by_distance)
 
 
 class NeighborQuery:

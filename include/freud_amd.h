@@ -256,6 +256,28 @@ int fa_rdf_reduce(const fa_box* box, const uint32_t* counts, uint32_t bins, floa
                   uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, int finite_size, float* rdf,
                   float* n_r);
 
+/* LocalDensity::compute over a NeighborList (freud/density/LocalDensity.cc:39-84;
+ * binding export-LocalDensity.cc): per query point of nl, num_neighbors sums
+ * 1 for bonds with d < r_max - diameter/2 and 1 + (r_max - (d + diameter/2)) /
+ * diameter otherwise, in bond order; density = num_neighbors / (pi r_max^2) in
+ * 2D, / (4/3 pi r_max^3) in 3D.  d_density and d_num_neighbors hold
+ * num_query_points floats. */
+int fa_local_density(const fa_nlist* nl, float r_max, float diameter, int is2D, float* d_density,
+                     float* d_num_neighbors);
+/* PMFTXY::accumulate over a NeighborList (freud/pmft/PMFTXY.cc:69-88 via
+ * BondHistogramCompute::accumulateGeneral): every bond vector (x, y) rotated by
+ * minus its query point's orientation angle (query_orientations: host,
+ * radians, num_query_points floats, as the reference takes them) is binned
+ * into the n_x x n_y RegularAxis histogram over [-x_max, x_max) x
+ * [-y_max, y_max), row-major, added to d_counts.  The rotation's cos and sin
+ * come from the host C library, as in the reference. */
+int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, float x_max, float y_max,
+                         uint32_t n_x, uint32_t n_y, uint32_t* d_counts);
+/* PMFTXY::reduce (PMFTXY.cc:62-66, PMFT.h:76-87): pcf (host, n_x * n_y) from
+ * host counts, in the reference's float order. */
+int fa_pmftxy_reduce(const fa_box* box, const uint32_t* counts, uint32_t n_x, uint32_t n_y, float x_max, float y_max,
+                     uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, float* pcf);
+
 /* ---------------------------------------------------------------- order
  * Steinhardt::compute over a NeighborList (freud/order/Steinhardt.cc:85-359;
  * binding export-Steinhardt.cc:19-34).  ls[n_l] are the l values.  Outputs

@@ -17,6 +17,9 @@ _lib = None
 
 __all__ = [
     "build",
+    "local_density",
+    "pmftxy_accumulate",
+    "pmftxy_reduce",
     "box_op",
     "nearest_plane_distance",
     "query",
@@ -342,3 +345,40 @@ def hexatic(box, points, nlist, k=6, weighted=False):
     lib.orc_hexatic(_p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(ij), _p(w), ctypes.c_uint64(len(ij)),
                     ctypes.c_uint32(k), int(bool(weighted)), _p(psi))
     return psi.reshape(-1, 2)[:, 0] + 1j * psi.reshape(-1, 2)[:, 1]
+
+
+def local_density(nlist, n_query_points, r_max, diameter, is2D=False):
+    """LocalDensity.cc:39-84 over an i-sorted NeighborList -> (density, num_neighbors)."""
+    lib = _load()
+    ij = np.ascontiguousarray(nlist["ij"], dtype=np.uint32)
+    d = _f32(nlist["distances"])
+    dens = np.empty(n_query_points, np.float32)
+    num = np.empty(n_query_points, np.float32)
+    lib.orc_local_density(ctypes.c_uint32(n_query_points), _p(ij), _p(d), ctypes.c_uint64(len(d)),
+                          ctypes.c_float(r_max), ctypes.c_float(diameter), int(bool(is2D)), _p(dens), _p(num))
+    return dens, num
+
+
+def pmftxy_accumulate(nlist, query_angles, x_max, y_max, n_x, n_y, counts=None):
+    """PMFTXY.cc:69-88: adds the rotated-bond histogram (n_x, n_y) to counts."""
+    lib = _load()
+    ij = np.ascontiguousarray(nlist["ij"], dtype=np.uint32)
+    v = _f32(nlist["vectors"]).reshape(-1, 3)
+    th = _f32(query_angles).ravel()
+    if counts is None:
+        counts = np.zeros((n_x, n_y), np.uint32)
+    lib.orc_pmftxy_accumulate(_p(ij), _p(v), ctypes.c_uint64(len(ij)), _p(th), ctypes.c_float(x_max),
+                              ctypes.c_float(y_max), ctypes.c_uint32(n_x), ctypes.c_uint32(n_y), _p(counts))
+    return counts
+
+
+def pmftxy_reduce(box, counts, x_max, y_max, n_points, n_query_points, n_frames):
+    lib = _load()
+    b, is2d = _boxp(box)
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    n_x, n_y = c.shape
+    pcf = np.empty((n_x, n_y), np.float32)
+    lib.orc_pmftxy_reduce(_p(b), is2d, _p(c), ctypes.c_uint32(n_x), ctypes.c_uint32(n_y), ctypes.c_float(x_max),
+                          ctypes.c_float(y_max), ctypes.c_uint32(n_points), ctypes.c_uint32(n_query_points),
+                          ctypes.c_uint32(n_frames), _p(pcf))
+    return pcf

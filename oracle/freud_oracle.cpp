@@ -35,6 +35,13 @@
 //                                    weight -> q_l = 1)
 //   SolidLiquid, Cluster, Hexatic .. FCC known answer; scipy connected components;
 //                                    closed forms
+//   LocalDensity ................... the reference's known answer
+//                                    (test_density_local_density.py:80-98), its
+//                                    statistical bounds, and a float64 brute force
+//   PMFTXY counts / pcf ............ the reference's known answers
+//                                    (test_pmft.py: two-particle bins, nearest-
+//                                    neighbour bins, orientation tests) and a
+//                                    numpy brute force of the rotated bins
 //   system `order` ................. parity unpinned bitwise (the reference's value
 //                                    depends on TBB's thread partition); pinned to
 //                                    tolerance by the FCC constant.
@@ -1479,6 +1486,74 @@ void orc_hexatic(const float* boxp, int is2D, const float* pts_, uint32_t np, co
         psi[2 * i] = acc.real();
         psi[2 * i + 1] = acc.imag();
     }
+}
+
+// ---------------------------------------------------------------- LocalDensity, PMFTXY
+// LocalDensity.cc:39-84 over an i-sorted bond list (bond order within a point).
+void orc_local_density(uint32_t nq, const uint32_t* ij, const float* dist, uint64_t nb, float r_max, float diameter,
+                       int is2D, float* density, float* num_neighbors)
+{
+    float const area = M_PI * r_max * r_max;
+    float const volume = static_cast<float>(4.0 / 3.0 * M_PI) * r_max * r_max * r_max;
+    std::fill(density, density + nq, 0.0f);
+    std::fill(num_neighbors, num_neighbors + nq, 0.0f);
+    uint64_t e = 0;
+    for (uint32_t i = 0; i < nq; ++i)
+    {
+        float num = 0;
+        for (; e < nb && ij[2 * e] == i; ++e)
+        {
+            float const d = dist[e];
+            if (d < (r_max - diameter / float(2.0)))
+                num += float(1.0);
+            else
+                num += float(1.0) + (r_max - (d + diameter / float(2.0))) / diameter;
+            num_neighbors[i] = num;
+            density[i] = is2D ? num / area : num / volume;
+        }
+    }
+}
+
+// PMFTXY.cc:25-88 (accumulate): each bond vector rotated by rotmat2::fromAngle(-theta_i)
+// (VectorMath.h:905-921, std::cos / std::sin of a float) and binned on two
+// RegularAxis (Histogram.h:123-179), row-major index bx * n_y + by.
+void orc_pmftxy_accumulate(const uint32_t* ij, const float* vec, uint64_t nb, const float* theta, float x_max,
+                           float y_max, uint32_t n_x, uint32_t n_y, uint32_t* counts)
+{
+    float const wx = (x_max - (-x_max)) / static_cast<float>(n_x);
+    float const wy = (y_max - (-y_max)) / static_cast<float>(n_y);
+    float const ix = static_cast<float>(1.0) / wx;
+    float const iy = static_cast<float>(1.0) / wy;
+    for (uint64_t e = 0; e < nb; ++e)
+    {
+        float const t = -theta[ij[2 * e]];
+        float const c = std::cos(t), sn = std::sin(t);
+        float const r0x = c, r0y = -sn, r1x = sn, r1y = c;
+        float const x = vec[3 * e], y = vec[3 * e + 1];
+        float const rx = r0x * x + r0y * y;
+        float const ry = r1x * x + r1y * y;
+        long const bx = regular_bin(rx, -x_max, x_max, ix, n_x);
+        long const by = regular_bin(ry, -y_max, y_max, iy, n_y);
+        if (bx < 0 || by < 0)
+            continue;
+        counts[size_t(bx) * n_y + size_t(by)] += 1;
+    }
+}
+
+// PMFTXY.cc:62-66 with PMFT.h:76-87
+void orc_pmftxy_reduce(const float* boxp, int is2D, const uint32_t* counts, uint32_t n_x, uint32_t n_y, float x_max,
+                       float y_max, uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, float* pcf)
+{
+    Box b = make_box(boxp, is2D);
+    float const dx = float(2.0) * x_max / float(n_x);
+    float const dy = float(2.0) * y_max / float(n_y);
+    float const jacobian = dx * dy;
+    float const jacobian_factor = (float) 1.0 / jacobian;
+    float const inv_num_dens = box_volume(b) / static_cast<float>(n_query_points);
+    float const norm_factor = float(1.0) / (static_cast<float>(n_frames) * static_cast<float>(n_points));
+    float const prefactor = inv_num_dens * norm_factor;
+    for (uint64_t t = 0; t < uint64_t(n_x) * n_y; ++t)
+        pcf[t] = static_cast<float>(counts[t]) * prefactor * jacobian_factor;
 }
 
 } // extern "C"

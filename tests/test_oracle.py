@@ -10,6 +10,7 @@ import pytest
 import brute
 from conftest import GOLDEN
 from freud_amd.box import Box
+from freud_amd import data
 from freud_amd.gsd import open_gsd
 
 
@@ -532,3 +533,140 @@ def test_knn_distance_tie(oracle, kind):
     d = full["distances"][sel]
     assert rsq[byj[2]] < rsq[byj[1]] and d[byj[2]] == d[byj[1]]
     npt.assert_array_equal(nl3["vectors"][row][0], r[byj[1]])
+
+
+# --------------------------------------------------------------------- LocalDensity, PMFTXY
+def _frac(dist, r_max, diameter):
+    # tests/test_density_local_density.py:11-18 of the reference
+    if dist < r_max - diameter / 2:
+        return 1
+    if dist > r_max + diameter / 2:
+        return 0
+    return -dist / diameter + r_max / diameter + 0.5
+
+
+def test_local_density_known_answer(oracle):
+    # tests/test_density_local_density.py:80-98 of the reference
+    box = (10.0, 10.0, 10.0, 0, 0, 0, False)
+    points = np.array([[0, 0, 0], [1, 0, 0]], np.float32)
+    query_points = np.array([[0, 1, 0], [-1, -1, 0], [4, 0, 0]], np.float32)
+    diameter, r_max = 1, 2
+    nl = oracle.query(box, points, query_points, dict(r_max=r_max + 0.5 * diameter, exclude_ii=False))
+    dens, num = oracle.local_density(nl, 3, r_max, diameter)
+    v_around = 4 / 3 * (r_max**3) * np.pi
+    cd1 = (1 + _frac(np.linalg.norm(points[1] - query_points[1]), r_max, diameter)) / v_around
+    npt.assert_allclose(dens, [2 / v_around, cd1, 0], rtol=1e-4)
+    npt.assert_allclose(num, dens * v_around, rtol=1e-5)
+
+
+def test_local_density_brute_force_and_bounds(oracle):
+    """A float64 brute force over all minimum images, and the reference's
+    bounds for 10,000 uniform points in a cube of side 10 (density within 1.5
+    of 10, neighbour count within 200 of 1130.97; test_density_local_density.py:44-78)."""
+    box, pos = data.make_random_system(10, 10000, seed=123)
+    bt = (box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D)
+    q = pos[:300]
+    nl = oracle.query(bt, pos, q, dict(r_max=3.5, exclude_ii=False), nthreads=8)
+    dens, num = oracle.local_density(nl, len(q), 3.0, 1.0)
+    npt.assert_array_less(np.abs(dens - 10.0), 1.5)
+    npt.assert_array_less(np.abs(num - 1130.973355292), 200)
+    d = q[:, None, :].astype(np.float64) - pos[None, :, :].astype(np.float64)
+    d -= 10.0 * np.round(d / 10.0)
+    r = np.linalg.norm(d, axis=-1)
+    w = np.where(r < 2.5, 1.0, np.where(r < 3.5, 1.0 + (3.0 - (r + 0.5)) / 1.0, 0.0))
+    npt.assert_allclose(num, w.sum(axis=1), rtol=2e-5)
+    npt.assert_allclose(dens, w.sum(axis=1) / (4 / 3 * np.pi * 27), rtol=2e-5)
+
+
+def _rot_bins(vec, theta, limits, bins):
+    """Bins of bond vectors rotated by -theta (the reference's get_bin,
+    test_pmft.py:401-410), in float64."""
+    c, s = np.cos(-theta), np.sin(-theta)
+    x = c * vec[:, 0] - s * vec[:, 1]
+    y = s * vec[:, 0] + c * vec[:, 1]
+    lim = np.asarray(limits, np.float64)
+    return np.floor((np.stack([x, y], 1) + lim) * np.asarray(bins) / (2 * lim)).astype(np.int64)
+
+
+def test_pmftxy_two_particles(oracle):
+    # tests/test_pmft.py:119-154, 378-410 of the reference
+    limits, bins = (3.6, 4.2), (100, 110)
+    box = (16.0, 16.0, 0.0, 0, 0, 0, True)
+    points = np.array([[-1.0, 0.0, 0.0], [1.0, 0.1, 0.0]], np.float32)
+    nl = oracle.query(box, points, points, dict(r_max=float(np.linalg.norm(limits)), exclude_ii=True))
+    counts = oracle.pmftxy_accumulate(nl, np.zeros(2, np.float32), *limits, *bins)
+    expected = np.zeros(bins, np.uint32)
+    for a, b in ((0, 1), (1, 0)):
+        bx, by = _rot_bins((points[b] - points[a])[None], np.zeros(1), limits, bins)[0]
+        expected[bx, by] = 1
+    npt.assert_array_equal(counts, expected)
+
+
+def test_pmftxy_nearest_neighbour_bins(oracle):
+    # tests/test_pmft.py:455-485 of the reference
+    box = (8.0, 8.0, 0.0, 0, 0, 0, True)
+    points = np.array([[0, 0, 0]], np.float32)
+    query_points = np.array([[1.1, 0.0, 0.0], [-1.2, 0.0, 0.0], [0.0, 1.3, 0.0], [0.0, -1.4, 0.0]], np.float32)
+    nl = oracle.query(box, points, query_points, dict(num_neighbors=1, exclude_ii=False))
+    counts = oracle.pmftxy_accumulate(nl, np.zeros(4, np.float32), 3, 3, 3, 3)
+    npt.assert_array_equal(counts, [[0, 1, 0], [1, 0, 1], [0, 1, 0]])
+    nl = oracle.query(box, query_points, points, dict(num_neighbors=1, exclude_ii=False))
+    counts = oracle.pmftxy_accumulate(nl, np.zeros(1, np.float32), 3, 3, 3, 3)
+    npt.assert_array_equal(counts, [[0, 0, 0], [0, 0, 0], [0, 1, 0]])
+
+
+def test_pmftxy_orientation(oracle):
+    # tests/test_pmft.py:487-558 of the reference: the query point's angle rotates its bonds
+    box = (8.0, 8.0, 0.0, 0, 0, 0, True)
+    points = np.array([[0.1, 0.1, 0]], np.float32)
+    points2 = np.array([[1, 0, 0]], np.float32)
+
+    def occupied(pts, qpts, angle):
+        nl = oracle.query(box, pts, qpts, dict(num_neighbors=1, exclude_ii=False))
+        c = oracle.pmftxy_accumulate(nl, np.full(len(qpts), angle, np.float32), 3, 3, 12, 12)
+        return tuple(np.asarray(np.where(c)).squeeze())
+
+    assert occupied(points, points2, 0.0) == (4, 6)
+    assert occupied(points2, points, 0.0) == (7, 5)
+    th = np.float32(np.deg2rad(45))
+    for pts, qpts in ((points, points2), (points2, points)):
+        vec = (pts - qpts).astype(np.float64)
+        expect = tuple(_rot_bins(vec, np.array([th], np.float64), (3, 3), (12, 12))[0])
+        assert occupied(pts, qpts, th) == expect
+
+
+def test_pmftxy_brute_force_and_ideal_gas(oracle):
+    """Random 2D system with random query angles: every bond's bin equals a
+    float64 rotation away from bin edges, and the pair correlation function
+    of an ideal gas averages to 1 (tests/test_pmft.py:160-189)."""
+    box, pts = data.make_random_system(10, 2500, is2D=True, seed=3)
+    bt = (box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D)
+    rng = np.random.default_rng(4)
+    theta = (rng.random(len(pts)) * 2 * np.pi).astype(np.float32)
+    lim = np.float32(np.sqrt(9 / 2))
+    nl = oracle.query(bt, pts, pts, dict(r_max=float(np.sqrt(2) * lim), exclude_ii=True), nthreads=8)
+    counts = oracle.pmftxy_accumulate(nl, theta, lim, lim, 100, 100)
+    vec = nl["vectors"].astype(np.float64)
+    th = theta[nl["ij"][:, 0]].astype(np.float64)
+    c, s = np.cos(-th), np.sin(-th)
+    x = c * vec[:, 0] - s * vec[:, 1]
+    y = s * vec[:, 0] + c * vec[:, 1]
+    t = (np.stack([x, y], 1) + float(lim)) * 100 / (2 * float(lim))
+    inside = np.all((t >= 0) & (t < 100), axis=1)
+    away = np.all(np.abs(t - np.round(t)) > 1e-4, axis=1)
+    b = np.floor(t).astype(np.int64)
+    ref = np.zeros((100, 100), np.int64)
+    np.add.at(ref, (b[inside & away, 0], b[inside & away, 1]), 1)
+    # bins next to a bond within rounding of an edge may hold it or not
+    near = np.zeros((100, 100), bool)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            near[np.clip(b[~away, 0] + dx, 0, 99), np.clip(b[~away, 1] + dy, 0, 99)] = True
+    sel = ~near
+    assert sel.sum() > 5000
+    npt.assert_array_equal(counts[sel], ref[sel])
+    pcf = oracle.pmftxy_reduce(bt, counts, lim, lim, len(pts), len(pts), 1)
+    e = np.linspace(-lim, lim, 101)
+    cc = (e[:-1] + e[1:]) / 2
+    rr = np.sqrt(cc[:, None] ** 2 + cc[None, :] ** 2)
+    assert abs(pcf[rr < 0.95 * np.sqrt(2) * lim].mean() - 1.0) < 0.05
