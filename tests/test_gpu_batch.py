@@ -157,3 +157,43 @@ def test_batch_rdf_every_launch_shape(oracle):
                 npt.assert_array_equal(counts[f], ref[f], err_msg=f"shape {shape} frame {f}")
     finally:
         _lib.tune_load({(1, key): before.get((1, key), -1)})
+
+
+def test_batched_trajectory_full_size(oracle):
+    """The batched trajectory at its benchmark size: 256 frames of 10k-200k
+    points (log-uniform; liquid, jittered FCC and BCC; cubic and triclinic),
+    ball r_max = 1.5, RDF 200 bins to r_max = 5, q6, run eagerly and then
+    replayed from the captured graph.  Every frame's RDF counts and bond count
+    equal the oracle's; the NeighborLists and q6 of the smallest, the largest
+    and a triclinic frame are checked in full."""
+    frames, kinds = freud_amd.data.mixed_frames(256, 10_000, 200_000, seed=4)
+    sizes = np.array([len(p) for _, p in frames])
+    assert sizes.min() >= 9_000 and sizes.max() <= 220_000 and sizes.sum() > 1.2e7
+    b = TrajectoryBatch(frames, ball_r_max=1.5, rdf_bins=200, rdf_r_max=5.0, l=6)
+    b.compute(use_graph=True)  # eager run, then capture
+    eager = b.rdf_counts.copy()
+    b.compute(use_graph=True)  # graph replay
+    assert b.config["graph"]
+    counts = b.rdf_counts
+    npt.assert_array_equal(counts, eager)
+    nb = b.num_bonds
+    for f, (box, pts) in enumerate(frames):
+        rc = oracle.rdf_accumulate(_bt(box), pts, pts, 200, 5.0, exclude_ii=True, nthreads=16)
+        npt.assert_array_equal(counts[f], rc, err_msg=f"frame {f}")
+    tri = [f for f, (box, _) in enumerate(frames) if box.xy != 0 or box.xz != 0 or box.yz != 0]
+    assert tri
+    sample = {int(np.argmin(sizes)), int(np.argmax(sizes)), tri[0]}
+    qls = b.ql()
+    for f, (box, pts) in enumerate(frames):
+        ref = oracle.query(_bt(box), pts, pts, dict(r_max=1.5, exclude_ii=True), nthreads=16)
+        assert int(nb[f]) == len(ref["ij"]), f"frame {f}"
+        if f not in sample:
+            continue
+        nl = b.neighbor_list(f)
+        npt.assert_array_equal(nl[:], ref["ij"], err_msg=f"frame {f}")
+        npt.assert_array_equal(nl.distances, ref["distances"], err_msg=f"frame {f}")
+        npt.assert_array_equal(nl.vectors, ref["vectors"], err_msg=f"frame {f}")
+        st = oracle.steinhardt(_bt(box), pts, ref, [6], nthreads=16)
+        ok = np.isfinite(st["ql"][:, 0])
+        npt.assert_array_equal(np.isfinite(qls[f]), ok)
+        npt.assert_allclose(qls[f][ok], st["ql"][ok, 0], atol=1e-5, err_msg=f"frame {f}")
