@@ -10,7 +10,7 @@ in ``include/freud_amd.h``.
 
 from freud_amd import batch, box, cluster, data, dcd, density, gsd, learned, locality, order, parallel, pmft
 from freud_amd.box import Box
-from freud_amd.locality import AABBQuery, LinkCell, NeighborList
+from freud_amd.locality import AABBQuery, CellQuery, LinkCell, NeighborList
 
 __version__ = "0.1.0"
 
@@ -18,6 +18,7 @@ __all__ = [
     "AABBQuery",
     "Box",
     "batch",
+    "CellQuery",
     "LinkCell",
     "NeighborList",
     "box",

@@ -21,6 +21,7 @@ FA_ERR_DEVICE = -4
 
 NQ_AABB = 0
 NQ_LINKCELL = 1
+NQ_CELLQUERY = 2
 
 MODE_NONE = 0
 MODE_BALL = 1
@@ -144,6 +145,8 @@ _SIGS = {
                                     ctypes.c_float, ctypes.POINTER(ctypes.c_void_p)]),
     "fa_nq_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "fa_nq_cell_width": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "fa_cellquery_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_float)]),
     "fa_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(fa_query_args),
                                 ctypes.c_int, ctypes.POINTER(fa_nlist)]),
     "fa_nlist_free": (ctypes.c_int, [ctypes.POINTER(fa_nlist)]),

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <functional>
 #include <string>
+#include <type_traits>
 
 #include "../../include/freud_amd.h"
 
@@ -244,9 +245,75 @@ __device__ inline void unpack_wrap(int p, int w[3])
 
 enum Formula
 {
-    AABB_FORMULA = 0,     // r = p_j - (q_i + image)   freud/locality/AABBQuery.cc:93,125
-    LINKCELL_FORMULA = 1, // r = box.wrap(p_j - q_i)   freud/locality/LinkCell.cc:522,616
+    AABB_FORMULA = 0,      // r = p_j - (q_i + image)   freud/locality/AABBQuery.cc:93,125
+    LINKCELL_FORMULA = 1,  // r = box.wrap(p_j - q_i)   freud/locality/LinkCell.cc:522,616
+    CELLQUERY_FORMULA = 2, // r = (p_j + ghost shift) - q_i   freud/locality/CellIterator.h:177-186
 };
+
+// The grid a CellQuery query builds (freud/locality/CellQuery.h:136-160
+// setupGrid, CellQuery.cc:56-113 buildGrid): cubic cells of width r_cut (the
+// query's r_max) over the box's bounding parallelepiped padded by one cell,
+// with ghost copies of points within r_cut (in fractional terms) of a face.
+// The GPU does not build it: it enumerates candidates on its own cell grid and
+// keeps exactly the (point, ghost) pairs this grid would visit.
+struct CQGrid
+{
+    float min[3]; // lowest corner of the grid
+    float inv;    // 1 / r_cut
+    int n[3];     // cells per axis
+    float frc[3]; // r_cut / nearest plane distance (fractional ghost margin)
+};
+
+// CellQuery::cell_idx_xyz / getCellIdxSafe (CellQuery.h:53-75): false when the
+// point's cell lies outside the grid.
+__host__ __device__ inline bool cq_cell(const CQGrid& c, float x, float y, float z, int out[3])
+{
+    out[0] = int(floorf((x - c.min[0]) * c.inv));
+    out[1] = int(floorf((y - c.min[1]) * c.inv));
+    out[2] = int(floorf((z - c.min[2]) * c.inv));
+    return out[0] >= 0 && out[1] >= 0 && out[2] >= 0 && out[0] < c.n[0] && out[1] < c.n[1] && out[2] < c.n[2];
+}
+
+// Box::getLatticeVector (freud/box/Box.h:503-518) as CellQuery.cc:80-83 takes
+// them (the third is zero in 2D).
+__host__ __device__ inline void cq_lattice(const fa_box& b, float lat[3][3])
+{
+    lat[0][0] = b.Lx;
+    lat[0][1] = 0.0f;
+    lat[0][2] = 0.0f;
+    lat[1][0] = b.Ly * b.xy;
+    lat[1][1] = b.Ly;
+    lat[1][2] = 0.0f;
+    if (b.is2D)
+    {
+        lat[2][0] = lat[2][1] = lat[2][2] = 0.0f;
+    }
+    else
+    {
+        lat[2][0] = b.Lz * b.xz;
+        lat[2][1] = b.Lz * b.yz;
+        lat[2][2] = b.Lz;
+    }
+}
+
+// The ghost displacement for shift s (components in {-1, 0, 1}), summed in
+// CellQuery.h:243-258's order: +-a, then +-b, then +-c, onto zero.
+__host__ __device__ inline void cq_shift(const float lat[3][3], int sx, int sy, int sz, float& dx, float& dy, float& dz)
+{
+    dx = 0.0f;
+    dy = 0.0f;
+    dz = 0.0f;
+    int const sv[3] = {sx, sy, sz};
+    for (int a = 0; a < 3; ++a)
+    {
+        if (sv[a] == 0)
+            continue;
+        float const sg = sv[a] == 1 ? 1.0f : -1.0f;
+        dx = dx + sg * lat[a][0];
+        dy = dy + sg * lat[a][1];
+        dz = dz + sg * lat[a][2];
+    }
+}
 
 // Per-dimension stencil interval [lo, hi] of cell offsets.  For the AABB
 // formula every offset is a distinct (cell, image) pair, so the interval is
@@ -265,6 +332,18 @@ template<int FORMULA> __device__ inline void stencil_interval(int R, int dim, in
         lo = -R;
         hi = R;
     }
+}
+
+// Calls a candidate visitor; visitors that take a seventh argument also get
+// the candidate's cell-ordered record (x, y, z, bits(j)).
+template<typename F>
+__device__ inline void visit_candidate(F& f, uint32_t j, float rx, float ry, float rz, float r_sq, int code,
+                                       const float4& p)
+{
+    if constexpr (std::is_invocable_v<F&, uint32_t, float, float, float, float, int, const float4&>)
+        f(j, rx, ry, rz, r_sq, code, p);
+    else
+        f(j, rx, ry, rz, r_sq, code);
 }
 
 template<int FORMULA, typename F>
@@ -301,7 +380,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
                 float const ry = p.y - iy;
                 float const rz = pz - iz;
                 float const r_sq = (rx * rx + ry * ry) + rz * rz;
-                f(__float_as_uint(p.w), rx, ry, rz, r_sq, code);
+                visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, code, p);
                 p = pn;
             }
         }
@@ -321,7 +400,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
                 float const ry = p.y - (qy + b.img[code][1]);
                 float const rz = pz - (qzz + b.img[code][2]);
                 float const r_sq = (rx * rx + ry * ry) + rz * rz;
-                f(__float_as_uint(p.w), rx, ry, rz, r_sq, code);
+                visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, code, p);
             }
         }
     }
@@ -336,7 +415,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
             float rx, ry, rz;
             wrap(b, p.x - qx, p.y - qy, p.z - qz, rx, ry, rz);
             float const r_sq = (rx * rx + ry * ry) + rz * rz;
-            f(__float_as_uint(p.w), rx, ry, rz, r_sq, 13);
+            visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, 13, p);
             p = pn;
         }
     }
@@ -398,6 +477,24 @@ template<int FORMULA>
 __device__ inline void bond_vector(const DevBox& b, float qx, float qy, float qz, float px, float py, float pz,
                                    int code, float& rx, float& ry, float& rz)
 {
+    if (FORMULA == CELLQUERY_FORMULA)
+    {
+        // ghost p_j + shift (shift = -image of the AABB image code), then minus q
+        fa_box const fb{b.L[0], b.L[1], b.L[2], b.xy, b.xz, b.yz, b.is2D};
+        float lat[3][3];
+        cq_lattice(fb, lat);
+        float dx, dy, dz;
+        cq_shift(lat, 1 - code / 9, 1 - (code / 3) % 3, 1 - code % 3, dx, dy, dz);
+        // a real point is taken as stored, a ghost as p_j + shift (CellQuery.cc:95-112)
+        bool const real = code == 13;
+        float const gx = real ? px : px + dx;
+        float const gy = real ? py : py + dy;
+        float const gz = real ? pz : pz + dz;
+        rx = gx - qx;
+        ry = gy - qy;
+        rz = gz - qz;
+        return;
+    }
     if (FORMULA == AABB_FORMULA)
     {
         float const qzz = b.is2D ? 0.0f : qz;
@@ -410,6 +507,73 @@ __device__ inline void bond_vector(const DevBox& b, float qx, float qy, float qz
     {
         wrap(b, px - qx, py - qy, pz - qz, rx, ry, rz);
     }
+}
+
+// CellQuery's view of an AABB-enumerated candidate p (image code `code`) for
+// query point q whose CellQuery cell is cq_q (CellIterator.h:94-187): the
+// candidate is the point itself (code 13) or its ghost p + shift, shift = minus
+// the image; the ghost must be one generateGhosts makes (CellQuery.h:221-272:
+// shifts only toward faces within the fractional margin), and the point or
+// ghost must lie in the grid, in the 3 x 3 x 3 cells around q's cell.  Then
+// r = (ghost) - q and r_sq = (x*x + y*y) + z*z.
+__device__ inline bool cq_candidate(const DevBox& b, const CQGrid& c, const int cq_q[3], float qx, float qy,
+                                    float qz, const float4& p, int code, float& rx, float& ry, float& rz,
+                                    float& r_sq)
+{
+    float gx = p.x, gy = p.y, gz = p.z;
+    if (code != 13)
+    {
+        int const sx = 1 - code / 9, sy = 1 - (code / 3) % 3, sz = 1 - code % 3;
+        float fx, fy, fz;
+        make_fractional(b, p.x, p.y, p.z, fx, fy, fz);
+        int const dx = int(fx <= c.frc[0]) - int(double(fx) >= 1.0 - double(c.frc[0]));
+        int const dy = int(fy <= c.frc[1]) - int(double(fy) >= 1.0 - double(c.frc[1]));
+        int const dz = b.is2D ? 0 : int(fz <= c.frc[2]) - int(double(fz) >= 1.0 - double(c.frc[2]));
+        if ((sx != 0 && sx != dx) || (sy != 0 && sy != dy) || (sz != 0 && sz != dz))
+            return false;
+        fa_box const fb{b.L[0], b.L[1], b.L[2], b.xy, b.xz, b.yz, b.is2D};
+        float lat[3][3];
+        cq_lattice(fb, lat);
+        float ddx, ddy, ddz;
+        cq_shift(lat, sx, sy, sz, ddx, ddy, ddz);
+        gx = p.x + ddx;
+        gy = p.y + ddy;
+        gz = p.z + ddz;
+    }
+    int cg[3];
+    if (!cq_cell(c, gx, gy, gz, cg))
+        return false;
+    if (abs(cg[0] - cq_q[0]) > 1 || abs(cg[1] - cq_q[1]) > 1 || abs(cg[2] - cq_q[2]) > 1)
+        return false;
+    rx = gx - qx;
+    ry = gy - qy;
+    rz = gz - qz;
+    r_sq = (rx * rx + ry * ry) + rz * rz;
+    return true;
+}
+
+// The bonds of query point q with the arithmetic of FORMULA: f(j, rx, ry, rz,
+// r_sq, code) for every candidate of the stencil (the CellQuery formula visits
+// the AABB candidates and keeps those its own grid would see; a query point
+// outside that grid has no bonds, CellIterator.h:52-57).
+template<int FORMULA, typename F>
+__device__ inline void visit_bonds(const DevBox& b, const DevGrid& g, const CQGrid& cq, float qx, float qy,
+                                   float qz, int R, F&& f)
+{
+    if constexpr (FORMULA == CELLQUERY_FORMULA)
+    {
+        int cq_q[3];
+        if (!cq_cell(cq, qx, qy, qz, cq_q))
+            return;
+        for_each_candidate<AABB_FORMULA>(
+            b, g, qx, qy, qz, R, false, [&](uint32_t j, float, float, float, float, int code, const float4& p) {
+                float rx, ry, rz, r_sq;
+                if (cq_candidate(b, cq, cq_q, qx, qy, qz, p, code, rx, ry, rz, r_sq))
+                    f(j, rx, ry, rz, r_sq, code);
+            });
+    }
+    else
+        for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false, f);
 }
 
 // ------------------------------------------------------------------ host-side objects
@@ -590,7 +754,7 @@ struct fa_nq
     fa_box box;
     const float* d_points; // caller-owned device array, N x 3
     uint32_t n;
-    int kind;         // FA_NQ_AABB or FA_NQ_LINKCELL
+    int kind;         // FA_NQ_AABB, FA_NQ_LINKCELL or FA_NQ_CELLQUERY
     float cell_width; // LinkCell's cell width (reference semantics)
     int lc_dim[3];    // LinkCell's grid dims (LinkCell.cc:278-314)
     // small cache of grids (one per query shape: e.g. ball query, RDF, kNN);
@@ -609,6 +773,8 @@ namespace fa {
 int validate_query_args(const fa_nq* nq, fa_query_args& a, uint32_t n_query);
 int ensure_grid(fa_nq* nq, const int dims[3]);
 void grid_dims_for(const fa_box& box, uint32_t n, float r, int R, int dims[3]);
+// The grid a CellQuery ball query of radius r_cut builds (CellQuery.h:136-160)
+CQGrid cq_grid_for(const fa_box& box, float r_cut);
 int nlist_reserve(fa_nlist* nl, uint64_t nb, uint32_t nq);
 int nlist_finish_counts(fa_nlist* nl);
 } // namespace fa

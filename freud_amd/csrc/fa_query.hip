@@ -22,6 +22,7 @@
 namespace {
 
 using fa::AABB_FORMULA;
+using fa::CELLQUERY_FORMULA;
 using fa::DevBox;
 using fa::DevGrid;
 using fa::LINKCELL_FORMULA;
@@ -92,12 +93,12 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uin
 template<int FORMULA, bool SELF>
 __device__ inline void ball_count_body(const DevBox& b, const DevGrid& g, const float* __restrict__ qpts, uint32_t p,
                                        float r_max_sq, float r_min_sq, int exclude_ii, int R,
-                                       uint32_t* __restrict__ counts)
+                                       uint32_t* __restrict__ counts, const fa::CQGrid& cq = fa::CQGrid{})
 {
     float qx, qy, qz;
     uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
     uint32_t c = 0;
-    fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
+    fa::visit_bonds<FORMULA>(b, g, cq, qx, qy, qz, R,
                                     [&](uint32_t j, float, float, float, float r_sq, int) {
                                         // range first: the index is needed for in-range candidates only
                                         if (r_sq < r_max_sq && r_sq >= r_min_sq && !(exclude_ii && j == i))
@@ -109,11 +110,12 @@ __device__ inline void ball_count_body(const DevBox& b, const DevGrid& g, const 
 template<int FORMULA, bool SELF>
 __global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
                                                             uint32_t nq, float r_max_sq, float r_min_sq,
-                                                            int exclude_ii, int R, uint32_t* __restrict__ counts)
+                                                            int exclude_ii, int R, uint32_t* __restrict__ counts,
+                                                            fa::CQGrid cq)
 {
     uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p < nq)
-        ball_count_body<FORMULA, SELF>(b, g, qpts, p, r_max_sq, r_min_sq, exclude_ii, R, counts);
+        ball_count_body<FORMULA, SELF>(b, g, qpts, p, r_max_sq, r_min_sq, exclude_ii, R, counts, cq);
 }
 
 template<typename K> __device__ inline void insertion_sort(K* a, uint32_t n)
@@ -198,7 +200,8 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
                                       int exclude_ii, int R, const uint32_t* __restrict__ offsets,
                                       uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                       float* __restrict__ w_out, float* __restrict__ v_out,
-                                      void* __restrict__ gkeys_raw, uint32_t key_cap)
+                                      void* __restrict__ gkeys_raw, uint32_t key_cap,
+                                      const fa::CQGrid& cq = fa::CQGrid{})
 {
     using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
     // key staging: dynamic LDS sized by the host from the bonds per point, so
@@ -236,7 +239,7 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
         uint32_t const beg = in_lds ? excl : offsets[i];
         uint32_t const end = beg + cnt;
         uint32_t k = beg;
-        fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
+        fa::visit_bonds<FORMULA>(b, g, cq, qx, qy, qz, R,
                                         [&](uint32_t j, float, float, float, float r_sq, int code) {
                                             if (r_sq < r_max_sq && r_sq >= r_min_sq && k < end &&
                                                 !(exclude_ii && j == i))
@@ -268,11 +271,13 @@ __global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, 
                                                            const uint32_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                                            float* __restrict__ w_out, float* __restrict__ v_out,
-                                                           void* __restrict__ gkeys_raw, uint32_t key_cap)
+                                                           void* __restrict__ gkeys_raw, uint32_t key_cap,
+                                                           fa::CQGrid cq)
 {
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     ball_fill_body<FORMULA, BYDIST, SELF>(b, g, qpts, p0, min(uint32_t(kBlock), nq - p0), pts, r_max_sq, r_min_sq,
-                                          exclude_ii, R, offsets, nb_out, d_out, w_out, v_out, gkeys_raw, key_cap);
+                                          exclude_ii, R, offsets, nb_out, d_out, w_out, v_out, gkeys_raw, key_cap,
+                                          cq);
 }
 
 // Segments with the reference's convention (0 for query points without bonds).
@@ -781,6 +786,38 @@ void grid_dims_for(const fa_box& box, uint32_t n, float r, int R, int dims[3])
     }
 }
 
+CQGrid cq_grid_for(const fa_box& box, float r_cut)
+{
+    // CellQuery::setupGrid (CellQuery.h:136-160) in the reference's float order
+    DevBox const db = make_devbox(box);
+    float const Lx = db.L[0], Ly = db.L[1], Lz = db.L[2];
+    CQGrid c;
+    c.inv = 1.0f / r_cut;
+    float const w_x = (Lx + Ly * std::abs(box.xy)) + Lz * std::abs(box.xz);
+    float const w_y = Ly + Lz * std::abs(box.yz);
+    float const w_z = Lz;
+    c.n[0] = int(w_x * c.inv) + 3;
+    c.n[1] = int(w_y * c.inv) + 3;
+    c.n[2] = int(w_z * c.inv) + 3;
+    float const box_min_x = Ly * std::min(0.0f, box.xy) + Lz * std::min(0.0f, box.xz);
+    float const box_min_y = Lz * std::min(0.0f, box.yz);
+    // makeAbsolute(0, 0, 0) (Box.h:212-222): lo + 0 * L, then the tilt terms
+    float ax = db.lo[0] + 0.0f * Lx, ay = db.lo[1] + 0.0f * Ly, az = db.lo[2] + 0.0f * Lz;
+    ax += box.xy * ay + box.xz * az;
+    ay += box.yz * az;
+    if (db.is2D)
+        az = 0.0f;
+    c.min[0] = (box_min_x - r_cut) + ax;
+    c.min[1] = (box_min_y - r_cut) + ay;
+    c.min[2] = -r_cut + az;
+    // CellQuery.cc:76-79: r_cut / nearest plane distance per axis
+    float npd[3];
+    nearest_plane_distance(box, npd);
+    for (int k = 0; k < 3; ++k)
+        c.frc[k] = r_cut / npd[k];
+    return c;
+}
+
 int ensure_grid(fa_nq* nq, const int dims[3])
 {
     ++nq->tick;
@@ -858,6 +895,22 @@ int validate_query_args(const fa_nq* nq, fa_query_args& a, uint32_t n_query)
     }
     else
         return fail(FA_ERR_RUNTIME, "Unknown mode");
+    if (nq->kind == FA_NQ_CELLQUERY)
+    {
+        // CellQuery::validateQueryArgs (CellQuery.h:175-204), run by query()
+        // whatever the number of query points
+        if (a.mode == FA_MODE_NEAREST)
+            return fail(FA_ERR_RUNTIME, "CellQuery only supports ball queries (r_max), not nearest queries "
+                                        "(num_neighbors). Use AABBQuery for nearest neighbor queries.");
+        float npd[3];
+        nearest_plane_distance(nq->box, npd);
+        if (npd[0] <= a.r_max * 2.0f || npd[1] <= a.r_max * 2.0f || (!nq->box.is2D && npd[2] <= a.r_max * 2.0f))
+            return fail(FA_ERR_RUNTIME, "The CellQuery r_max is too large for this box.");
+        if (a.r_max <= 0)
+            return fail(FA_ERR_INVALID, "r_max must be positive.");
+        if (a.r_max <= a.r_min)
+            return fail(FA_ERR_INVALID, "r_max must be greater than r_min.");
+    }
     if (n_query == 0)
         return FA_OK;
     if (a.r_max <= 0)
@@ -946,16 +999,19 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
     // query points == the grid's points: process them in cell order
     bool const self = (qpts == nq->d_points) && (n_query == nq->n);
+    fa::CQGrid const cq = nq->kind == FA_NQ_CELLQUERY ? fa::cq_grid_for(nq->box, a.r_max) : fa::CQGrid{};
     auto count = [&](auto kern) {
         FA_PROF_BEGIN("ball_count_kernel");
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query, r_max_sq, r_min_sq,
-                           a.exclude_ii, R, out->counts);
+                           a.exclude_ii, R, out->counts, cq);
         FA_PROF_END();
     };
     if (n_query == 0)
         ;
     else if (nq->kind == FA_NQ_AABB)
         self ? count(ball_count_kernel<AABB_FORMULA, true>) : count(ball_count_kernel<AABB_FORMULA, false>);
+    else if (nq->kind == FA_NQ_CELLQUERY)
+        self ? count(ball_count_kernel<CELLQUERY_FORMULA, true>) : count(ball_count_kernel<CELLQUERY_FORMULA, false>);
     else
         self ? count(ball_count_kernel<LINKCELL_FORMULA, true>) : count(ball_count_kernel<LINKCELL_FORMULA, false>);
     FA_HIP(hipGetLastError());
@@ -976,12 +1032,21 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
         FA_PROF_BEGIN("ball_fill_kernel");
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), kb, s, db, g, qpts, n_query, nq->d_points, r_max_sq,
                            r_min_sq, a.exclude_ii, R, out->offsets, out->neighbors, out->distances, out->weights,
-                           out->vectors, gkeys, key_cap);
+                           out->vectors, gkeys, key_cap, cq);
         FA_PROF_END();
     };
     if (n_query)
     {
-        if (nq->kind == FA_NQ_AABB)
+        if (nq->kind == FA_NQ_CELLQUERY)
+        {
+            if (self)
+                by_distance ? launch(ball_fill_kernel<CELLQUERY_FORMULA, true, true>)
+                            : launch(ball_fill_kernel<CELLQUERY_FORMULA, false, true>);
+            else
+                by_distance ? launch(ball_fill_kernel<CELLQUERY_FORMULA, true, false>)
+                            : launch(ball_fill_kernel<CELLQUERY_FORMULA, false, false>);
+        }
+        else if (nq->kind == FA_NQ_AABB)
         {
             if (self)
                 by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true, true>)
@@ -1024,6 +1089,26 @@ float knn_radius_limit(const fa_nq* nq, const fa_query_args& a)
             break;
     }
     return last;
+}
+
+// The exhaustive kNN kernel costs (images + k) candidate passes over all
+// points per query: num_neighbors > 64 sends every query there, and so does
+// an answer reaching half the smallest plane distance.  Work beyond this many
+// candidate evaluations (a few seconds) is refused with an error instead of
+// being started.
+constexpr double kExhaustiveWorkLimit = 68719476736.0; // 2^36
+
+int exhaustive_work_check(const fa_nq* nq, uint32_t n_exhaustive, uint32_t k)
+{
+    double const images = nq->box.is2D ? 9.0 : 27.0;
+    double const work = double(n_exhaustive) * double(nq->n) * (images + double(k));
+    if (work > kExhaustiveWorkLimit)
+        return fa::fail(FA_ERR_RUNTIME,
+                        "nearest-neighbour query: " + std::to_string(n_exhaustive) +
+                            " query points need the exhaustive search (num_neighbors > 64, or neighbours beyond "
+                            "half the box), " + std::to_string(n_exhaustive) + " x " + std::to_string(nq->n) +
+                            " points is beyond its limit; use num_neighbors <= 64 or fewer points");
+    return FA_OK;
 }
 
 int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args& a, int by_distance, fa_nlist* out)
@@ -1146,7 +1231,10 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         }
         else
         {
-            // every query takes the exhaustive path
+            // every query takes the exhaustive path; refuse work that would
+            // run for minutes (the check is repeated below for flagged queries)
+            if (int rc2 = exhaustive_work_check(nq, n_query, k))
+                return rc2;
             int dims[3];
             fa::grid_dims_for(nq->box, nq->n, float(std::max(rk, 1e-6)), 1, dims);
             if (int rc2 = fa::ensure_grid(nq, dims))
@@ -1199,6 +1287,8 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         return rc;
     if (h_nflag)
     {
+        if (int rc = exhaustive_work_check(nq, h_nflag, k))
+            return rc;
         // bounded work space: at most ~256 MiB of per-query candidate keys at once
         uint64_t const per_q = uint64_t(nq->n) * sizeof(uint64_t);
         uint32_t const chunk = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (256ull << 20) / per_q)));
@@ -1269,6 +1359,8 @@ int fa_nq_create(const fa_box* box, const float* d_points, uint32_t n, int kind,
         return fa::fail(FA_ERR_INVALID, "Cannot create a NeighborQuery with 0 particles.");
     if (n >= (1u << 27))
         return fa::fail(FA_ERR_INVALID, "freud_amd supports fewer than 2^27 points per NeighborQuery");
+    if (kind != FA_NQ_AABB && kind != FA_NQ_LINKCELL && kind != FA_NQ_CELLQUERY)
+        return fa::fail(FA_ERR_INVALID, "unknown NeighborQuery kind");
     fa_nq* nq = new fa_nq();
     nq->box = *box;
     nq->d_points = d_points;
@@ -1312,6 +1404,21 @@ int fa_nq_destroy(fa_nq* nq)
 int fa_nq_cell_width(const fa_nq* nq, float* width)
 {
     *width = nq->cell_width;
+    return FA_OK;
+}
+
+int fa_cellquery_grid(const fa_box* box, float r_cut, int32_t* n_xyz, float* min_pos, float* inverse_width)
+{
+    // CellQuery.cc:60-63
+    if (!(r_cut > 0))
+        return fa::fail(FA_ERR_RUNTIME, "CellQuery::buildGrid called with invalid r_cut (must be positive).");
+    fa::CQGrid const c = fa::cq_grid_for(*box, r_cut);
+    for (int k = 0; k < 3; ++k)
+    {
+        n_xyz[k] = c.n[k];
+        min_pos[k] = c.min[k];
+    }
+    *inverse_width = c.inv;
     return FA_OK;
 }
 

@@ -34,6 +34,7 @@
 namespace {
 
 using fa::AABB_FORMULA;
+using fa::CELLQUERY_FORMULA;
 using fa::DevBox;
 using fa::DevGrid;
 using fa::LINKCELL_FORMULA;
@@ -811,7 +812,8 @@ template<int FORMULA, bool LDS_HIST>
 __global__ __launch_bounds__(kBlock) void rdf_self_kernel(DevBox b, DevGrid g, uint32_t n, float q_rmax_sq,
                                                           float q_rmin_sq, int exclude_ii, int R,
                                                           const float* __restrict__ T, uint32_t nbins, float vmin,
-                                                          float inv_width, uint32_t* __restrict__ counts)
+                                                          float inv_width, uint32_t* __restrict__ counts,
+                                                          fa::CQGrid cq)
 {
     extern __shared__ float smem[];
     float* sT = smem;
@@ -823,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void rdf_self_kernel(DevBox b, DevGrid g, u
     {
         float4 const q = g.cpos[t];
         uint32_t const qi = __float_as_uint(q.w);
-        fa::for_each_candidate<FORMULA>(b, g, q.x, q.y, q.z, R, false,
+        fa::visit_bonds<FORMULA>(b, g, cq, q.x, q.y, q.z, R,
                                         [&](uint32_t j, float, float, float, float r_sq, int) {
                                             if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq && !(exclude_ii && j == qi))
                                             {
@@ -842,7 +844,7 @@ __global__ __launch_bounds__(kBlock) void rdf_query_kernel(DevBox b, DevGrid g, 
                                                            uint32_t nq, float q_rmax_sq, float q_rmin_sq,
                                                            int exclude_ii, int R, const float* __restrict__ T,
                                                            uint32_t nbins, float vmin, float inv_width,
-                                                           uint32_t* __restrict__ counts)
+                                                           uint32_t* __restrict__ counts, fa::CQGrid cq)
 {
     extern __shared__ float smem[];
     float* sT = smem;
@@ -853,7 +855,7 @@ __global__ __launch_bounds__(kBlock) void rdf_query_kernel(DevBox b, DevGrid g, 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x)
     {
         float const qx = qpts[3 * i], qy = qpts[3 * i + 1], qz = qpts[3 * i + 2];
-        fa::for_each_candidate<FORMULA>(b, g, qx, qy, qz, R, false,
+        fa::visit_bonds<FORMULA>(b, g, cq, qx, qy, qz, R,
                                         [&](uint32_t j, float, float, float, float r_sq, int) {
                                             if (r_sq < q_rmax_sq && r_sq >= q_rmin_sq && !(exclude_ii && j == i))
                                             {
@@ -1419,23 +1421,30 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
         else                                                                                      \
             FA_KLAUNCH((KERNEL<FORM, false>), dim3(blocks), dim3(kBlock), shmem, s, __VA_ARGS__); \
     } while (0)
+    fa::CQGrid const cq = nq->kind == FA_NQ_CELLQUERY ? fa::cq_grid_for(nq->box, q_r_max) : fa::CQGrid{};
     if (self)
     {
         if (nq->kind == FA_NQ_AABB)
             FA_RDF_LAUNCH(rdf_self_kernel, AABB_FORMULA, db, g, n_query, qmax_sq, qmin_sq, exclude_ii, R, T, bins,
-                          r_min, inv_width, d_counts);
+                          r_min, inv_width, d_counts, cq);
+        else if (nq->kind == FA_NQ_CELLQUERY)
+            FA_RDF_LAUNCH(rdf_self_kernel, CELLQUERY_FORMULA, db, g, n_query, qmax_sq, qmin_sq, exclude_ii, R, T,
+                          bins, r_min, inv_width, d_counts, cq);
         else
             FA_RDF_LAUNCH(rdf_self_kernel, LINKCELL_FORMULA, db, g, n_query, qmax_sq, qmin_sq, exclude_ii, R, T, bins,
-                          r_min, inv_width, d_counts);
+                          r_min, inv_width, d_counts, cq);
     }
     else
     {
         if (nq->kind == FA_NQ_AABB)
             FA_RDF_LAUNCH(rdf_query_kernel, AABB_FORMULA, db, g, d_query_points, n_query, qmax_sq, qmin_sq,
-                          exclude_ii, R, T, bins, r_min, inv_width, d_counts);
+                          exclude_ii, R, T, bins, r_min, inv_width, d_counts, cq);
+        else if (nq->kind == FA_NQ_CELLQUERY)
+            FA_RDF_LAUNCH(rdf_query_kernel, CELLQUERY_FORMULA, db, g, d_query_points, n_query, qmax_sq, qmin_sq,
+                          exclude_ii, R, T, bins, r_min, inv_width, d_counts, cq);
         else
             FA_RDF_LAUNCH(rdf_query_kernel, LINKCELL_FORMULA, db, g, d_query_points, n_query, qmax_sq, qmin_sq,
-                          exclude_ii, R, T, bins, r_min, inv_width, d_counts);
+                          exclude_ii, R, T, bins, r_min, inv_width, d_counts, cq);
     }
 #undef FA_RDF_LAUNCH
     FA_HIP(hipGetLastError());

@@ -478,6 +478,19 @@ class LinkCell(NeighborQuery):
         return w.value
 
 
+class CellQuery(NeighborQuery):
+    """freud.locality.CellQuery (freud/locality.py:900-921): ball queries over
+    a cell grid of width r_max holding the points and their ghost copies near
+    the faces; bond vectors r = (p_j + ghost shift) - q_i
+    (CellIterator.h:177-186).  Nearest-neighbour queries raise RuntimeError
+    (CellQuery.h:178-183)."""
+
+    _kind = _lib.NQ_CELLQUERY
+
+    def __init__(self, box, points):
+        self._setup(box, points)
+
+
 class _PairCompute(_Compute):
     """freud/locality.py:924-1021."""
 

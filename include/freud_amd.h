@@ -30,6 +30,7 @@ extern "C" {
 
 #define FA_NQ_AABB 0     /* freud.locality.AABBQuery: r = p_j - (q_i + image) */
 #define FA_NQ_LINKCELL 1 /* freud.locality.LinkCell:  r = box.wrap(p_j - q_i)  */
+#define FA_NQ_CELLQUERY 2 /* freud.locality.CellQuery: r = (p_j + ghost shift) - q_i, ball queries only */
 
 #define FA_MODE_NONE 0
 #define FA_MODE_BALL 1
@@ -181,14 +182,19 @@ int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, int* rdf_F, uint64_t* 
 int fa_box_transform(const fa_box* box, int op, const float* d_in, float* d_out, uint64_t n);
 
 /* ---------------------------------------------------------------- locality
- * NeighborQuery constructors: AABBQuery (freud/locality/AABBQuery.cc:18-26) and
+ * NeighborQuery constructors: AABBQuery (freud/locality/AABBQuery.cc:18-26),
  * LinkCell (freud/locality/LinkCell.cc:222-260; cell_width 0 = the reference's
- * default width).  d_points (n x 3) is caller-owned and must outlive the handle
+ * default width) and CellQuery (freud/locality/CellQuery.h:29-40; its ghost
+ * grid is a function of each query's r_max, CellQuery.cc:24-34).  d_points (n x 3) is caller-owned and must outlive the handle
  * (the reference also keeps a raw pointer, NeighborQuery.h:93-95). */
 int fa_nq_create(const fa_box* box, const float* d_points, uint32_t n, int kind, float cell_width, fa_nq** out);
 int fa_nq_destroy(fa_nq* nq);
 /* LinkCell::getCellWidth (LinkCell.h) */
 int fa_nq_cell_width(const fa_nq* nq, float* width);
+/* The cell grid a CellQuery ball query of radius r_cut uses
+ * (freud/locality/CellQuery.h:136-160 setupGrid: getNx/getNy/getNz,
+ * getMinPos, getCellInverseWidth).  Host arithmetic only; no device work. */
+int fa_cellquery_grid(const fa_box* box, float r_cut, int32_t* n_xyz, float* min_pos, float* inverse_width);
 
 /* NeighborQuery::query(...)->toNeighborList(sort_by_distance)
  * (freud/locality/NeighborQuery.h:131-142, 434-481; binding

@@ -23,6 +23,7 @@ __all__ = [
     "box_op",
     "nearest_plane_distance",
     "query",
+    "cellquery_grid",
     "all_pairs",
     "rdf_accumulate",
     "rdf_bin",
@@ -79,6 +80,7 @@ def _load():
         _lib.orc_wigner3j_table.restype = ctypes.c_uint32
         _lib.orc_cluster.restype = ctypes.c_uint32
         _lib.orc_solidliquid.restype = ctypes.c_uint32
+        _lib.orc_cellquery_grid.restype = ctypes.c_uint32
     return _lib
 
 
@@ -164,8 +166,9 @@ def _take_nlist(nl):
 def query(box, points, query_points, qargs, nq_type="aabb", cell_width=0.0, sort_by_distance=False, nthreads=1):
     """Ball / kNN query -> dict(ij, distances, weights, vectors) in NeighborList order.
 
-    nq_type: 'aabb' (AABBQuery formula r = p_j - (q_i + image)) or 'linkcell'
-    (LinkCell formula r = box.wrap(p_j - q_i)).
+    nq_type: 'aabb' (AABBQuery formula r = p_j - (q_i + image)), 'linkcell'
+    (LinkCell formula r = box.wrap(p_j - q_i)) or 'cellquery' (CellQuery: ghosts
+    near the faces, r = (p_j + shift) - q_i; ball queries only).
     """
     lib = _load()
     b, is2d = _boxp(box)
@@ -175,11 +178,33 @@ def query(box, points, query_points, qargs, nq_type="aabb", cell_width=0.0, sort
     nl = _Nlist()
     rc = lib.orc_query(
         _p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(qpts), ctypes.c_uint32(len(qpts)),
-        {"aabb": 0, "linkcell": 1}[nq_type], ctypes.c_float(cell_width), qp, int(bool(sort_by_distance)),
+        {"aabb": 0, "linkcell": 1, "cellquery": 2}[nq_type], ctypes.c_float(cell_width), qp, int(bool(sort_by_distance)),
         int(nthreads), ctypes.byref(nl))
     if rc:
         _check(rc)
     return _take_nlist(nl)
+
+
+def cellquery_grid(box, points, r_cut):
+    """CellQuery's grid for radius r_cut (CellQuery.h:136-160, CellQuery.cc:56-113):
+    dict(n=(nx, ny, nz), min_pos, inverse_width, n_total, counts, counts_real),
+    the per-cell arrays shaped (nz, ny, nx)."""
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    n = np.zeros(3, np.int32)
+    mn = np.zeros(3, np.float32)
+    inv = ctypes.c_float()
+    lib.orc_cellquery_grid(_p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), ctypes.c_float(r_cut), _p(n), _p(mn),
+                           ctypes.byref(inv), None, None)
+    ncell = int(n[0]) * int(n[1]) * int(n[2])
+    counts = np.zeros(ncell, np.uint32)
+    real = np.zeros(ncell, np.uint32)
+    total = lib.orc_cellquery_grid(_p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), ctypes.c_float(r_cut), _p(n),
+                                   _p(mn), ctypes.byref(inv), _p(counts), _p(real))
+    shape = (int(n[2]), int(n[1]), int(n[0]))
+    return dict(n=tuple(int(v) for v in n), min_pos=mn, inverse_width=np.float32(inv.value), n_total=int(total),
+                counts=counts.reshape(shape), counts_real=real.reshape(shape))
 
 
 def all_pairs(box, points, query_points, exclude_ii=True):
@@ -226,7 +251,7 @@ def rdf_accumulate(box, points, query_points, bins, r_max, r_min=0.0, exclude_ii
     qrm = r_max if query_r_max is None else query_r_max
     rc = lib.orc_rdf_accumulate(
         _p(b), is2d, _p(pts), ctypes.c_uint32(len(pts)), _p(qpts), ctypes.c_uint32(len(qpts)),
-        {"aabb": 0, "linkcell": 1}[nq_type], ctypes.c_float(cell_width), ctypes.c_float(qrm),
+        {"aabb": 0, "linkcell": 1, "cellquery": 2}[nq_type], ctypes.c_float(cell_width), ctypes.c_float(qrm),
         ctypes.c_float(query_r_min), int(bool(exclude_ii)), ctypes.c_uint32(bins), ctypes.c_float(r_min),
         ctypes.c_float(r_max), int(nthreads), _p(counts))
     _check(rc)

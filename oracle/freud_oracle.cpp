@@ -15,6 +15,10 @@
 //                                    tests/test_box_box.py
 //   ball query (AABB, LinkCell) .... numpy brute force with the same per-formula
 //                                    float32 arithmetic; hand-built known answers
+//   ball query (CellQuery) ......... the reference's AABB == CellQuery bond-set
+//                                    tests; numpy brute force of the ghost
+//                                    arithmetic (p_j + shift) - q_i; closed-form
+//                                    grid sizes; the reference's error cases
 //   kNN ............................ brute-force k-smallest (also with r_min);
 //                                    known answers incl. the reference's r_min
 //                                    case; ties are broken by point index
@@ -496,6 +500,234 @@ void linkcell_ball_point(const LinkCell& lc, const std::vector<v3>& pts, v3 q, u
     }
 }
 
+// ---------------------------------------------------------------- CellQuery
+// A cubic grid of width r_cut over the box's bounding parallelepiped plus one
+// cell of padding, holding every point and the ghost copies generateGhosts
+// makes for points within r_cut (fractionally) of a face.
+struct CellQueryGrid
+{
+    float inv;
+    v3 min;
+    int n[3];
+    std::vector<uint32_t> start; // cell c holds entries [start[c], start[c + 1])
+    std::vector<v3> pos;         // point or ghost position
+    std::vector<uint32_t> id;    // index of the real point
+    std::vector<uint32_t> real;  // real points per cell
+
+    // CellQuery.h:53-75 (cell_idx_xyz / getCellIdxSafe)
+    bool cell_of(v3 p, int c[3]) const
+    {
+        c[0] = int(std::floor((p.x - min.x) * inv));
+        c[1] = int(std::floor((p.y - min.y) * inv));
+        c[2] = int(std::floor((p.z - min.z) * inv));
+        return c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < n[0] && c[1] < n[1] && c[2] < n[2];
+    }
+};
+
+// CellQuery.h:136-160 (setupGrid), CellQuery.cc:56-113 (buildGrid) and
+// CellQuery.h:221-298 (generateGhosts: faces, then edges, then the corner;
+// each shift sums +-a, +-b, +-c in that order onto zero).  Entries of a cell
+// are kept real points first, then ghosts (placeParticlesInSortedOrder).
+void build_cellquery(CellQueryGrid& g, const Box& b, const std::vector<v3>& pts, float r_cut)
+{
+    g.inv = 1.0f / r_cut;
+    float const Lx = b.L[0], Ly = b.L[1], Lz = b.L[2];
+    float const w_x = Lx + (Ly * std::abs(b.xy)) + (Lz * std::abs(b.xz));
+    float const w_y = Ly + (Lz * std::abs(b.yz));
+    float const w_z = Lz;
+    g.n[0] = int(w_x * g.inv) + 3;
+    g.n[1] = int(w_y * g.inv) + 3;
+    g.n[2] = int(w_z * g.inv) + 3;
+    float const box_min_x = (Ly * std::min(0.0f, b.xy)) + (Lz * std::min(0.0f, b.xz));
+    float const box_min_y = Lz * std::min(0.0f, b.yz);
+    v3 const a0 = make_absolute(b, {0.0f, 0.0f, 0.0f});
+    g.min = add({box_min_x - r_cut, box_min_y - r_cut, -r_cut}, a0);
+    v3 const npd = nearest_plane_distance(b);
+    v3 const frc = {r_cut / npd.x, r_cut / npd.y, r_cut / npd.z};
+    v3 const la = {Lx, 0.0f, 0.0f};
+    v3 const lb = {Ly * b.xy, Ly, 0.0f};
+    v3 const lc = b.is2D ? v3{0.0f, 0.0f, 0.0f} : v3{Lz * b.xz, Lz * b.yz, Lz};
+    size_t const ncell = size_t(g.n[0]) * g.n[1] * g.n[2];
+    std::vector<uint32_t> real_count(ncell, 0), count(ncell, 0);
+    struct Entry
+    {
+        uint32_t cell;
+        v3 p;
+        uint32_t id;
+        bool ghost;
+    };
+    std::vector<Entry> entries;
+    entries.reserve(pts.size());
+    auto cell_index = [&](const int c[3]) { return uint32_t((c[2] * g.n[1] + c[1]) * g.n[0] + c[0]); };
+    for (size_t i = 0; i < pts.size(); ++i)
+    {
+        v3 const p = pts[i];
+        v3 const f = make_fractional(b, p);
+        int const dx = int(f.x <= frc.x) - int(double(f.x) >= 1.0 - double(frc.x));
+        int const dy = int(f.y <= frc.y) - int(double(f.y) >= 1.0 - double(frc.y));
+        int dz = int(f.z <= frc.z) - int(double(f.z) >= 1.0 - double(frc.z));
+        if (b.is2D)
+            dz = 0;
+        std::vector<v3> shifts;
+        auto site = [&](int si, int sj, int sk) {
+            v3 sh = {0.0f, 0.0f, 0.0f};
+            if (si != 0)
+                sh = add(sh, si == 1 ? la : v3{-la.x, -la.y, -la.z});
+            if (sj != 0)
+                sh = add(sh, sj == 1 ? lb : v3{-lb.x, -lb.y, -lb.z});
+            if (sk != 0)
+                sh = add(sh, sk == 1 ? lc : v3{-lc.x, -lc.y, -lc.z});
+            shifts.push_back(sh);
+        };
+        if (dx != 0 || dy != 0 || dz != 0)
+        {
+            if (dx != 0)
+                site(dx, 0, 0);
+            if (dy != 0)
+                site(0, dy, 0);
+            if (dz != 0)
+                site(0, 0, dz);
+            if (dx != 0 && dy != 0)
+                site(dx, dy, 0);
+            if (dx != 0 && dz != 0)
+                site(dx, 0, dz);
+            if (dy != 0 && dz != 0)
+                site(0, dy, dz);
+            if (dx != 0 && dy != 0 && dz != 0)
+                site(dx, dy, dz);
+        }
+        int c[3];
+        for (const v3& sh : shifts)
+        {
+            v3 const gp = add(p, sh);
+            if (g.cell_of(gp, c))
+            {
+                entries.push_back({cell_index(c), gp, uint32_t(i), true});
+                ++count[cell_index(c)];
+            }
+        }
+        if (g.cell_of(p, c))
+        {
+            entries.push_back({cell_index(c), p, uint32_t(i), false});
+            ++count[cell_index(c)];
+            ++real_count[cell_index(c)];
+        }
+    }
+    g.start.assign(ncell + 1, 0);
+    for (size_t c = 0; c < ncell; ++c)
+        g.start[c + 1] = g.start[c] + count[c];
+    g.real = real_count;
+    std::vector<uint32_t> real_fill(ncell, 0), ghost_fill(ncell, 0);
+    g.pos.resize(entries.size());
+    g.id.resize(entries.size());
+    for (const Entry& e : entries)
+    {
+        uint32_t const at = e.ghost ? g.start[e.cell] + real_count[e.cell] + ghost_fill[e.cell]++
+                                    : g.start[e.cell] + real_fill[e.cell]++;
+        g.pos[at] = e.p;
+        g.id[at] = e.id;
+    }
+}
+
+// CellIterator.h:94-187 (CellQueryBallIterator): the 3 x 3 x 3 cells around the
+// query point's cell (none if that cell is outside the grid), r = entry - q.
+template<typename Emit>
+void cellquery_ball_point(const CellQueryGrid& g, v3 q, uint32_t qi, float r_max, float r_min, bool exclude_ii,
+                          Emit&& emit)
+{
+    float const r_max_sq = r_max * r_max;
+    float const r_min_sq = r_min * r_min;
+    int c[3];
+    if (!g.cell_of(q, c))
+        return;
+    for (int dz = -1; dz <= 1; ++dz)
+    {
+        int const z = c[2] + dz;
+        if (z < 0 || z >= g.n[2])
+            continue;
+        for (int dy = -1; dy <= 1; ++dy)
+        {
+            int const y = c[1] + dy;
+            if (y < 0 || y >= g.n[1])
+                continue;
+            int const x0 = std::max(c[0] - 1, 0), x1 = std::min(c[0] + 1, g.n[0] - 1);
+            uint32_t const row = uint32_t((z * g.n[1] + y) * g.n[0]);
+            for (uint32_t k = g.start[row + x0]; k < g.start[row + x1 + 1]; ++k)
+            {
+                uint32_t const j = g.id[k];
+                if (exclude_ii && j == qi)
+                    continue;
+                v3 const r = sub(g.pos[k], q);
+                float const r_sq = dot(r, r);
+                if (r_sq < r_max_sq && r_sq >= r_min_sq)
+                    emit(j, r, r_sq);
+            }
+        }
+    }
+}
+
+thread_local char g_err[256];
+
+// CellQuery::validateQueryArgs (CellQuery.h:175-204) after the base class's
+// mode inference and ball / nearest checks (NeighborQuery.h:195-228).
+int validate_cellquery(const Box& b, QueryParams* qp)
+{
+    if (qp->mode == 0)
+    {
+        if (qp->k != 0xffffffffu)
+            qp->mode = 2;
+        else if (qp->r_max != -1.0f)
+            qp->mode = 1;
+    }
+    if (qp->mode == 1)
+    {
+        if (qp->r_max == -1.0f)
+        {
+            std::snprintf(g_err, sizeof g_err, "You must set r_max in the query arguments when performing ball queries.");
+            return -2;
+        }
+        if (qp->k != 0xffffffffu)
+        {
+            std::snprintf(g_err, sizeof g_err, "You cannot set num_neighbors in the query arguments when performing ball queries.");
+            return -2;
+        }
+    }
+    else if (qp->mode == 2)
+    {
+        if (qp->k == 0xffffffffu)
+        {
+            std::snprintf(g_err, sizeof g_err, "You must set num_neighbors in the query arguments when performing number of neighbor queries.");
+            return -2;
+        }
+        std::snprintf(g_err, sizeof g_err,
+                      "CellQuery only supports ball queries (r_max), not nearest queries (num_neighbors). Use "
+                      "AABBQuery for nearest neighbor queries.");
+        return -2;
+    }
+    else
+    {
+        std::snprintf(g_err, sizeof g_err, "Unknown mode");
+        return -2;
+    }
+    v3 const npd = nearest_plane_distance(b);
+    if (npd.x <= qp->r_max * 2.0f || npd.y <= qp->r_max * 2.0f || (!b.is2D && npd.z <= qp->r_max * 2.0f))
+    {
+        std::snprintf(g_err, sizeof g_err, "The CellQuery r_max is too large for this box.");
+        return -2;
+    }
+    if (qp->r_max <= 0)
+    {
+        std::snprintf(g_err, sizeof g_err, "r_max must be positive.");
+        return -1;
+    }
+    if (qp->r_max <= qp->r_min)
+    {
+        std::snprintf(g_err, sizeof g_err, "r_max must be greater than r_min.");
+        return -1;
+    }
+    return 0;
+}
+
 inline bool knn_less(const Bond& a, const Bond& b)
 {
     // distance first; ties broken by point index (reference tie order is unpinned)
@@ -545,8 +777,6 @@ std::vector<v3> to_v3(const float* p, uint32_t n)
         v[i] = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     return v;
 }
-
-thread_local char g_err[256];
 
 } // namespace
 
@@ -659,7 +889,8 @@ int orc_validate_query(const float* boxp, int is2D, uint32_t n_points, QueryPara
     return 0;
 }
 
-// Full query -> sorted NeighborList.  nq_type 0 = AABBQuery, 1 = LinkCell.
+// Full query -> sorted NeighborList.  nq_type 0 = AABBQuery, 1 = LinkCell,
+// 2 = CellQuery (ball queries only).
 int orc_query(const float* boxp, int is2D, const float* pts_, uint32_t np, const float* qpts_, uint32_t nq,
               int nq_type, float cell_width, QueryParams qp, int sort_by_distance, int nthreads,
               NlistOut* out)
@@ -668,14 +899,27 @@ int orc_query(const float* boxp, int is2D, const float* pts_, uint32_t np, const
     out->dist = out->w = out->vec = nullptr;
     out->n = 0;
     Box b = make_box(boxp, is2D);
-    int rc = orc_validate_query(boxp, is2D, np, &qp);
+    int rc = nq_type == 2 ? validate_cellquery(b, &qp) : orc_validate_query(boxp, is2D, np, &qp);
     if (rc)
         return rc;
     std::vector<v3> pts = to_v3(pts_, np), qpts = to_v3(qpts_, nq);
     v3 npd = nearest_plane_distance(b);
     std::vector<std::vector<Bond>> per_thread(std::max(nthreads, 1));
 
-    if (nq_type == 0)
+    if (nq_type == 2)
+    {
+        CellQueryGrid grid;
+        build_cellquery(grid, b, pts, qp.r_max);
+        parallel_for(nq, nthreads, [&](size_t beg, size_t end, int t) {
+            auto& local = per_thread[t];
+            for (size_t i = beg; i < end; ++i)
+                cellquery_ball_point(grid, qpts[i], uint32_t(i), qp.r_max, qp.r_min, qp.exclude_ii,
+                                     [&](uint32_t j, v3 r, float r_sq) {
+                                         local.push_back({uint32_t(i), j, std::sqrt(r_sq), 1.0f, r});
+                                     });
+        });
+    }
+    else if (nq_type == 0)
     {
         if (qp.mode == 1)
         {
@@ -834,6 +1078,32 @@ int orc_query(const float* boxp, int is2D, const float* pts_, uint32_t np, const
     return 0;
 }
 
+// CellQuery grid for radius r_cut over a point set (CellQuery.h:136-160,
+// CellQuery.cc:56-113): dims, lower corner, inverse width; when counts is not
+// null, total and real occupancy per cell (ncell = nx*ny*nz entries, index
+// (z * ny + y) * nx + x).  Returns the number of entries, points plus ghosts.
+uint32_t orc_cellquery_grid(const float* boxp, int is2D, const float* pts_, uint32_t np, float r_cut, int* n,
+                            float* mn, float* inv, uint32_t* counts, uint32_t* counts_real)
+{
+    Box b = make_box(boxp, is2D);
+    CellQueryGrid g;
+    build_cellquery(g, b, to_v3(pts_, np), r_cut);
+    for (int k = 0; k < 3; ++k)
+        n[k] = g.n[k];
+    mn[0] = g.min.x;
+    mn[1] = g.min.y;
+    mn[2] = g.min.z;
+    *inv = g.inv;
+    size_t const ncell = size_t(g.n[0]) * g.n[1] * g.n[2];
+    if (counts)
+        for (size_t c = 0; c < ncell; ++c)
+        {
+            counts[c] = g.start[c + 1] - g.start[c];
+            counts_real[c] = g.real[c];
+        }
+    return uint32_t(g.pos.size());
+}
+
 // NeighborList.cc:84-128 all_pairs (note the reference's vector is wrap(q_i - p_j))
 int orc_all_pairs(const float* boxp, int is2D, const float* pts_, uint32_t np, const float* qpts_, uint32_t nq,
                   int exclude_ii, NlistOut* out)
@@ -915,7 +1185,7 @@ int orc_rdf_accumulate(const float* boxp, int is2D, const float* pts_, uint32_t 
 {
     Box b = make_box(boxp, is2D);
     QueryParams qp = {1, 0xffffffffu, q_r_max, q_r_min, -1.0f, -1.0f, exclude_ii};
-    int rc = orc_validate_query(boxp, is2D, np, &qp);
+    int rc = nq_type == 2 ? validate_cellquery(b, &qp) : orc_validate_query(boxp, is2D, np, &qp);
     if (rc)
         return rc;
     v3 npd = nearest_plane_distance(b);
@@ -924,7 +1194,22 @@ int orc_rdf_accumulate(const float* boxp, int is2D, const float* pts_, uint32_t 
     std::vector<v3> pts = to_v3(pts_, np), qpts = to_v3(qpts_, nq);
     int nt = std::max(nthreads, 1);
     std::vector<std::vector<uint32_t>> local(nt, std::vector<uint32_t>(bins, 0));
-    if (nq_type == 0)
+    if (nq_type == 2)
+    {
+        CellQueryGrid grid;
+        build_cellquery(grid, b, pts, qp.r_max);
+        parallel_for(nq, nthreads, [&](size_t beg, size_t end, int t) {
+            auto& h = local[t];
+            for (size_t i = beg; i < end; ++i)
+                cellquery_ball_point(grid, qpts[i], uint32_t(i), qp.r_max, qp.r_min, qp.exclude_ii,
+                                     [&](uint32_t, v3, float r_sq) {
+                                         long k = regular_bin(std::sqrt(r_sq), r_min, r_max, inv, bins);
+                                         if (k >= 0)
+                                             h[k] += 1;
+                                     });
+        });
+    }
+    else if (nq_type == 0)
     {
         if (npd.x <= qp.r_max * 2.0 || npd.y <= qp.r_max * 2.0 || (!b.is2D && npd.z <= qp.r_max * 2.0))
         {

@@ -3,7 +3,8 @@ to check the oracle's tree and cell searches (test helpers).
 
 All arithmetic is float32 with no fused multiply-add, in the operation order of
 freud/locality/AABBQuery.cc:93,118-127 (AABB formula), LinkCell.cc:522-525
-(LinkCell formula) and NeighborQuery.h:549-556 (image vectors).
+(LinkCell formula), NeighborQuery.h:549-556 (image vectors) and
+CellQuery.h:136-298 / CellIterator.h:94-187 (CellQuery ghosts and grid).
 """
 
 import numpy as np
@@ -69,6 +70,94 @@ def ball_linkcell(box, points, query_points, r_max, r_min=0.0, exclude_ii=False)
         if exclude_ii and i == j:
             continue
         bonds.append((i, j, np.sqrt(rsq[i, j]), r[i, j]))
+    return _to_nlist(bonds)
+
+
+def cellquery_min_pos(box, r_cut):
+    """Lower grid corner (box_min_x - r, box_min_y - r, -r) + makeAbsolute(0, 0, 0)."""
+    r = f32(r_cut)
+    Ly = f32(box.Ly)
+    Lz = f32(0 if box.is2D else box.Lz)
+    bmx = f32(Ly * min(f32(0), f32(box.xy))) + f32(Lz * min(f32(0), f32(box.xz)))
+    bmy = f32(Lz * min(f32(0), f32(box.yz)))
+    a0 = box.make_absolute(np.zeros(3, f32))
+    return np.array([f32(bmx - r) + a0[0], f32(bmy - r) + a0[1], f32(-r) + a0[2]], dtype=f32)
+
+
+def cellquery_entries(box, points, r_cut):
+    """Every entry of CellQuery's grid as (point index, position): the points
+    and each ghost p + shift whose shift generateGhosts makes (components s_a
+    in {-1, 0, 1}, each nonzero one equal to the point's face flag
+    d_a = [f_a <= r/npd_a] - [f_a >= 1 - r/npd_a], d_z = 0 in 2D), all kept
+    only when their cell lies in the grid.  Brute force over all 27 shifts."""
+    p = np.asarray(points, f32)
+    r = f32(r_cut)
+    npd = box.nearest_plane_distance()
+    f = box.make_fractional(p)
+    frc = np.array([r / f32(npd[0]), r / f32(npd[1]), (r / f32(npd[2])) if not box.is2D else f32(np.inf)],
+                   dtype=f32)
+    d = (f <= frc).astype(int) - (f.astype(np.float64) >= 1.0 - frc.astype(np.float64)).astype(int)
+    if box.is2D:
+        d[:, 2] = 0
+    Lx, Ly = f32(box.Lx), f32(box.Ly)
+    Lz = f32(0 if box.is2D else box.Lz)
+    lat = [np.array([Lx, 0, 0], f32), np.array([Ly * f32(box.xy), Ly, 0], f32),
+           np.zeros(3, f32) if box.is2D else np.array([Lz * f32(box.xz), Lz * f32(box.yz), Lz], f32)]
+    idx, pos = [np.arange(len(p))], [p]
+    for s in np.ndindex(3, 3, 3):
+        s = np.array(s) - 1
+        if not s.any():
+            continue
+        ok = np.all((s == 0) | (s[None, :] == d), axis=1)
+        sh = np.zeros(3, f32)
+        for a in range(3):
+            if s[a]:
+                sh = (sh + (lat[a] if s[a] == 1 else -lat[a])).astype(f32)
+        idx.append(np.nonzero(ok)[0])
+        pos.append((p[ok] + sh).astype(f32))
+    idx = np.concatenate(idx)
+    pos = np.concatenate(pos).astype(f32)
+    cell = cellquery_cells(box, pos, r)
+    inside = np.all(cell >= 0, axis=1) & np.all(cell < np.array(cellquery_dims(box, r)), axis=1)
+    return idx[inside], pos[inside]
+
+
+def cellquery_dims(box, r_cut):
+    inv = f32(1) / f32(r_cut)
+    Lx, Ly = f32(box.Lx), f32(box.Ly)
+    Lz = f32(0 if box.is2D else box.Lz)
+    wx = f32(f32(Lx + f32(Ly * abs(f32(box.xy)))) + f32(Lz * abs(f32(box.xz))))
+    wy = f32(Ly + f32(Lz * abs(f32(box.yz))))
+    return (int(f32(wx * inv)) + 3, int(f32(wy * inv)) + 3, int(f32(Lz * inv)) + 3)
+
+
+def cellquery_cells(box, pos, r_cut):
+    inv = f32(1) / f32(r_cut)
+    mn = cellquery_min_pos(box, r_cut)
+    return np.floor(((np.asarray(pos, f32) - mn).astype(f32) * inv).astype(f32)).astype(np.int64)
+
+
+def ball_cellquery(box, points, query_points, r_max, r_min=0.0, exclude_ii=False):
+    """All (i, entry) with the entry's cell within one cell (Chebyshev) of query
+    point i's cell, i's cell inside the grid, r = entry - q_i,
+    r_sq < r_max^2 and r_sq >= r_min^2."""
+    j, pos = cellquery_entries(box, points, r_max)
+    q = np.asarray(query_points, f32)
+    dims = np.array(cellquery_dims(box, r_max))
+    cq = cellquery_cells(box, q, r_max)
+    ce = cellquery_cells(box, pos, r_max)
+    q_in = np.all(cq >= 0, axis=1) & np.all(cq < dims, axis=1)
+    near = np.all(np.abs(cq[:, None, :] - ce[None, :, :]) <= 1, axis=2) & q_in[:, None]
+    r = (pos[None, :, :] - q[:, None, :]).astype(f32)
+    rsq = dot3(r)
+    rmax2 = f32(r_max) * f32(r_max)
+    rmin2 = f32(r_min) * f32(r_min)
+    keep = near & (rsq < rmax2) & (rsq >= rmin2)
+    if exclude_ii:
+        keep &= j[None, :] != np.arange(len(q))[:, None]
+    bonds = []
+    for i, e in zip(*np.nonzero(keep)):
+        bonds.append((i, j[e], np.sqrt(rsq[i, e]), r[i, e]))
     return _to_nlist(bonds)
 
 

@@ -155,6 +155,22 @@ def test_knn_exhaustive_paths(oracle):
         _assert_nlist_equal(nl, ref)
 
 
+def test_knn_large_k(oracle):
+    """num_neighbors above 64 (the exhaustive kernel) on 4096 uniform points,
+    k = 65, 100 and 128; and a clear error instead of minutes of work when the
+    exhaustive search would cover too many point pairs (checked on the host
+    before any kernel runs)."""
+    box, pts = SYSTEMS[0][1], SYSTEMS[0][2]
+    aq = freud_amd.AABBQuery(box, pts)
+    for k in (65, 100, 128):
+        qa = dict(num_neighbors=k, exclude_ii=True)
+        nl = aq.query(pts, qa).toNeighborList()
+        _assert_nlist_equal(nl, oracle.query(_bt(box), pts, pts, qa, nthreads=8))
+    big_box, big = freud_amd.data.fcc_jittered(40, 1.5874, 0.05 * 1.5874, seed=2)  # 256,000 points
+    with pytest.raises(RuntimeError, match="exhaustive search"):
+        freud_amd.AABBQuery(big_box, big).query(big, dict(num_neighbors=80)).toNeighborList()
+
+
 @pytest.mark.parametrize("name,box,pts", SYSTEMS[:2] + SYSTEMS[3:], ids=[s[0] for s in SYSTEMS[:2] + SYSTEMS[3:]])
 def test_knn_after_ball_query(oracle, name, box, pts):
     """A kNN query on a neighbour-query object whose ball query already built a

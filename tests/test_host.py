@@ -230,3 +230,44 @@ def test_tuner_variant_counts():
     assert _lib.tune_variants(3, 4) == 4
     with pytest.raises(ValueError):
         _lib.tune_variants(9, 0)
+
+
+@pytest.mark.parametrize(
+    ("box", "r_cut", "cells"),
+    [
+        # test_locality_neighbor_query.py:879-899 of the reference
+        (Box.cube(10), 2.5, [7, 7, 7]),
+        (Box.cube(10), 2.4999, [7, 7, 7]),
+        (Box(1000, 100, 10), 1.0, [1003, 103, 13]),
+        (Box(10000, 1000, 100), 1.0, [10003, 1003, 103]),
+        (Box(2.0, 2.0, 2.0, 1.5, 1.0, 1.0), 1.0, [10, 7, 5]),
+        (Box(1.0, 2.0, 3.0, 1.5, 0.5, 1.0), 0.4, [16, 15, 10]),
+        (Box.square(5), 2.49, None),
+        (Box(6.0, 6.0, 5.0, 0.1, -20.3, 0.0), 1.0, None),
+    ],
+)
+def test_cellquery_grid_matches_oracle(oracle, box, r_cut, cells):
+    """The library's CellQuery grid (host arithmetic) against the reference's
+    known cell counts and, where the oracle can build the grid, its lower
+    corner and inverse width bit for bit."""
+    import brute
+
+    n = np.zeros(3, np.int32)
+    mn = np.zeros(3, np.float32)
+    inv = ctypes.c_float()
+    cbox = _lib.make_box(box)
+    assert _lib.lib().fa_cellquery_grid(ctypes.byref(cbox), r_cut, n.ctypes.data, mn.ctypes.data,
+                                        ctypes.byref(inv)) == 0
+    if cells is not None:
+        assert list(n) == cells
+    assert tuple(n) == brute.cellquery_dims(box, r_cut)
+    npt.assert_array_equal(mn, brute.cellquery_min_pos(box, r_cut))
+    assert inv.value == np.float32(1) / np.float32(r_cut)
+    if int(n[0]) * int(n[1]) * int(n[2]) < 10**7:
+        g = oracle.cellquery_grid((box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D), np.zeros((1, 3)),
+                                  r_cut)
+        assert tuple(n) == g["n"]
+        npt.assert_array_equal(mn, g["min_pos"])
+    # a non-positive width is refused like CellQuery::buildGrid
+    assert _lib.lib().fa_cellquery_grid(ctypes.byref(cbox), 0.0, n.ctypes.data, mn.ctypes.data,
+                                        ctypes.byref(inv)) != 0

@@ -670,3 +670,166 @@ def test_pmftxy_brute_force_and_ideal_gas(oracle):
     cc = (e[:-1] + e[1:]) / 2
     rr = np.sqrt(cc[:, None] ** 2 + cc[None, :] ** 2)
     assert abs(pcf[rr < 0.95 * np.sqrt(2) * lim].mean() - 1.0) < 0.05
+
+
+# --------------------------------------------------------------------- CellQuery
+@pytest.mark.parametrize(
+    ("box", "r_cut", "cells"),
+    [
+        # test_locality_neighbor_query.py:879-899 of the reference (setupGrid by hand)
+        (Box.cube(10), 2.5, [7, 7, 7]),
+        (Box.cube(10), 2.4999, [7, 7, 7]),
+        (Box(1000, 100, 10), 1.0, [1003, 103, 13]),
+        (Box(2.0, 2.0, 2.0, 1.5, 1.0, 1.0), 1.0, [10, 7, 5]),
+        (Box(1.0, 2.0, 3.0, 1.5, 0.5, 1.0), 0.4, [16, 15, 10]),
+    ],
+)
+def test_cellquery_known_cell_counts(oracle, box, r_cut, cells):
+    # (the reference's Box(10000, 1000, 100) case, 1e9 cells, is checked in
+    # dims only, by the library's host grid function)
+    g = oracle.cellquery_grid(_box_tuple(box), np.zeros((1, 3)), r_cut)
+    assert list(g["n"]) == cells
+    assert list(brute.cellquery_dims(box, r_cut)) == cells
+    npt.assert_array_equal(g["min_pos"], brute.cellquery_min_pos(box, r_cut))
+    # the grid spans the box's corners padded by r_cut (test_grid_large_enough)
+    corners = box.make_absolute(np.array(list(np.ndindex(2, 2, 2)), np.float32))
+    for sgn in (-1, 1):
+        c = (corners + sgn * np.float32(r_cut) - g["min_pos"]) * g["inverse_width"]
+        assert np.all(c >= 0) and np.all(c < np.array(g["n"]))
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000])
+def test_cellquery_ghost_occupancy(oracle, n):
+    # test_locality_neighbor_query.py:919-977 of the reference: n points at the
+    # origin and one at the corner (5, 5, 5), whose 7 ghosts land in the
+    # corner cells of the 5 x 5 x 5 grid
+    box = Box.cube(10)
+    pts = np.array([*np.zeros((n, 3)), [5, 5, 5]], np.float32)
+    g = oracle.cellquery_grid(_box_tuple(box), pts, 4.999)
+    assert g["n"] == (5, 5, 5)
+    assert g["counts_real"][2, 2, 2] == n
+    assert g["counts_real"].sum() == n + 1
+    assert g["n_total"] == g["counts"].sum() == n + 8
+    for c in [(1, 1, 1), (1, 1, 3), (1, 3, 1), (1, 3, 3), (3, 1, 1), (3, 1, 3), (3, 3, 1), (3, 3, 3)]:
+        assert g["counts"][c] == 1
+    # r_max = 5: the origin points sit exactly on the margin and get no ghosts
+    g = oracle.cellquery_grid(_box_tuple(box), pts, 5.0)
+    assert g["n_total"] == n + 8
+    # the entries are the brute-force set of points and ghosts
+    j, _ = brute.cellquery_entries(box, pts, 4.999)
+    assert len(j) == n + 8
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("shape", ["cubic", "triclinic", "2d"])
+def test_cellquery_bruteforce(oracle, seed, shape):
+    rng = np.random.default_rng(seed)
+    if shape == "cubic":
+        box = Box.cube(6.0)
+    elif shape == "triclinic":
+        box = Box(6.0, 6.5, 7.0, 0.2, -0.25, 0.15)
+    else:
+        box = Box.square(6.0)
+    f = rng.random((90, 3)).astype(np.float32)
+    # points on the faces and a little outside the box too
+    f[:6] = np.array([[0, 0.5, 0.5], [1, 0.5, 0.5], [0.5, 0, 0.5], [0.02, 0.99, 0.01], [-0.01, 0.5, 0.5],
+                      [1.005, 1.01, 0.5]], np.float32)
+    pts = box.make_absolute(f)
+    if box.is2D:
+        pts[:, 2] = 0
+    q = box.make_absolute(rng.random((40, 3)).astype(np.float32))
+    if box.is2D:
+        q[:, 2] = 0
+    for r_max, r_min, qp, ex in [(1.999, 0.0, pts, True), (1.2, 0.5, pts, False), (2.5, 0.0, q, False)]:
+        got = oracle.query(_box_tuple(box), pts, qp, dict(r_max=r_max, r_min=r_min, exclude_ii=ex),
+                           nq_type="cellquery")
+        _same_nlist(got, brute.ball_cellquery(box, pts, qp, r_max, r_min=r_min, exclude_ii=ex))
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("is2D", [True, False])
+def test_cellquery_equals_aabb(oracle, seed, is2D):
+    # TestAABBQueryVsCellQuery (test_locality_neighbor_query.py:1082-1205 of the reference)
+    rng = np.random.RandomState(seed)
+    L = rng.uniform(10, 50) if is2D else rng.uniform(10, 30)
+    N = rng.randint(10, 200) if is2D else rng.randint(10, 100)
+    box, pts = data.make_random_system(L, N, is2D=is2D, seed=seed)
+    r_max = L * rng.uniform(0.1, 0.4)
+    _, qpts = data.make_random_system(L, 60, is2D=is2D, seed=seed + 10000)
+    for qp, ex in ((pts, True), (qpts, False)):
+        a = oracle.query(_box_tuple(box), pts, qp, dict(r_max=r_max, exclude_ii=ex))
+        c = oracle.query(_box_tuple(box), pts, qp, dict(r_max=r_max, exclude_ii=ex), nq_type="cellquery")
+        npt.assert_array_equal(c["ij"], a["ij"])
+        npt.assert_allclose(c["distances"], a["distances"], rtol=1e-5)
+        npt.assert_allclose(c["vectors"], a["vectors"], rtol=1e-5, atol=1e-5)
+        # and against float64 minimum-image distances
+        d = qp[:, None, :].astype(np.float64) - pts[None, :, :].astype(np.float64)
+        d64 = np.linalg.norm(box.wrap(d.reshape(-1, 3).astype(np.float32)).astype(np.float64), axis=1)
+        d64 = d64.reshape(len(qp), len(pts))
+        if ex:
+            np.fill_diagonal(d64, np.inf)
+        far = np.abs(d64 - r_max) > 1e-4
+        expect = {(int(i), int(j)) for i, j in zip(*np.nonzero((d64 < r_max) & far))}
+        got = {(int(i), int(j)) for i, j in c["ij"]}
+        assert expect == {b for b in got if far[b]}
+
+
+def test_cellquery_known_answers(oracle):
+    box = Box.cube(10)
+    # test_r_min (test_locality_neighbor_query.py:728-749)
+    pts = np.array([[0, 0, 0], [1, 0, 0], [3, 0, 0], [2, 0, 0]], np.float32)
+    nl = oracle.query(_box_tuple(box), pts, pts, dict(mode="ball", r_max=2.9, r_min=1.1, exclude_ii=True),
+                      nq_type="cellquery")
+    assert [set(nl["ij"][nl["ij"][:, 0] == i, 1]) for i in range(4)] == [{3}, {2}, {1}, {0}]
+    # test_duplicate_cell_shells (one bond) and test_duplicate_cell_shells2 (three)
+    nl = oracle.query(_box_tuple(Box.square(5)), [[0.9, 0, 0]], [[-1.5, 0, 0]], dict(r_max=2.45),
+                      nq_type="cellquery")
+    assert len(nl["ij"]) == 1
+    p2 = np.array([[1.5132198, 6.67087, 3.1856632], [1.3913784, -2.3667011, 4.5227165],
+                   [-3.6133137, 9.043476, 0.8957424]], np.float32)
+    nl = oracle.query(_box_tuple(Box.cube(21)), p2, p2[:1], dict(r_max=10), nq_type="cellquery")
+    assert len(nl["ij"]) == 3
+    # test_query_point_outside_grid_bounds_ball: no bonds
+    nl = oracle.query(_box_tuple(Box.cube(20)), [[0, 0, 0]], [[-15, 0, 0]], dict(mode="ball", r_max=5.0),
+                      nq_type="cellquery")
+    assert len(nl["ij"]) == 0
+    # the bond vector is the ghost's position minus q (CellIterator.h:177):
+    # (p + (-L)) - q in float32, which differs in the last bit here from the
+    # AABB order p - (q + L)
+    f = np.float32
+    px, qx, L = f(3.7559254), f(-3.7140155), f(7.77)
+    p = np.array([[px, 0, 0]], np.float32)
+    q = np.array([[qx, 0, 0]], np.float32)
+    cube = (L, L, L, 0, 0, 0, False)
+    nl = oracle.query(cube, p, q, dict(r_max=1.0), nq_type="cellquery")
+    assert nl["vectors"][0, 0] == (px + (-L)) - qx
+    aabb = oracle.query(cube, p, q, dict(r_max=1.0))
+    assert aabb["vectors"][0, 0] == px - (qx + L)
+    assert nl["vectors"][0, 0] != aabb["vectors"][0, 0]
+
+
+def test_cellquery_errors(oracle):
+    pts = np.array([[0, 0, 0], [1, 1, 0]], np.float32)
+    box = (3, 4, 5, 1, 0.5, 0.1, False)
+    # test_invalid_r_max_r_min_bounds
+    for qa in (dict(r_max=0), dict(r_max=-0.5), dict(r_max=0.1, r_min=0.1), dict(r_max=0.1, r_min=0.2)):
+        with pytest.raises(ValueError):
+            oracle.query(box, pts, pts, qa, nq_type="cellquery")
+    # nearest queries are refused, also when mode is inferred
+    for qa in (dict(mode="nearest", num_neighbors=3), dict(num_neighbors=1)):
+        with pytest.raises(RuntimeError, match="CellQuery only supports"):
+            oracle.query((10, 10, 10, 0, 0, 0, False), pts, pts, qa, nq_type="cellquery")
+    # test_too_large_r_max_raises
+    with pytest.raises(RuntimeError, match="CellQuery r_max is too large"):
+        oracle.query((5, 5, 0, 0, 0, 0, True), pts, pts, dict(r_max=5), nq_type="cellquery")
+    # an empty query still validates (CellQuery::query)
+    with pytest.raises(RuntimeError, match="too large"):
+        oracle.query((5, 5, 0, 0, 0, 0, True), pts, np.zeros((0, 3), np.float32), dict(r_max=5),
+                     nq_type="cellquery")
+
+
+def test_cellquery_rdf_counts_match_bonds(oracle):
+    box, pts = data.make_random_system(12, 3000, seed=4)
+    counts = oracle.rdf_accumulate(_box_tuple(box), pts, pts, 60, 3.0, nq_type="cellquery", nthreads=4)
+    nl = oracle.query(_box_tuple(box), pts, pts, dict(r_max=3.0, exclude_ii=True), nq_type="cellquery", nthreads=4)
+    npt.assert_array_equal(counts, oracle.rdf_bin(nl["distances"], 60, 0.0, 3.0))
