@@ -107,17 +107,6 @@ __device__ inline void ball_count_body(const DevBox& b, const DevGrid& g, const 
     counts[i] = c;
 }
 
-template<int FORMULA, bool SELF>
-__global__ __launch_bounds__(kBlock) void ball_count_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                            uint32_t nq, float r_max_sq, float r_min_sq,
-                                                            int exclude_ii, int R, uint32_t* __restrict__ counts,
-                                                            fa::CQGrid cq)
-{
-    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (p < nq)
-        ball_count_body<FORMULA, SELF>(b, g, qpts, p, r_max_sq, r_min_sq, exclude_ii, R, counts, cq);
-}
-
 template<typename K> __device__ inline void insertion_sort(K* a, uint32_t n)
 {
     for (uint32_t x = 1; x < n; ++x)
@@ -264,20 +253,170 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
                                            w_out, v_out);
 }
 
+// ------------------------------------------------------------------ ball query: one walk
+// The ball query walks the cell stencil once.  The walk runs in the order that
+// keeps a wave's stencils together (cell order for self queries) and keeps
+// each query point's first C keys (j << 5 | image code, unsorted) in LDS; the
+// block then packs its points' keys into its own region of a scratch array
+// with coalesced stores, and records each point's bond count and where its
+// keys start.  The emission pass runs in index order: a block's 256 points own
+// one contiguous range of the NeighborList, so its 28-byte bonds go out with
+// full-line stores (written in cell order they would land as ~13-bond
+// segments scattered over partly written lines).  A point with more than C
+// bonds walks its stencil again inside the emission pass.
+template<int FORMULA, bool SELF>
+__global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, float r_max_sq, float r_min_sq,
+                                                           int exclude_ii, int R, fa::CQGrid cq, uint32_t C,
+                                                           uint32_t* __restrict__ counts, uint32_t* __restrict__ pos,
+                                                           uint32_t* __restrict__ rows)
+{
+    // s_rows[m * kBlock + tid]: key m of thread tid (consecutive lanes, consecutive banks)
+    uint32_t* s_rows = reinterpret_cast<uint32_t*>(fa_dyn_lds);
+    __shared__ uint32_t s_off[kBlock + 1];
+    __shared__ uint32_t s_tmp[kBlock];
+    uint32_t const tile = fa::xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const p0 = tile * kBlock;
+    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
+    uint32_t const tid = threadIdx.x;
+    uint32_t c = 0, i = 0;
+    if (tid < nloc)
+    {
+        float qx, qy, qz;
+        i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
+        fa::visit_bonds<FORMULA>(b, g, cq, qx, qy, qz, R,
+                                 [&](uint32_t j, float, float, float, float r_sq, int code) {
+                                     if (r_sq < r_max_sq && r_sq >= r_min_sq && !(exclude_ii && j == i))
+                                     {
+                                         if (c < C)
+                                             s_rows[c * kBlock + tid] = (j << 5) | uint32_t(code);
+                                         ++c;
+                                     }
+                                 });
+    }
+    uint32_t const kept = min(c, C);
+    uint32_t excl;
+    uint32_t const total = block_exclusive_scan(kept, s_tmp, excl);
+    uint32_t const region = tile * kBlock * C; // this block's slots in rows
+    if (tid < nloc)
+    {
+        s_off[tid] = excl;
+        counts[i] = c;
+        pos[i] = region + excl;
+    }
+    if (tid == 0)
+        s_off[nloc] = total;
+    __syncthreads();
+    for (uint32_t t = tid; t < total; t += kBlock)
+    {
+        uint32_t lo = 0, hi = nloc;
+        while (hi - lo > 1)
+        {
+            uint32_t const mid = (lo + hi) >> 1;
+            if (s_off[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        rows[region + t] = s_rows[(t - s_off[lo]) * kBlock + lo];
+    }
+}
+
+// Index-order emission: query points [p0, p0 + 256) of the caller's order.
+// Keys come from the point's row (or, past C bonds, from a second walk), are
+// sorted per point (by j and image, or by distance), and the block writes its
+// contiguous range of bonds.
 template<int FORMULA, bool BYDIST, bool SELF>
-__global__ __launch_bounds__(kBlock) void ball_fill_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                           uint32_t nq, const float* __restrict__ pts, float r_max_sq,
-                                                           float r_min_sq, int exclude_ii, int R,
+__global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, const float* __restrict__ pts,
+                                                           float r_max_sq, float r_min_sq, int exclude_ii, int R,
+                                                           fa::CQGrid cq, uint32_t C, const uint32_t* __restrict__ pos,
+                                                           const uint32_t* __restrict__ rows,
                                                            const uint32_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                                            float* __restrict__ w_out, float* __restrict__ v_out,
-                                                           void* __restrict__ gkeys_raw, uint32_t key_cap,
-                                                           fa::CQGrid cq)
+                                                           void* __restrict__ gkeys_raw, uint32_t key_cap)
 {
+    using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
+    K* s_keys = reinterpret_cast<K*>(fa_dyn_lds);
+    __shared__ uint32_t s_off[kBlock + 1];
+    __shared__ uint32_t s_i[kBlock];
+    __shared__ uint32_t s_tmp[kBlock];
+    __shared__ float s_q[3 * kBlock];
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
-    ball_fill_body<FORMULA, BYDIST, SELF>(b, g, qpts, p0, min(uint32_t(kBlock), nq - p0), pts, r_max_sq, r_min_sq,
-                                          exclude_ii, R, offsets, nb_out, d_out, w_out, v_out, gkeys_raw, key_cap,
-                                          cq);
+    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
+    uint32_t const tid = threadIdx.x;
+    uint32_t const i = p0 + tid;
+    float qx = 0.0f, qy = 0.0f, qz = 0.0f;
+    uint32_t cnt = 0;
+    if (tid < nloc)
+    {
+        qx = qpts[3 * i];
+        qy = qpts[3 * i + 1];
+        qz = qpts[3 * i + 2];
+        cnt = offsets[i + 1] - offsets[i];
+        s_i[tid] = i;
+        s_q[3 * tid] = qx;
+        s_q[3 * tid + 1] = qy;
+        s_q[3 * tid + 2] = qz;
+    }
+    uint32_t excl;
+    uint32_t const total = block_exclusive_scan(cnt, s_tmp, excl);
+    if (tid < nloc)
+        s_off[tid] = excl;
+    if (tid == 0)
+        s_off[nloc] = total;
+    __syncthreads();
+    bool const in_lds = total <= key_cap;
+    K* keys = in_lds ? s_keys : reinterpret_cast<K*>(gkeys_raw);
+    if (tid < nloc)
+    {
+        uint32_t const beg = in_lds ? excl : offsets[i];
+        auto key_of = [&](uint32_t jc) -> K {
+            if (BYDIST)
+            {
+                uint32_t const j = jc >> 5;
+                float rx, ry, rz;
+                fa::bond_vector<FORMULA>(b, qx, qy, qz, pts[3 * j], pts[3 * j + 1], pts[3 * j + 2], int(jc & 31u),
+                                         rx, ry, rz);
+                float const r_sq = (rx * rx + ry * ry) + rz * rz;
+                return K((uint64_t(float_order_bits(sqrtf(r_sq))) << 32) | jc);
+            }
+            return K(jc);
+        };
+        if (cnt <= C)
+        {
+            const uint32_t* row = rows + pos[i];
+            for (uint32_t m = 0; m < cnt; ++m)
+                keys[beg + m] = key_of(row[m]);
+        }
+        else
+        {
+            uint32_t k = beg;
+            uint32_t const end = beg + cnt;
+            fa::visit_bonds<FORMULA>(b, g, cq, qx, qy, qz, R,
+                                     [&](uint32_t j, float, float, float, float r_sq, int code) {
+                                         if (r_sq < r_max_sq && r_sq >= r_min_sq && k < end && !(exclude_ii && j == i))
+                                         {
+                                             uint32_t const jc = (j << 5) | uint32_t(code);
+                                             if (BYDIST)
+                                                 keys[k] = K((uint64_t(float_order_bits(sqrtf(r_sq))) << 32) | jc);
+                                             else
+                                                 keys[k] = K(jc);
+                                             ++k;
+                                         }
+                                     });
+        }
+        insertion_sort(keys + beg, cnt);
+        if (!in_lds)
+            for (uint32_t m = 0; m < cnt; ++m)
+                write_bond_at<FORMULA, BYDIST>(b, qx, qy, qz, pts, i, uint64_t(keys[beg + m]),
+                                               uint64_t(offsets[i]) + m, nb_out, d_out, w_out, v_out);
+    }
+    __syncthreads();
+    if (in_lds)
+        store_block_bonds<FORMULA, BYDIST>(b, s_q, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+                                           w_out, v_out);
 }
 
 // Segments with the reference's convention (0 for query points without bonds).
@@ -983,6 +1122,20 @@ int launch_knn_lc(const fa::DevBox& db, const DevGrid& g, const float* qpts, uin
     return FA_OK;
 }
 
+// Keys kept per point by the one-walk ball query: the bonds a point of a
+// uniform system at this density has, with slack, at most 32 (a 32 KB LDS
+// stage per block); points beyond it are walked again by the emission pass.
+constexpr uint32_t kWalkMaxSlots = kKeyBytes / (kBlock * sizeof(uint32_t));
+uint32_t ball_row_slots(const fa_nq* nq, float r_max)
+{
+    double const vol = std::max(double(fa::box_volume(nq->box)), 1e-30);
+    double const dens = double(nq->n) / vol;
+    double const r = double(r_max);
+    double const expect = nq->box.is2D ? dens * M_PI * r * r : dens * (4.0 / 3.0) * M_PI * r * r * r;
+    double const c = std::ceil((expect * 1.15 + 4.0) / 4.0) * 4.0;
+    return uint32_t(std::min(std::max(c, 8.0), double(kWalkMaxSlots)));
+}
+
 int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args& a, int by_distance, fa_nlist* out)
 {
     hipStream_t s = fa::stream();
@@ -997,23 +1150,39 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
         return rc;
     float const r_max_sq = a.r_max * a.r_max, r_min_sq = a.r_min * a.r_min;
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
-    // query points == the grid's points: process them in cell order
+    // query points == the grid's points: the walk takes them in cell order
     bool const self = (qpts == nq->d_points) && (n_query == nq->n);
     fa::CQGrid const cq = nq->kind == FA_NQ_CELLQUERY ? fa::cq_grid_for(nq->box, a.r_max) : fa::CQGrid{};
-    auto count = [&](auto kern) {
-        FA_PROF_BEGIN("ball_count_kernel");
-        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, db, g, qpts, n_query, r_max_sq, r_min_sq,
-                           a.exclude_ii, R, out->counts, cq);
+    uint32_t const C = ball_row_slots(nq, a.r_max);
+    uint32_t* rows = nullptr;
+    uint32_t* pos = nullptr;
+    if (n_query)
+    {
+        // each block owns kBlock * C slots; positions are 32-bit
+        size_t const slots = size_t(blocks) * kBlock * C;
+        if (slots >= (size_t(1) << 32))
+            return fa::fail(FA_ERR_RUNTIME, "ball query: too many query points for the key staging");
+        void* tmp = nullptr;
+        if (int rc = fa::scratch(21, sizeof(uint32_t) * (slots + size_t(n_query)) + 256, &tmp))
+            return rc;
+        rows = static_cast<uint32_t*>(tmp);
+        pos = rows + slots;
+    }
+    size_t const walk_lds = sizeof(uint32_t) * size_t(kBlock) * C;
+    auto walk = [&](auto kern) {
+        FA_PROF_BEGIN("ball_walk_kernel");
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), walk_lds, s, db, g, qpts, n_query, r_max_sq, r_min_sq,
+                           a.exclude_ii, R, cq, C, out->counts, pos, rows);
         FA_PROF_END();
     };
     if (n_query == 0)
         ;
     else if (nq->kind == FA_NQ_AABB)
-        self ? count(ball_count_kernel<AABB_FORMULA, true>) : count(ball_count_kernel<AABB_FORMULA, false>);
+        self ? walk(ball_walk_kernel<AABB_FORMULA, true>) : walk(ball_walk_kernel<AABB_FORMULA, false>);
     else if (nq->kind == FA_NQ_CELLQUERY)
-        self ? count(ball_count_kernel<CELLQUERY_FORMULA, true>) : count(ball_count_kernel<CELLQUERY_FORMULA, false>);
+        self ? walk(ball_walk_kernel<CELLQUERY_FORMULA, true>) : walk(ball_walk_kernel<CELLQUERY_FORMULA, false>);
     else
-        self ? count(ball_count_kernel<LINKCELL_FORMULA, true>) : count(ball_count_kernel<LINKCELL_FORMULA, false>);
+        self ? walk(ball_walk_kernel<LINKCELL_FORMULA, true>) : walk(ball_walk_kernel<LINKCELL_FORMULA, false>);
     FA_HIP(hipGetLastError());
     uint64_t total = 0;
     if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
@@ -1028,44 +1197,33 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
         return rc;
     size_t const kb = fill_key_bytes(total, n_query, by_distance ? 8 : 4);
     uint32_t const key_cap = uint32_t(kb / (by_distance ? 8 : 4));
-    auto launch = [&](auto kern) {
-        FA_PROF_BEGIN("ball_fill_kernel");
+    auto emit = [&](auto kern) {
+        FA_PROF_BEGIN("ball_emit_kernel");
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), kb, s, db, g, qpts, n_query, nq->d_points, r_max_sq,
-                           r_min_sq, a.exclude_ii, R, out->offsets, out->neighbors, out->distances, out->weights,
-                           out->vectors, gkeys, key_cap, cq);
+                           r_min_sq, a.exclude_ii, R, cq, C, pos, rows, out->offsets, out->neighbors, out->distances,
+                           out->weights, out->vectors, gkeys, key_cap);
         FA_PROF_END();
     };
+#define FA_BALL_EMIT(FORM)                                                                                    \
+    do                                                                                                        \
+    {                                                                                                         \
+        if (self)                                                                                             \
+            by_distance ? emit(ball_emit_kernel<FORM, true, true>) : emit(ball_emit_kernel<FORM, false, true>); \
+        else                                                                                                  \
+            by_distance ? emit(ball_emit_kernel<FORM, true, false>)                                           \
+                        : emit(ball_emit_kernel<FORM, false, false>);                                         \
+    } while (0)
     if (n_query)
     {
-        if (nq->kind == FA_NQ_CELLQUERY)
-        {
-            if (self)
-                by_distance ? launch(ball_fill_kernel<CELLQUERY_FORMULA, true, true>)
-                            : launch(ball_fill_kernel<CELLQUERY_FORMULA, false, true>);
-            else
-                by_distance ? launch(ball_fill_kernel<CELLQUERY_FORMULA, true, false>)
-                            : launch(ball_fill_kernel<CELLQUERY_FORMULA, false, false>);
-        }
-        else if (nq->kind == FA_NQ_AABB)
-        {
-            if (self)
-                by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true, true>)
-                            : launch(ball_fill_kernel<AABB_FORMULA, false, true>);
-            else
-                by_distance ? launch(ball_fill_kernel<AABB_FORMULA, true, false>)
-                            : launch(ball_fill_kernel<AABB_FORMULA, false, false>);
-        }
+        if (nq->kind == FA_NQ_AABB)
+            FA_BALL_EMIT(AABB_FORMULA);
+        else if (nq->kind == FA_NQ_CELLQUERY)
+            FA_BALL_EMIT(CELLQUERY_FORMULA);
         else
-        {
-            if (self)
-                by_distance ? launch(ball_fill_kernel<LINKCELL_FORMULA, true, true>)
-                            : launch(ball_fill_kernel<LINKCELL_FORMULA, false, true>);
-            else
-                by_distance ? launch(ball_fill_kernel<LINKCELL_FORMULA, true, false>)
-                            : launch(ball_fill_kernel<LINKCELL_FORMULA, false, false>);
-        }
+            FA_BALL_EMIT(LINKCELL_FORMULA);
         FA_HIP(hipGetLastError());
     }
+#undef FA_BALL_EMIT
     return fa::nlist_finish_counts(out);
 }
 
