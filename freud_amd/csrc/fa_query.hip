@@ -322,11 +322,14 @@ __global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, 
     }
 }
 
-// Index-order emission: query points [p0, p0 + 256) of the caller's order.
+// Emission: with INDEX, query points [p0, p0 + 256) of the caller's order,
+// whose bonds are one contiguous range; otherwise the walk's processing order
+// (the block's keys are its own packed region, and its neighbours' positions
+// stay in cache, which matters more for points stored in random order).
 // Keys come from the point's row (or, past C bonds, from a second walk), are
 // sorted per point (by j and image, or by distance), and the block writes its
-// contiguous range of bonds.
-template<int FORMULA, bool BYDIST, bool SELF>
+// bonds.
+template<int FORMULA, bool BYDIST, bool SELF, bool INDEX>
 __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
                                                            uint32_t nq, const float* __restrict__ pts,
                                                            float r_max_sq, float r_min_sq, int exclude_ii, int R,
@@ -346,14 +349,19 @@ __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, 
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
     uint32_t const tid = threadIdx.x;
-    uint32_t const i = p0 + tid;
+    uint32_t i = p0 + tid;
     float qx = 0.0f, qy = 0.0f, qz = 0.0f;
     uint32_t cnt = 0;
     if (tid < nloc)
     {
-        qx = qpts[3 * i];
-        qy = qpts[3 * i + 1];
-        qz = qpts[3 * i + 2];
+        if (INDEX)
+        {
+            qx = qpts[3 * i];
+            qy = qpts[3 * i + 1];
+            qz = qpts[3 * i + 2];
+        }
+        else
+            i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
         cnt = offsets[i + 1] - offsets[i];
         s_i[tid] = i;
         s_q[3 * tid] = qx;
@@ -501,13 +509,16 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
                                                           float rmin_thr, int exclude_ii, float wmin,
                                                           float half_min_plane, int rmax_ring,
                                                           uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
-                                                          uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag)
+                                                          uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag,
+                                                          uint32_t* __restrict__ pos)
 {
     uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= nq)
         return;
     float qx, qy, qz;
     uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
+    if (SELF)
+        pos[i] = p;
     TopK<KMAX> best;
     best.init();
     uint32_t found = 0;
@@ -566,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
     uint32_t const n = min(found, k);
     counts[i] = n;
     for (uint32_t m = 0; m < n; ++m)
-        sel[uint64_t(m) * nq + p] = best.at(m);
+        sel[uint64_t(p) * k + m] = best.at(m);
 }
 
 // LinkCell formula kNN (LinkCell.cc:575-679): every point in cells within
@@ -600,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void knn_linkcell_kernel(DevBox b, DevGrid 
     uint32_t const n = min(found, k);
     counts[i] = n;
     for (uint32_t m = 0; m < n; ++m)
-        sel[uint64_t(m) * nq + i] = best.at(m);
+        sel[uint64_t(i) * k + m] = best.at(m);
 }
 
 // Exhaustive kNN for flagged queries (one block per query): every point and
@@ -614,12 +625,15 @@ __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const 
                                                                 float rlim_sq, float r_min, int exclude_ii,
                                                                 uint64_t* __restrict__ work,
                                                                 uint32_t* __restrict__ counts,
-                                                                uint64_t* __restrict__ sel)
+                                                                uint64_t* __restrict__ sel,
+                                                                uint32_t* __restrict__ pos)
 {
     // flagged holds processing positions: with cpos (queries in cell order)
     // the point index is the cell-ordered entry's, else the position itself
     uint32_t const p = flagged[blockIdx.x];
     uint32_t const i = cpos ? __float_as_uint(cpos[p].w) : p;
+    if (cpos && threadIdx.x == 0)
+        pos[i] = p;
     uint64_t* w = work + uint64_t(blockIdx.x) * np;
     float const qx = qpts[3 * i], qy = qpts[3 * i + 1];
     float const qz = b.is2D ? 0.0f : qpts[3 * i + 2];
@@ -674,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const 
             break;
         if (threadIdx.x == 0)
         {
-            sel[uint64_t(m) * nq + p] = mn;
+            sel[uint64_t(p) * k + m] = mn;
             s_cnt = m + 1;
         }
         uint32_t const jm = uint32_t(mn) >> 5;
@@ -686,15 +700,18 @@ __global__ __launch_bounds__(kBlock) void knn_exhaustive_kernel(DevBox b, const 
         counts[i] = s_cnt;
 }
 
-// Writes a kNN result: per query, its selected keys sorted into the
-// NeighborList order ((i, j) or (i, d, j)).
-// Writes a kNN result: per query, its selected keys sorted into the
-// NeighborList order ((i, j) or (i, d, j)), staged per block in LDS and stored
-// cooperatively (query points in the same processing order as the search).
-template<int FORMULA, bool BYDIST, bool SELF>
-__global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                           uint32_t nq, const float* __restrict__ pts, uint32_t k,
+// Writes a kNN result.  With INDEX a block takes 256 consecutive query points,
+// whose bonds are one contiguous range of the NeighborList (full-line
+// stores); otherwise the search's processing order, which keeps neighbour
+// positions in cache for points stored in random order.  Each point's
+// selected keys (point-major rows by processing position: pos[i] for
+// cell-ordered self queries) are sorted into the NeighborList order ((i, j) or
+// (i, d, j)), staged in LDS and stored cooperatively.
+template<int FORMULA, bool BYDIST, bool SELF, bool INDEX>
+__global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts, uint32_t nq,
+                                                           const float* __restrict__ pts, uint32_t k,
                                                            const uint64_t* __restrict__ sel,
+                                                           const uint32_t* __restrict__ pos,
                                                            const uint32_t* __restrict__ offsets,
                                                            uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
                                                            float* __restrict__ w_out, float* __restrict__ v_out,
@@ -710,11 +727,19 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     uint32_t const tid = threadIdx.x;
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
-    uint32_t i = 0, n = 0;
+    uint32_t i = p0 + tid;
+    uint32_t n = 0;
     if (tid < nloc)
     {
         float qx, qy, qz;
-        i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
+        if (INDEX)
+        {
+            qx = qpts[3 * i];
+            qy = qpts[3 * i + 1];
+            qz = qpts[3 * i + 2];
+        }
+        else
+            i = query_at<SELF>(g, qpts, p0 + tid, qx, qy, qz);
         n = offsets[i + 1] - offsets[i];
         s_i[tid] = i;
         s_q[3 * tid] = qx;
@@ -732,12 +757,13 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     if (tid < nloc)
     {
         K* keys = in_lds ? s_keys + excl : reinterpret_cast<K*>(tmp + uint64_t(i) * k);
+        const uint64_t* row = sel + uint64_t(INDEX && SELF ? pos[i] : p0 + tid) * k;
         for (uint32_t m = 0; m < n; ++m)
         {
-            uint64_t const s = sel[uint64_t(m) * nq + p0 + tid];
-            uint32_t const jc = uint32_t(s);
+            uint64_t const sk = row[m];
+            uint32_t const jc = uint32_t(sk);
             if (BYDIST)
-                keys[m] = K((uint64_t(float_order_bits(knn_key_distance(s))) << 32) | jc);
+                keys[m] = K((uint64_t(float_order_bits(knn_key_distance(sk))) << 32) | jc);
             else
                 keys[m] = K(jc);
         }
@@ -1093,7 +1119,8 @@ float sqrt_threshold(float r)
 template<int KMAX>
 int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, uint32_t nq, uint32_t k, float rlim,
                     float r_min, int exclude_ii, float wmin, float hmp, int rmax_ring, uint32_t* counts,
-                    uint64_t* sel, uint32_t* flagged, uint32_t* nflag, const float* self_points, uint32_t self_n)
+                    uint64_t* sel, uint32_t* flagged, uint32_t* nflag, const float* self_points, uint32_t self_n,
+                    uint32_t* pos)
 {
     float const rlim_sq = rlim * rlim;
     float const rmin_thr = sqrt_threshold(r_min);
@@ -1102,11 +1129,11 @@ int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, u
     if (self)
         hipLaunchKernelGGL((knn_aabb_kernel<KMAX, true>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring,
-                           counts, sel, flagged, nflag);
+                           counts, sel, flagged, nflag, pos);
     else
         hipLaunchKernelGGL((knn_aabb_kernel<KMAX, false>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
                            fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring,
-                           counts, sel, flagged, nflag);
+                           counts, sel, flagged, nflag, pos);
     FA_PROF_END();
     FA_HIP(hipGetLastError());
     return FA_OK;
@@ -1204,14 +1231,27 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
                            out->weights, out->vectors, gkeys, key_cap);
         FA_PROF_END();
     };
-#define FA_BALL_EMIT(FORM)                                                                                    \
-    do                                                                                                        \
-    {                                                                                                         \
-        if (self)                                                                                             \
-            by_distance ? emit(ball_emit_kernel<FORM, true, true>) : emit(ball_emit_kernel<FORM, false, true>); \
-        else                                                                                                  \
-            by_distance ? emit(ball_emit_kernel<FORM, true, false>)                                           \
-                        : emit(ball_emit_kernel<FORM, false, false>);                                         \
+    // self queries of points stored in random order emit in processing order
+    bool index_order = true;
+    if (self && n_query)
+    {
+        const uint32_t* sorder = nullptr;
+        if (int rc = fa::spatial_order(nq->box, nq->d_points, nq->n, &sorder))
+            return rc;
+        index_order = sorder == nullptr;
+    }
+#define FA_BALL_EMIT(FORM)                                                                                       \
+    do                                                                                                           \
+    {                                                                                                            \
+        if (!self)                                                                                               \
+            by_distance ? emit(ball_emit_kernel<FORM, true, false, true>)                                        \
+                        : emit(ball_emit_kernel<FORM, false, false, true>);                                      \
+        else if (index_order)                                                                                    \
+            by_distance ? emit(ball_emit_kernel<FORM, true, true, true>)                                         \
+                        : emit(ball_emit_kernel<FORM, false, true, true>);                                       \
+        else                                                                                                     \
+            by_distance ? emit(ball_emit_kernel<FORM, true, true, false>)                                        \
+                        : emit(ball_emit_kernel<FORM, false, true, false>);                                      \
     } while (0)
     if (n_query)
     {
@@ -1281,15 +1321,17 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         out->num_bonds = 0;
         return fa::nlist_finish_counts(out);
     }
-    // scratch: selections (nq*k u64), write temp (nq*k u64), flagged list
+    // scratch: selections (nq*k u64, point-major by processing position),
+    // write temp (nq*k u64), flagged list, processing position per point
     void* tmp = nullptr;
     uint64_t const selb = uint64_t(n_query) * k * sizeof(uint64_t);
-    if (int rc = fa::scratch(4, 2 * selb + sizeof(uint32_t) * (size_t(n_query) + 64), &tmp))
+    if (int rc = fa::scratch(4, 2 * selb + sizeof(uint32_t) * (2 * size_t(n_query) + 64), &tmp))
         return rc;
     uint64_t* sel = static_cast<uint64_t*>(tmp);
     uint64_t* wtmp = sel + uint64_t(n_query) * k;
     uint32_t* nflag = reinterpret_cast<uint32_t*>(wtmp + uint64_t(n_query) * k);
     uint32_t* flagged = nflag + 16;
+    uint32_t* pos = flagged + n_query + 48;
     FA_HIP(hipMemsetAsync(nflag, 0, sizeof(uint32_t), s));
     float npd[3];
     fa::nearest_plane_distance(nq->box, npd);
@@ -1321,7 +1363,7 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
             FA_HIP(hipMemsetAsync(nflag, 0, sizeof(uint32_t), s));
 #define FA_KNN_LAUNCH(KM)                                                                                     \
     launch_knn_aabb<KM>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin, min_pd / 2.0f, \
-                        rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n)
+                        rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n, pos)
             if (k <= 8)
                 return FA_KNN_LAUNCH(8);
             if (k <= 12)
@@ -1461,7 +1503,7 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
             bool const cell_order = (qpts == nq->d_points) && (n_query == nq->n);
             FA_KLAUNCH(knn_exhaustive_kernel, dim3(nc), dim3(kBlock), 0, s, *dbp, nq->d_points, nq->n, qpts, n_query,
                        cell_order ? nq->cur->cpos : nullptr, flagged + c0, k, rlim * rlim, a.r_min, a.exclude_ii,
-                       static_cast<uint64_t*>(work), out->counts, sel);
+                       static_cast<uint64_t*>(work), out->counts, sel, pos);
             FA_HIP(hipGetLastError());
         }
         if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total))
@@ -1471,33 +1513,47 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         return rc;
     out->num_bonds = total;
     unsigned const blocks = (n_query + kBlock - 1) / kBlock;
+    // the AABB search takes self queries in cell order (rows at pos[i])
     bool const self = (qpts == nq->d_points) && (n_query == nq->n);
-    DevGrid const g = nq->cur->dev();
     // at most k keys per query point: the staging holds a whole block's
     size_t const ksz = by_distance ? 8 : 4;
     size_t const kb = std::min<size_t>(size_t(k) * kBlock * ksz, size_t(kKeyBytes));
     uint32_t const key_cap = uint32_t(kb / ksz);
+    DevGrid const g = nq->cur->dev();
     auto launch = [&](auto kern) {
         FA_PROF_BEGIN("knn_write_kernel");
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), kb, s, *dbp, g, qpts, n_query, nq->d_points, k, sel,
-                           out->offsets, out->neighbors, out->distances, out->weights, out->vectors, wtmp, key_cap);
+                           pos, out->offsets, out->neighbors, out->distances, out->weights, out->vectors, wtmp,
+                           key_cap);
         FA_PROF_END();
     };
     if (nq->kind == FA_NQ_AABB)
     {
+        // self queries of points stored in random order write in processing order
+        bool index_order = true;
         if (self)
-            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, true>)
-                        : launch(knn_write_kernel<AABB_FORMULA, false, true>);
+        {
+            const uint32_t* sorder = nullptr;
+            if (int rc = fa::spatial_order(nq->box, nq->d_points, nq->n, &sorder))
+                return rc;
+            index_order = sorder == nullptr;
+        }
+        if (!self)
+            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, false, true>)
+                        : launch(knn_write_kernel<AABB_FORMULA, false, false, true>);
+        else if (index_order)
+            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, true, true>)
+                        : launch(knn_write_kernel<AABB_FORMULA, false, true, true>);
         else
-            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, false>)
-                        : launch(knn_write_kernel<AABB_FORMULA, false, false>);
+            by_distance ? launch(knn_write_kernel<AABB_FORMULA, true, true, false>)
+                        : launch(knn_write_kernel<AABB_FORMULA, false, true, false>);
     }
     else
     {
         // the LinkCell search runs in index order and stores its selections by
-        // point index: the write pass takes the same order
-        by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, false>)
-                    : launch(knn_write_kernel<LINKCELL_FORMULA, false, false>);
+        // point index
+        by_distance ? launch(knn_write_kernel<LINKCELL_FORMULA, true, false, true>)
+                    : launch(knn_write_kernel<LINKCELL_FORMULA, false, false, true>);
     }
     FA_HIP(hipGetLastError());
     return fa::nlist_finish_counts(out);
