@@ -104,10 +104,12 @@ def algorithmic_bytes(name, n, b_ball, b_knn, k=12, l=6, bins=200):
     compulsory and are not counted here."""
     nm = 2 * l + 1
     table = {
-        "ball_count_kernel": 12 * n + 12 * n + 4 * n,
-        "ball_fill_kernel": 12 * n + 12 * n + 4 * (n + 1) + 28 * b_ball,
-        "knn_aabb_kernel": 12 * n + 12 * n + 4 * n + 8 * k * n,
-        "knn_write_kernel": 8 * k * n + 4 * (n + 1) + 28 * b_knn,
+        # one walk: cell-ordered points in, counts + positions + packed keys out
+        "ball_walk_kernel": 16 * n + 4 * n + 4 * n + 4 * b_ball,
+        # emission: offsets, query and neighbour positions, positions, keys in; bonds out
+        "ball_emit_kernel": 4 * (n + 1) + 12 * n + 12 * n + 4 * n + 4 * b_ball + 28 * b_ball,
+        "knn_aabb_kernel": 12 * n + 12 * n + 4 * n + 4 * n + 8 * k * n,
+        "knn_write_kernel": 8 * k * n + 4 * n + 4 * (n + 1) + 12 * n + 28 * b_knn,
         "rdf_self_kernel": 16 * n + 4 * bins,
         "rdf_cluster_kernel": 16 * n + 4 * bins,
         "qlm_kernel": 24 * b_ball + 8 * nm * n + 4 * n,
