@@ -23,6 +23,10 @@ NQ_AABB = 0
 NQ_LINKCELL = 1
 NQ_CELLQUERY = 2
 
+PMFT_XYT = 0
+PMFT_R12 = 1
+PMFT_XYZ = 2
+
 MODE_NONE = 0
 MODE_BALL = 1
 MODE_NEAREST = 2
@@ -169,6 +173,17 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_void_p]),
     "fa_pmftxy_accumulate": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_float,
                                             ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_pmftxyt_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_void_p]),
+    "fa_pmftr12_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_pmftxyz_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                             ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_pmft_reduce": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_void_p]),
     "fa_pmftxy_reduce": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
                                         ctypes.c_uint32, ctypes.c_void_p]),

@@ -1,6 +1,6 @@
-// fa_density.hip -- LocalDensity (freud/density/LocalDensity.cc) and PMFTXY
-// (freud/pmft/PMFTXY.cc, freud/pmft/PMFT.h) over a NeighborList: the other
-// consumers of the query path beside the RDF.
+// fa_density.hip -- LocalDensity (freud/density/LocalDensity.cc) and the PMFT
+// family (freud/pmft/PMFTXY.cc, PMFTXYT.cc, PMFTR12.cc, PMFTXYZ.cc, PMFT.h)
+// over a NeighborList: the other consumers of the query path beside the RDF.
 //
 // LocalDensity is one pass over each query point's bond segment (the bonds of
 // a query point are contiguous in a NeighborList), summing the reference's
@@ -119,6 +119,206 @@ __global__ __launch_bounds__(kBlock) void pmftxy_kernel(const uint32_t* __restri
     }
 }
 
+// ---------------------------------------------------------------- PMFTXYT, PMFTR12, PMFTXYZ
+// Three-axis RegularAxis histogram, row-major (Histogram.h:327-351): a bond
+// counts only if every coordinate falls inside its axis.
+struct Axis3
+{
+    float min[3], max[3], inv[3];
+    uint32_t n[3];
+};
+
+__device__ inline uint32_t bin3(const Axis3& A, float v0, float v1, float v2)
+{
+    uint32_t const b0 = axis_bin(v0, A.min[0], A.max[0], A.inv[0], A.n[0]);
+    uint32_t const b1 = axis_bin(v1, A.min[1], A.max[1], A.inv[1], A.n[1]);
+    uint32_t const b2 = axis_bin(v2, A.min[2], A.max[2], A.inv[2], A.n[2]);
+    if (b0 == kOverflow || b1 == kOverflow || b2 == kOverflow)
+        return kOverflow;
+    return (b0 * A.n[1] + b1) * A.n[2] + b2;
+}
+
+// constants::TWO_PI (freud/box/Box.h:24), the float nearest 2 pi
+constexpr float kTwoPi = float(2.0 * M_PI);
+
+// util::modulusPositive (utils.h:29-32): fmod(fmod(a, b) + b, b), each fmod exact
+__device__ inline float modulus_positive(float a, float b) { return fmodf(fmodf(a, b) + b, b); }
+
+// The angle of (x, y): atan2 in double, rounded once to float (correctly
+// rounded but for rare double roundings).  The reference calls the host C
+// library's float atan2, whose last bit depends on the library; bins can only
+// differ for bonds whose angle lies within an ulp or two of a bin edge.
+__device__ inline float angle_of(float y, float x) { return float(atan2(double(y), double(x))); }
+
+// quat rotate (VectorMath.h:810-822): (s^2 - v.v) b + 2 s (v x b) + 2 (v.b) v,
+// in that float order, q = (s, x, y, z)
+__device__ inline void quat_rotate(float4 q, float bx, float by, float bz, float& ox, float& oy, float& oz)
+{
+    float const vv = (q.y * q.y + q.z * q.z) + q.w * q.w;
+    float const c1 = q.x * q.x - vv;
+    float const vb = (q.y * bx + q.z * by) + q.w * bz;
+    float const c2 = 2.0f * q.x;
+    float const c3 = 2.0f * vb;
+    float const cx = q.z * bz - q.w * by;
+    float const cy = q.w * bx - q.y * bz;
+    float const cz = q.y * by - q.z * bx;
+    ox = (c1 * bx + c2 * cx) + c3 * q.y;
+    oy = (c1 * by + c2 * cy) + c3 * q.z;
+    oz = (c1 * bz + c2 * cz) + c3 * q.w;
+}
+
+enum
+{
+    PMFT_XYT = 0,
+    PMFT_R12 = 1,
+    PMFT_XYZ = 2,
+};
+
+// One pass over the bonds.  Per-point inputs: XYT takes cos / sin of minus the
+// query angle (q_cs) and the points' angles (p_theta); R12 the points' and the
+// query points' angles; XYZ the query quaternions and the equivalent
+// orientations (every bond is binned once per equivalent orientation).
+template<int MODE, bool LDS>
+__global__ __launch_bounds__(kBlock) void pmft3_kernel(const uint32_t* __restrict__ ij, const float* __restrict__ vec,
+                                                       const float* __restrict__ dist, uint64_t nb,
+                                                       const float2* __restrict__ q_cs,
+                                                       const float* __restrict__ p_theta,
+                                                       const float* __restrict__ q_theta,
+                                                       const float4* __restrict__ q_quat,
+                                                       const float4* __restrict__ equiv, uint32_t n_equiv, Axis3 A,
+                                                       uint32_t* __restrict__ counts)
+{
+    extern __shared__ uint32_t s_hist3[];
+    uint32_t const nbins = A.n[0] * A.n[1] * A.n[2];
+    if (LDS)
+    {
+        for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+            s_hist3[t] = 0u;
+        __syncthreads();
+    }
+    auto add = [&](uint32_t bin) {
+        if (bin != kOverflow)
+        {
+            if (LDS)
+                atomicAdd(&s_hist3[bin], 1u);
+            else
+                atomicAdd(&counts[bin], 1u);
+        }
+    };
+    uint64_t const stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t b = uint64_t(fa::xcd_tile(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x; b < nb; b += stride)
+    {
+        uint32_t const i = ij[2 * b], j = ij[2 * b + 1];
+        float const dx = vec[3 * b], dy = vec[3 * b + 1], dz = vec[3 * b + 2];
+        if constexpr (MODE == PMFT_XYT)
+        {
+            // PMFTXYT.cc:84-99
+            float2 const cs = q_cs[i];
+            float const rx = cs.x * dx + (-cs.y) * dy;
+            float const ry = cs.y * dx + cs.x * dy;
+            float const d_theta = angle_of(-dy, -dx);
+            float const t = modulus_positive(p_theta[j] - d_theta, kTwoPi);
+            add(bin3(A, rx, ry, t));
+        }
+        else if constexpr (MODE == PMFT_R12)
+        {
+            // PMFTR12.cc:96-110
+            float const d_theta1 = angle_of(dy, dx);
+            float const d_theta2 = angle_of(-dy, -dx);
+            float const t1 = modulus_positive(p_theta[j] - d_theta1, kTwoPi);
+            float const t2 = modulus_positive(q_theta[i] - d_theta2, kTwoPi);
+            add(bin3(A, dist[b], t1, t2));
+        }
+        else
+        {
+            // PMFTXYZ.cc:125-142: v = rotate(conj(q_i), delta), then rotate(equiv_k, v)
+            float4 q = q_quat[i];
+            q.y = -q.y;
+            q.z = -q.z;
+            q.w = -q.w;
+            float vx, vy, vz;
+            quat_rotate(q, dx, dy, dz, vx, vy, vz);
+            for (uint32_t k = 0; k < n_equiv; ++k)
+            {
+                float wx, wy, wz;
+                quat_rotate(equiv[k], vx, vy, vz, wx, wy, wz);
+                add(bin3(A, wx, wy, wz));
+            }
+        }
+    }
+    if (LDS)
+    {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+            if (s_hist3[t])
+                atomicAdd(&counts[t], s_hist3[t]);
+    }
+}
+
+// RegularAxis(n, min, max) (Histogram.h:126-138)
+void set_axis(Axis3& A, int k, uint32_t n, float mn, float mx)
+{
+    A.min[k] = mn;
+    A.max[k] = mx;
+    volatile float w = (mx - mn) / float(n);
+    A.inv[k] = 1.0f / w;
+    A.n[k] = n;
+}
+
+int grid_for(uint64_t work, int cap);
+
+// Uploads host arrays of the given byte sizes into one scratch region and
+// returns device pointers (null where the host pointer is null).
+int upload_arrays(const void* const* host, const size_t* bytes, int count, void** dev)
+{
+    size_t total = 0;
+    for (int a = 0; a < count; ++a)
+        total += (bytes[a] + 255) & ~size_t(255);
+    char* base = nullptr;
+    if (int rc = fa::scratch(22, total + 256, (void**) &base))
+        return rc;
+    hipStream_t s = fa::stream();
+    size_t off = 0;
+    for (int a = 0; a < count; ++a)
+    {
+        dev[a] = nullptr;
+        if (host[a] && bytes[a])
+        {
+            dev[a] = base + off;
+            FA_HIP(hipMemcpyAsync(dev[a], host[a], bytes[a], hipMemcpyHostToDevice, s));
+        }
+        off += (bytes[a] + 255) & ~size_t(255);
+    }
+    // the copies read caller memory that may change after this call
+    FA_HIP(hipStreamSynchronize(s));
+    return FA_OK;
+}
+
+template<int MODE>
+int launch_pmft3(const fa_nlist* nl, const Axis3& A, const float2* q_cs, const float* p_theta, const float* q_theta,
+                 const float4* q_quat, const float4* equiv, uint32_t n_equiv, uint32_t* d_counts)
+{
+    uint64_t const nbins = uint64_t(A.n[0]) * A.n[1] * A.n[2];
+    uint64_t const nb = nl->num_bonds;
+    hipStream_t s = fa::stream();
+    if (nbins <= kMaxLdsBins2D)
+        FA_KLAUNCH((pmft3_kernel<MODE, true>), dim3(grid_for(nb, 2048)), dim3(kBlock), sizeof(uint32_t) * nbins, s,
+                   nl->neighbors, nl->vectors, nl->distances, nb, q_cs, p_theta, q_theta, q_quat, equiv, n_equiv, A,
+                   d_counts);
+    else
+        FA_KLAUNCH((pmft3_kernel<MODE, false>), dim3(grid_for(nb, 4096)), dim3(kBlock), 0, s, nl->neighbors,
+                   nl->vectors, nl->distances, nb, q_cs, p_theta, q_theta, q_quat, equiv, n_equiv, A, d_counts);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int check_bins3(uint32_t a, uint32_t b, uint32_t c)
+{
+    if (uint64_t(a) * b * c > 0xffffffffull)
+        return fa::fail(FA_ERR_INVALID, "PMFT histogram too large");
+    return FA_OK;
+}
+
 int grid_for(uint64_t work, int cap)
 {
     uint64_t const b = (work + kBlock - 1) / kBlock;
@@ -212,6 +412,184 @@ int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, fl
         FA_KLAUNCH(pmftxy_kernel<false>, dim3(grid_for(nb, 4096)), dim3(kBlock), 0, s, nl->neighbors, nl->vectors,
                    nb, cs, A, d_counts);
     FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int fa_pmftxyt_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, float x_max,
+                          float y_max, uint32_t n_x, uint32_t n_y, uint32_t n_t, uint32_t* d_counts)
+{
+    if (int rc = fa_init())
+        return rc;
+    // PMFTXYT.cc:27-46
+    if (n_x < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYT requires at least 1 bin in X.");
+    if (n_y < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYT requires at least 1 bin in Y.");
+    if (n_t < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYT requires at least 1 bin in T.");
+    if (x_max < 0)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYT requires that x_max must be positive.");
+    if (y_max < 0)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYT requires that y_max must be positive.");
+    if (int rc = check_bins3(n_x, n_y, n_t))
+        return rc;
+    uint32_t const nq = nl->num_query_points, np = nl->num_points;
+    if (nl->num_bonds == 0 || nq == 0)
+        return FA_OK;
+    // cos / sin of minus each query angle with the host C library
+    // (rotmat2::fromAngle, VectorMath.h:912-921)
+    std::vector<float> h_cs(2 * size_t(nq));
+    for (uint32_t i = 0; i < nq; ++i)
+    {
+        float const t = -query_orientations[i];
+        h_cs[2 * i] = std::cos(t);
+        h_cs[2 * i + 1] = std::sin(t);
+    }
+    const void* host[2] = {h_cs.data(), orientations};
+    size_t const bytes[2] = {sizeof(float2) * size_t(nq), sizeof(float) * size_t(np)};
+    void* dev[2];
+    if (int rc = upload_arrays(host, bytes, 2, dev))
+        return rc;
+    Axis3 A;
+    set_axis(A, 0, n_x, -x_max, x_max);
+    set_axis(A, 1, n_y, -y_max, y_max);
+    set_axis(A, 2, n_t, 0.0f, kTwoPi);
+    return launch_pmft3<PMFT_XYT>(nl, A, static_cast<const float2*>(dev[0]), static_cast<const float*>(dev[1]),
+                                  nullptr, nullptr, nullptr, 0, d_counts);
+}
+
+int fa_pmftr12_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, float r_max,
+                          uint32_t n_r, uint32_t n_t1, uint32_t n_t2, uint32_t* d_counts)
+{
+    if (int rc = fa_init())
+        return rc;
+    // PMFTR12.cc:29-46
+    if (n_r < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTR12 requires at least 1 bin in R.");
+    if (n_t1 < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTR12 requires at least 1 bin in T1.");
+    if (n_t2 < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTR12 requires at least 1 bin in T2.");
+    if (r_max < 0)
+        return fa::fail(FA_ERR_INVALID, "PMFTR12 requires that r_max must be positive.");
+    if (int rc = check_bins3(n_r, n_t1, n_t2))
+        return rc;
+    uint32_t const nq = nl->num_query_points, np = nl->num_points;
+    if (nl->num_bonds == 0 || nq == 0)
+        return FA_OK;
+    const void* host[2] = {orientations, query_orientations};
+    size_t const bytes[2] = {sizeof(float) * size_t(np), sizeof(float) * size_t(nq)};
+    void* dev[2];
+    if (int rc = upload_arrays(host, bytes, 2, dev))
+        return rc;
+    Axis3 A;
+    set_axis(A, 0, n_r, 0.0f, r_max);
+    set_axis(A, 1, n_t1, 0.0f, kTwoPi);
+    set_axis(A, 2, n_t2, 0.0f, kTwoPi);
+    return launch_pmft3<PMFT_R12>(nl, A, nullptr, static_cast<const float*>(dev[0]),
+                                  static_cast<const float*>(dev[1]), nullptr, nullptr, 0, d_counts);
+}
+
+int fa_pmftxyz_accumulate(const fa_nlist* nl, const float* query_orientations, const float* equiv_orientations,
+                          uint32_t n_equiv, float x_max, float y_max, float z_max, uint32_t n_x, uint32_t n_y,
+                          uint32_t n_z, uint32_t* d_counts)
+{
+    if (int rc = fa_init())
+        return rc;
+    // PMFTXYZ.cc:22-48
+    if (n_x < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYZ requires at least 1 bin in X.");
+    if (n_y < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYZ requires at least 1 bin in Y.");
+    if (n_z < 1)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYZ requires at least 1 bin in Z.");
+    if (x_max < 0)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYZ requires that x_max must be positive.");
+    if (y_max < 0)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYZ requires that y_max must be positive.");
+    if (z_max < 0)
+        return fa::fail(FA_ERR_INVALID, "PMFTXYZ requires that z_max must be positive.");
+    if (int rc = check_bins3(n_x, n_y, n_z))
+        return rc;
+    uint32_t const nq = nl->num_query_points;
+    if (nl->num_bonds == 0 || nq == 0 || n_equiv == 0)
+        return FA_OK;
+    const void* host[2] = {query_orientations, equiv_orientations};
+    size_t const bytes[2] = {sizeof(float4) * size_t(nq), sizeof(float4) * size_t(n_equiv)};
+    void* dev[2];
+    if (int rc = upload_arrays(host, bytes, 2, dev))
+        return rc;
+    Axis3 A;
+    set_axis(A, 0, n_x, -x_max, x_max);
+    set_axis(A, 1, n_y, -y_max, y_max);
+    set_axis(A, 2, n_z, -z_max, z_max);
+    return launch_pmft3<PMFT_XYZ>(nl, A, nullptr, nullptr, nullptr, static_cast<const float4*>(dev[0]),
+                                  static_cast<const float4*>(dev[1]), n_equiv, d_counts);
+}
+
+int fa_pmft_reduce(int kind, const fa_box* box, const uint32_t* counts, const uint32_t* nbins, const float* maxes,
+                   uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, uint32_t n_equiv, float* pcf)
+{
+    // PMFT.h:76-87 (XYT, R12) and PMFTXYZ.cc:92-103, in the reference's float order
+    uint64_t const total = uint64_t(nbins[0]) * nbins[1] * nbins[2];
+    float const inv_num_dens = fa::box_volume(*box) / float(n_query_points);
+    volatile float den = float(n_frames) * float(n_points);
+    if (kind == FA_PMFT_XYZ)
+        den = den * float(n_equiv);
+    float const norm_factor = float(1.0) / den;
+    float const prefactor = inv_num_dens * norm_factor;
+    std::vector<float> jf;
+    if (kind == FA_PMFT_XYT)
+    {
+        // PMFTXYT.cc:52-57,68-72
+        volatile float dx = float(2.0) * maxes[0] / float(nbins[0]);
+        volatile float dy = float(2.0) * maxes[1] / float(nbins[1]);
+        volatile float dt = 1 / float(nbins[2]);
+        volatile float dxy = dx * dy;
+        volatile float jac = dxy * dt;
+        jf.assign(1, float(1.0) / jac);
+    }
+    else if (kind == FA_PMFT_XYZ)
+    {
+        // PMFTXYZ.cc:53-70,96
+        volatile float dx = float(2.0) * maxes[0] / float(nbins[0]);
+        volatile float dy = float(2.0) * maxes[1] / float(nbins[1]);
+        volatile float dz = float(2.0) * maxes[2] / float(nbins[2]);
+        volatile float dxy = dx * dy;
+        volatile float jac = dxy * dz;
+        jf.assign(1, float(1.0) / jac);
+    }
+    else if (kind == FA_PMFT_R12)
+    {
+        // PMFTR12.cc:58-77: 1 / (r_center * (dr * dt1 * dt2)) per r bin;
+        // centers (edge_i + edge_{i+1}) / 2 of RegularAxis(n_r, 0, r_max)
+        uint32_t const n_r = nbins[0];
+        volatile float w = (maxes[0] - 0.0f) / float(n_r);
+        float const width = w;
+        volatile float dr = maxes[0] / float(n_r);
+        volatile float dt1 = kTwoPi / float(nbins[1]);
+        volatile float dt2 = 1 / float(nbins[2]);
+        volatile float p1 = dr * dt1;
+        volatile float product = p1 * dt2;
+        jf.resize(n_r);
+        for (uint32_t r = 0; r < n_r; ++r)
+        {
+            volatile float e0 = 0.0f + float(r) * width;
+            volatile float e1 = 0.0f + float(r + 1) * width;
+            volatile float c = (e0 + e1) / float(2.0);
+            volatile float rp = c * product;
+            jf[r] = float(1.0) / rp;
+        }
+    }
+    else
+        return fa::fail(FA_ERR_INVALID, "unknown PMFT kind");
+    uint64_t const per_r = uint64_t(nbins[1]) * nbins[2];
+    for (uint64_t t = 0; t < total; ++t)
+    {
+        float const j = kind == FA_PMFT_R12 ? jf[t / per_r] : jf[0];
+        volatile float a = float(counts[t]) * prefactor;
+        pcf[t] = a * j;
+    }
     return FA_OK;
 }
 

@@ -284,6 +284,41 @@ int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, fl
 int fa_pmftxy_reduce(const fa_box* box, const uint32_t* counts, uint32_t n_x, uint32_t n_y, float x_max, float y_max,
                      uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, float* pcf);
 
+/* PMFTXYT::accumulate (freud/pmft/PMFTXYT.cc:76-100): per bond, (x, y)
+ * rotated by minus the query point's angle (host cos / sin, as PMFTXY) and
+ * t = modulusPositive(theta_j - atan2(-y, -x), 2 pi), binned into
+ * n_x x n_y x n_t over [-x_max, x_max) x [-y_max, y_max) x [0, 2 pi).
+ * orientations: host, num_points angles; query_orientations: host,
+ * num_query_points angles.  The reference's angle comes from the host C
+ * library's float atan2; here it is atan2 in double rounded to float. */
+int fa_pmftxyt_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, float x_max,
+                          float y_max, uint32_t n_x, uint32_t n_y, uint32_t n_t, uint32_t* d_counts);
+/* PMFTR12::accumulate (freud/pmft/PMFTR12.cc:90-111): per bond (d, t1, t2)
+ * with t1 = modulusPositive(theta_j - atan2(y, x), 2 pi) and
+ * t2 = modulusPositive(theta_i - atan2(-y, -x), 2 pi), binned into
+ * n_r x n_t1 x n_t2 over [0, r_max) x [0, 2 pi)^2.  Host angle arrays as
+ * fa_pmftxyt_accumulate. */
+int fa_pmftr12_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, float r_max,
+                          uint32_t n_r, uint32_t n_t1, uint32_t n_t2, uint32_t* d_counts);
+/* PMFTXYZ::accumulate (freud/pmft/PMFTXYZ.cc:106-145): per bond and per
+ * equivalent orientation k, v = rotate(equiv_k, rotate(conj(q_i), r)) binned
+ * into n_x x n_y x n_z over [-x_max, x_max) x [-y_max, y_max) x
+ * [-z_max, z_max).  query_orientations: host quaternions (s, x, y, z), one
+ * per query point; equiv_orientations: host, n_equiv quaternions. */
+int fa_pmftxyz_accumulate(const fa_nlist* nl, const float* query_orientations, const float* equiv_orientations,
+                          uint32_t n_equiv, float x_max, float y_max, float z_max, uint32_t n_x, uint32_t n_y,
+                          uint32_t n_z, uint32_t* d_counts);
+#define FA_PMFT_XYT 0
+#define FA_PMFT_R12 1
+#define FA_PMFT_XYZ 2
+/* PMFT::reduce (freud/pmft/PMFT.h:76-87 with the Jacobians of PMFTXYT.cc:52-57,
+ * PMFTR12.cc:58-77) and PMFTXYZ::reduce (PMFTXYZ.cc:92-103, normalised also by
+ * n_equiv): pcf (host, nbins[0] * nbins[1] * nbins[2]) from host counts.
+ * maxes: (x_max, y_max, -) for XYT, (r_max, -, -) for R12, (x_max, y_max,
+ * z_max) for XYZ. */
+int fa_pmft_reduce(int kind, const fa_box* box, const uint32_t* counts, const uint32_t* nbins, const float* maxes,
+                   uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, uint32_t n_equiv, float* pcf);
+
 /* ---------------------------------------------------------------- order
  * Steinhardt::compute over a NeighborList (freud/order/Steinhardt.cc:85-359;
  * binding export-Steinhardt.cc:19-34).  ls[n_l] are the l values.  Outputs

@@ -20,6 +20,8 @@ __all__ = [
     "local_density",
     "pmftxy_accumulate",
     "pmftxy_reduce",
+    "pmft3_accumulate",
+    "pmft3_reduce",
     "box_op",
     "nearest_plane_distance",
     "query",
@@ -407,3 +409,52 @@ def pmftxy_reduce(box, counts, x_max, y_max, n_points, n_query_points, n_frames)
                           ctypes.c_float(y_max), ctypes.c_uint32(n_points), ctypes.c_uint32(n_query_points),
                           ctypes.c_uint32(n_frames), _p(pcf))
     return pcf
+
+
+_PMFT3_KIND = {"xyt": 0, "r12": 1, "xyz": 2}
+
+
+def _pmft3_axes(kind, maxes, bins):
+    two_pi = np.float32(2.0 * np.pi)
+    m = [np.float32(v) for v in maxes]
+    if kind == "xyt":
+        return [(-m[0], m[0]), (-m[1], m[1]), (np.float32(0), two_pi)]
+    if kind == "r12":
+        return [(np.float32(0), m[0]), (np.float32(0), two_pi), (np.float32(0), two_pi)]
+    return [(-m[0], m[0]), (-m[1], m[1]), (-m[2], m[2])]
+
+
+def pmft3_accumulate(kind, nl, maxes, bins, p_theta=None, q_theta=None, q_quat=None, equiv=None, counts=None):
+    """PMFTXYT ('xyt'), PMFTR12 ('r12') or PMFTXYZ ('xyz') bin counts of a
+    NeighborList dict (PMFTXYT.cc:76-100, PMFTR12.cc:90-111, PMFTXYZ.cc:106-145)."""
+    lib = _load()
+    ax = _pmft3_axes(kind, maxes, bins)
+    n = np.array(bins, np.uint32)
+    lo = np.array([a[0] for a in ax], np.float32)
+    hi = np.array([a[1] for a in ax], np.float32)
+    if counts is None:
+        counts = np.zeros(int(np.prod(n)), np.uint32)
+    ij = np.ascontiguousarray(nl["ij"], np.uint32)
+    vec = _f32(nl["vectors"])
+    dist = _f32(nl["distances"])
+    pt = _f32(p_theta) if p_theta is not None else np.zeros(1, np.float32)
+    qt = _f32(q_theta) if q_theta is not None else np.zeros(1, np.float32)
+    qq = _f32(q_quat) if q_quat is not None else np.zeros(4, np.float32)
+    eq = _f32(equiv) if equiv is not None else np.array([1, 0, 0, 0], np.float32)
+    n_equiv = eq.size // 4
+    lib.orc_pmft3_accumulate(_PMFT3_KIND[kind], _p(ij), _p(vec), _p(dist), ctypes.c_uint64(len(dist)), _p(pt),
+                             _p(qt), _p(qq), _p(eq), ctypes.c_uint32(n_equiv), _p(n), _p(lo), _p(hi), _p(counts))
+    return counts.reshape(tuple(int(v) for v in n))
+
+
+def pmft3_reduce(kind, box, counts, maxes, n_points, n_query_points, n_frames, n_equiv=1):
+    lib = _load()
+    b, is2d = _boxp(box)
+    c = np.ascontiguousarray(counts, dtype=np.uint32)
+    n = np.array(c.shape, np.uint32)
+    mx = np.array(list(maxes) + [0.0] * (3 - len(maxes)), np.float32)
+    pcf = np.empty(c.size, np.float32)
+    lib.orc_pmft3_reduce(_PMFT3_KIND[kind], _p(b), is2d, _p(c), _p(n), _p(mx), ctypes.c_uint32(n_points),
+                         ctypes.c_uint32(n_query_points), ctypes.c_uint32(n_frames), ctypes.c_uint32(n_equiv),
+                         _p(pcf))
+    return pcf.reshape(c.shape)

@@ -1841,4 +1841,137 @@ void orc_pmftxy_reduce(const float* boxp, int is2D, const uint32_t* counts, uint
         pcf[t] = static_cast<float>(counts[t]) * prefactor * jacobian_factor;
 }
 
+// PMFTXYT.cc:76-100, PMFTR12.cc:90-111, PMFTXYZ.cc:106-145 (kind 0, 1, 2).
+// n, lo, hi: the three RegularAxis(n, lo, hi).  XYT and R12 take angles
+// (p_theta per point, q_theta per query point); XYZ takes query quaternions
+// (s, x, y, z) and n_equiv equivalent orientations.
+static float modulus_positive_f(float a, float b) { return std::fmod(std::fmod(a, b) + b, b); }
+
+struct quatf
+{
+    float s;
+    v3 v;
+};
+// VectorMath.h:810-822
+static v3 quat_rotate(const quatf& a, v3 b)
+{
+    float const c1 = a.s * a.s - dot(a.v, a.v);
+    v3 const cr = {a.v.y * b.z - a.v.z * b.y, a.v.z * b.x - a.v.x * b.z, a.v.x * b.y - a.v.y * b.x};
+    float const c2 = 2 * a.s;
+    float const c3 = 2 * dot(a.v, b);
+    v3 const t1 = {c1 * b.x, c1 * b.y, c1 * b.z};
+    v3 const t2 = {c2 * cr.x, c2 * cr.y, c2 * cr.z};
+    v3 const t3 = {c3 * a.v.x, c3 * a.v.y, c3 * a.v.z};
+    return add(add(t1, t2), t3);
+}
+
+void orc_pmft3_accumulate(int kind, const uint32_t* ij, const float* vec, const float* dist, uint64_t nb,
+                          const float* p_theta, const float* q_theta, const float* q_quat, const float* equiv,
+                          uint32_t n_equiv, const uint32_t* n, const float* lo, const float* hi, uint32_t* counts)
+{
+    float const two_pi = float(2.0 * M_PI); // constants::TWO_PI
+    float inv[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        float const w = (hi[k] - lo[k]) / static_cast<float>(n[k]);
+        inv[k] = static_cast<float>(1.0) / w;
+    }
+    auto bin = [&](float a, float b, float c) {
+        long const b0 = regular_bin(a, lo[0], hi[0], inv[0], n[0]);
+        long const b1 = regular_bin(b, lo[1], hi[1], inv[1], n[1]);
+        long const b2 = regular_bin(c, lo[2], hi[2], inv[2], n[2]);
+        if (b0 < 0 || b1 < 0 || b2 < 0)
+            return;
+        counts[(size_t(b0) * n[1] + size_t(b1)) * n[2] + size_t(b2)] += 1;
+    };
+    for (uint64_t e = 0; e < nb; ++e)
+    {
+        uint32_t const i = ij[2 * e], j = ij[2 * e + 1];
+        v3 const d = {vec[3 * e], vec[3 * e + 1], vec[3 * e + 2]};
+        if (kind == 0)
+        {
+            // rotmat2::fromAngle(-theta_i) * (x, y), then the point's relative angle
+            float const t = -q_theta[i];
+            float const c = std::cos(t), sn = std::sin(t);
+            float const rx = c * d.x + (-sn) * d.y;
+            float const ry = sn * d.x + c * d.y;
+            float const d_theta = std::atan2(-d.y, -d.x);
+            float const tt = modulus_positive_f(p_theta[j] - d_theta, two_pi);
+            bin(rx, ry, tt);
+        }
+        else if (kind == 1)
+        {
+            float const d_theta1 = std::atan2(d.y, d.x);
+            float const d_theta2 = std::atan2(-d.y, -d.x);
+            float const t1 = modulus_positive_f(p_theta[j] - d_theta1, two_pi);
+            float const t2 = modulus_positive_f(q_theta[i] - d_theta2, two_pi);
+            bin(dist[e], t1, t2);
+        }
+        else
+        {
+            quatf const q = {q_quat[4 * i], {-q_quat[4 * i + 1], -q_quat[4 * i + 2], -q_quat[4 * i + 3]}};
+            for (uint32_t k = 0; k < n_equiv; ++k)
+            {
+                v3 v = quat_rotate(q, d);
+                quatf const eq = {equiv[4 * k], {equiv[4 * k + 1], equiv[4 * k + 2], equiv[4 * k + 3]}};
+                v = quat_rotate(eq, v);
+                bin(v.x, v.y, v.z);
+            }
+        }
+    }
+}
+
+// PMFT::reduce (PMFT.h:76-87) with the Jacobians of PMFTXYT.cc:52-57 and
+// PMFTR12.cc:58-77; PMFTXYZ::reduce (PMFTXYZ.cc:92-103).  maxes as the library.
+void orc_pmft3_reduce(int kind, const float* boxp, int is2D, const uint32_t* counts, const uint32_t* n,
+                      const float* maxes, uint32_t n_points, uint32_t n_query_points, uint32_t n_frames,
+                      uint32_t n_equiv, float* pcf)
+{
+    Box b = make_box(boxp, is2D);
+    float const two_pi = float(2.0 * M_PI);
+    size_t const total = size_t(n[0]) * n[1] * n[2];
+    std::vector<float> jf(kind == 1 ? n[0] : 1);
+    float inv_num_dens = box_volume(b) / static_cast<float>(n_query_points);
+    float norm_factor;
+    if (kind == 2)
+        norm_factor = (float) 1.0 / ((float) n_frames * (float) n_points * (float) n_equiv);
+    else
+        norm_factor = float(1.0) / (static_cast<float>(n_frames) * static_cast<float>(n_points));
+    float const prefactor = inv_num_dens * norm_factor;
+    if (kind == 0)
+    {
+        float const dx = float(2.0) * maxes[0] / float(n[0]);
+        float const dy = float(2.0) * maxes[1] / float(n[1]);
+        float const dt = 1 / float(n[2]);
+        float const jacobian = dx * dy * dt;
+        jf[0] = (float) 1.0 / jacobian;
+    }
+    else if (kind == 2)
+    {
+        float const dx = float(2.0) * maxes[0] / float(n[0]);
+        float const dy = float(2.0) * maxes[1] / float(n[1]);
+        float const dz = float(2.0) * maxes[2] / float(n[2]);
+        float const jacobian = dx * dy * dz;
+        jf[0] = (float) 1.0 / jacobian;
+    }
+    else
+    {
+        float const width = (maxes[0] - 0.0f) / static_cast<float>(n[0]);
+        float const dr = maxes[0] / float(n[0]);
+        float const dt1 = two_pi / float(n[1]);
+        float const dt2 = 1 / float(n[2]);
+        float const product = dr * dt1 * dt2;
+        for (uint32_t r = 0; r < n[0]; ++r)
+        {
+            float const e0 = 0.0f + static_cast<float>(r) * width;
+            float const e1 = 0.0f + static_cast<float>(r + 1) * width;
+            float const c = (e0 + e1) / static_cast<float>(2.0);
+            jf[r] = (float) 1.0 / (c * product);
+        }
+    }
+    size_t const per_r = size_t(n[1]) * n[2];
+    for (size_t t = 0; t < total; ++t)
+        pcf[t] = static_cast<float>(counts[t]) * prefactor * (kind == 1 ? jf[t / per_r] : jf[0]);
+}
+
 } // extern "C"

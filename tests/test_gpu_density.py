@@ -104,3 +104,186 @@ def test_pmftxy_known_answers():
     assert tuple(np.asarray(np.where(pm.bin_counts)).squeeze()) == (7, 5)
     with pytest.raises(ValueError, match="only supports"):
         pm.compute((Box.cube(8), p1), np.zeros(1, np.float32))
+
+
+# ------------------------------------------------------------------ PMFTXYT, PMFTR12, PMFTXYZ
+_TWO_PI = 2 * np.pi
+
+
+def _quat_axis_angle(axis, angles):
+    axis = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    a = np.atleast_1d(np.asarray(angles, np.float64))
+    return np.concatenate([np.cos(a / 2)[:, None], np.sin(a / 2)[:, None] * axis[None, :]], 1)
+
+
+def _alternating_lattice(size, angle=0.0, extra_shell=2):
+    # tests/util.py:66-112 of the reference
+    c, s = np.cos(angle), np.sin(angle)
+    rot = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    p1, p2 = [], []
+    for i in range(-extra_shell, size + extra_shell):
+        for j in range(-extra_shell, size + extra_shell):
+            p = rot.dot(np.array([i, j, 0]))
+            if (i + j) % 2 == 0:
+                p2.append(p)
+            elif 0 <= i < size and 0 <= j < size:
+                p1.append(p)
+    return np.array(p1, np.float32), np.array(p2, np.float32)
+
+
+def test_pmftxyz_matches_oracle(oracle):
+    box, pts = freud_amd.data.make_random_system(9, 3000, seed=7)
+    pts = pts.astype(np.float32)
+    rng = np.random.default_rng(8)
+    q = rng.normal(size=(len(pts), 4))
+    q = (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+    equiv = np.array([[1, 0, 0, 0], [0, 0, 0, 1]], np.float32)
+    lim = (1.6, 1.7, 1.8)
+    pm = freud_amd.pmft.PMFTXYZ(*lim, (20, 22, 24))
+    pm.compute((box, pts), q, equiv_orientations=equiv)
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=float(pm.r_max), exclude_ii=True), nthreads=8)
+    counts = oracle.pmft3_accumulate("xyz", ref_nl, lim, (20, 22, 24), q_quat=q, equiv=equiv)
+    npt.assert_array_equal(pm.bin_counts, counts)
+    npt.assert_array_equal(pm._pcf, oracle.pmft3_reduce("xyz", _bt(box), counts, lim, len(pts), len(pts), 1, 2))
+    pm.compute((box, pts), q, equiv_orientations=equiv, reset=False)
+    npt.assert_array_equal(pm.bin_counts, 2 * counts)
+    with pytest.raises(RuntimeError, match="equivalent orientations"):
+        pm.compute((box, pts), q, equiv_orientations=equiv[:1], reset=False)
+    assert pm.nbins == [20, 22, 24]
+    npt.assert_allclose(pm.bin_edges[2], np.linspace(-1.8, 1.8, 25, dtype=np.float32), atol=1e-5)
+    # a 2D box is refused (Box::enforce3D)
+    with pytest.raises(ValueError, match="only supports"):
+        freud_amd.pmft.PMFTXYZ(*lim, 4).compute((Box.square(9), pts * [1, 1, 0]), q)
+
+
+def _angle_parity(gpu, ref, axes):
+    """Counts equal except for bonds whose angle lies within 4 ulp of a bin
+    edge of its angle axis (the reference's angle is the host library's float
+    atan2, the GPU's atan2 in double rounded to float).  axes: [(angles of
+    every bond in float64, bins of that axis)]."""
+    risky = None
+    for t_vals, n_t in axes:
+        u = t_vals * n_t / _TWO_PI
+        r = np.abs(u - np.round(u)) < 4 * np.spacing(np.float32(_TWO_PI)) * n_t / _TWO_PI
+        risky = r if risky is None else risky | r
+    diff = gpu.astype(np.int64) - ref.astype(np.int64)
+    assert np.abs(diff).sum() <= 2 * int(risky.sum())
+    assert abs(int(gpu.sum()) - int(ref.sum())) <= int(risky.sum())
+    return int(np.abs(diff).sum()), int(risky.sum())
+
+
+@pytest.mark.parametrize("cls", ["xyt", "r12"])
+def test_pmft_angles_match_oracle(oracle, cls):
+    box, pts = freud_amd.data.make_random_system(20, 6000, is2D=True, seed=11)
+    pts = pts.astype(np.float32)
+    rng = np.random.default_rng(12)
+    theta = (rng.random(len(pts)) * 2 * np.pi).astype(np.float32)
+    if cls == "xyt":
+        pm = freud_amd.pmft.PMFTXYT(2.2, 2.3, (30, 32, 36))
+        maxes, bins = (2.2, 2.3), (30, 32, 36)
+    else:
+        pm = freud_amd.pmft.PMFTR12(3.1, (25, 36, 40))
+        maxes, bins = (3.1,), (25, 36, 40)
+    pm.compute((box, pts), theta)
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=float(pm.r_max), exclude_ii=True), nthreads=8)
+    counts = oracle.pmft3_accumulate(cls, ref_nl, maxes, bins, p_theta=theta, q_theta=theta)
+    v = ref_nl["vectors"].astype(np.float64)
+    i, j = ref_nl["ij"][:, 0], ref_nl["ij"][:, 1]
+    if cls == "xyt":
+        t = (theta[j] - np.arctan2(-v[:, 1], -v[:, 0])) % _TWO_PI
+        d, r = _angle_parity(pm.bin_counts, counts, [(t, bins[2])])
+    else:
+        t1 = (theta[j] - np.arctan2(v[:, 1], v[:, 0])) % _TWO_PI
+        t2 = (theta[i] - np.arctan2(-v[:, 1], -v[:, 0])) % _TWO_PI
+        d, r = _angle_parity(pm.bin_counts, counts, [(t1, bins[1]), (t2, bins[2])])
+    print(f"{cls}: {d} count differences, {r} bonds within 4 ulp of an angle edge", flush=True)
+    pcf = oracle.pmft3_reduce(cls, _bt(box), pm.bin_counts, maxes, len(pts), len(pts), 1)
+    npt.assert_array_equal(pm._pcf, pcf)
+
+
+def test_pmft_angle_known_answers():
+    # two particles (test_pmft.py:119-154)
+    pts = np.array([[-1.0, 0.0, 0.0], [1.0, 0.1, 0.0]], np.float32)
+    quats = np.array([[1, 0, 0, 0], [1, 0, 0, 0]], np.float32)
+    pm = freud_amd.pmft.PMFTR12(5.23, (10, 20, 30))
+    pm.compute((Box.square(16), pts), quats)
+    exp = np.zeros((10, 20, 30), np.int64)
+    for q, p in ((0, 1), (1, 0)):
+        r = pts[p].astype(np.float64) - pts[q]
+        exp[int(np.floor(np.linalg.norm(r) * 10 / 5.23)), int(np.floor((-np.arctan2(r[1], r[0]) % _TWO_PI) * 20 / _TWO_PI)),
+            int(np.floor((-np.arctan2(-r[1], -r[0]) % _TWO_PI) * 30 / _TWO_PI))] += 1
+    npt.assert_array_equal(pm.bin_counts, exp)
+    pm = freud_amd.pmft.PMFTXYT(3.6, 4.2, (20, 30, 40))
+    pm.compute((Box.square(16), pts), quats)
+    lim = np.array([3.6, 4.2])
+    exp = np.zeros((20, 30, 40), np.int64)
+    for q, p in ((0, 1), (1, 0)):
+        r = pts[p].astype(np.float64) - pts[q]
+        xy = np.floor((r[:2] + lim) * np.array([20, 30]) / (2 * lim)).astype(int)
+        exp[xy[0], xy[1], int(np.floor((-np.arctan2(-r[1], -r[0]) % _TWO_PI) * 40 / _TWO_PI))] += 1
+    npt.assert_array_equal(pm.bin_counts, exp)
+    with pytest.raises(ValueError, match="only supports"):
+        pm.compute((Box.cube(16), pts), quats)
+    # points != query points on an alternating lattice (test_pmft.py:248-270, 315-344)
+    points, query_points = _alternating_lattice(10, 0.01, 2)
+    box = Box.square(50)
+    pm = freud_amd.pmft.PMFTR12(2.3, 10)
+    pm.compute((box, points), np.zeros(len(points)), query_points, np.zeros(len(query_points)),
+               neighbors=dict(mode="ball", r_max=2.3, exclude_ii=False))
+    with np.errstate(divide="ignore"):
+        assert np.count_nonzero(np.isinf(pm.pmft) == 0) == 12
+        assert len(np.unique(pm.pmft)) == 3
+    pm = freud_amd.pmft.PMFTXYT(2.5, 2.5, (10, 10, 4))
+    pm.compute((box, points), np.zeros(len(points)), query_points, np.zeros(len(query_points)),
+               neighbors=dict(mode="ball", r_max=float(np.sqrt(2 * 2.5**2)), exclude_ii=False))
+    with np.errstate(divide="ignore"):
+        for t in range(4):
+            assert np.count_nonzero(np.isinf(pm.pmft[..., t]) == 0) == 3
+        assert len(np.unique(pm.pmft)) == 2
+
+
+@pytest.mark.parametrize("angle", [0.0, np.pi / 4])
+@pytest.mark.parametrize("query_angle", [0.01, np.pi / 4 + 0.01])
+def test_pmftxyt_nontrivial_orientations(angle, query_angle):
+    # test_pmft.py:346-376
+    p = np.array([[-1.0, 0.0, 0.0]], np.float32)
+    q = np.array([[0.9, 0.1, 0.0]], np.float32)
+    pm = freud_amd.pmft.PMFTXYT(2, 2, 4)
+    pm.compute((Box.square(6), p), [angle], q, [query_angle])
+    r = p[0].astype(np.float64) - q[0]
+    c, s = np.cos(-query_angle), np.sin(-query_angle)
+    rr = np.array([c * r[0] - s * r[1], s * r[0] + c * r[1]])
+    xy = np.floor((rr + 2) * 4 / 4).astype(int)
+    tb = int(np.floor(((angle - np.arctan2(-r[1], -r[0])) % _TWO_PI) * 4 / _TWO_PI))
+    assert tuple(np.asarray(np.where(pm.bin_counts)).flatten()) == (xy[0], xy[1], tb)
+    assert pm.bin_counts.sum() == 1
+
+
+def test_pmftxyz_known_answers():
+    # test_pmft.py:708-727 (shift), 729-779 (nearest neighbours)
+    points = np.array([[1, 1, 1], [0, 0, 0]], np.float32)
+    qo = np.array([[1, 0, 0, 0], [1, 0, 0, 0]], np.float32)
+    noshift = freud_amd.pmft.PMFTXYZ(0.5, 0.5, 0.5, 3, shiftvec=[0, 0, 0])
+    shift = freud_amd.pmft.PMFTXYZ(0.5, 0.5, 0.5, 3, shiftvec=[1, 1, 1])
+    for pm in (noshift, shift):
+        pm.compute((Box.cube(3), points), qo)
+    with np.errstate(divide="ignore"):
+        assert np.isfinite(noshift.pmft).sum() == 0
+        assert np.isfinite(shift.pmft).sum() == 1
+    box = Box.cube(8)
+    pts = np.array([[0, 0, 0]], np.float32)
+    qp = np.array([[1.1, 0, 0], [-1.2, 0, 0], [0, 1.3, 0], [0, -1.4, 0], [0, 0, 1.5], [0, 0, -1.6]], np.float32)
+    pm = freud_amd.pmft.PMFTXYZ(3, 3, 3, 3)
+    pm.compute((box, pts), np.tile([1.0, 0, 0, 0], (6, 1)), qp, neighbors={"mode": "nearest", "num_neighbors": 1})
+    npt.assert_array_equal(pm.bin_counts, [[[0, 0, 0], [0, 1, 0], [0, 0, 0]], [[0, 1, 0], [1, 0, 1], [0, 1, 0]],
+                                           [[0, 0, 0], [0, 1, 0], [0, 0, 0]]])
+    pm.compute((box, qp), [[1.0, 0, 0, 0]], pts, neighbors={"mode": "nearest", "num_neighbors": 1})
+    assert pm.bin_counts[2, 1, 1] == 1 and pm.bin_counts.sum() == 1
+    # the query point's orientation applies (test_pmft.py:781-857)
+    p1 = np.array([[0.1, 0.1, 0]], np.float32)
+    p2 = np.array([[1, 0, 0]], np.float32)
+    pm = freud_amd.pmft.PMFTXYZ(3, 3, 3, 12)
+    pm.compute((box, p1), _quat_axis_angle([0, 0, 1], 0.0), p2, neighbors={"mode": "nearest", "num_neighbors": 1})
+    npt.assert_array_equal(np.asarray(np.where(pm.bin_counts)).squeeze(), (4, 6, 6))
+    pm.compute((box, p2), _quat_axis_angle([0, 0, 1], 0.0), p1, neighbors={"mode": "nearest", "num_neighbors": 1})
+    npt.assert_array_equal(np.asarray(np.where(pm.bin_counts)).squeeze(), (7, 5, 6))

@@ -833,3 +833,203 @@ def test_cellquery_rdf_counts_match_bonds(oracle):
     counts = oracle.rdf_accumulate(_box_tuple(box), pts, pts, 60, 3.0, nq_type="cellquery", nthreads=4)
     nl = oracle.query(_box_tuple(box), pts, pts, dict(r_max=3.0, exclude_ii=True), nq_type="cellquery", nthreads=4)
     npt.assert_array_equal(counts, oracle.rdf_bin(nl["distances"], 60, 0.0, 3.0))
+
+
+# --------------------------------------------------------------------- PMFTXYT, PMFTR12, PMFTXYZ
+_TWO_PI = 2 * np.pi
+
+
+def _quat_axis_angle(axis, angles):
+    axis = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    a = np.atleast_1d(np.asarray(angles, np.float64))
+    return np.concatenate([np.cos(a / 2)[:, None], np.sin(a / 2)[:, None] * axis[None, :]], 1)
+
+
+def _quat_rotate64(q, v):
+    """Rotation of v by unit quaternion q = (s, x, y, z), float64 (q v q*)."""
+    q = np.asarray(q, np.float64).reshape(-1, 4)
+    v = np.asarray(v, np.float64).reshape(-1, 3)
+    s, u = q[:, :1], q[:, 1:]
+    return (s * s - np.sum(u * u, 1, keepdims=True)) * v + 2 * s * np.cross(u, v) + 2 * np.sum(u * v, 1,
+                                                                                            keepdims=True) * u
+
+
+def _two_particle_nl(oracle, box, pts, r_max):
+    return oracle.query(_box_tuple(box), pts, pts, dict(r_max=float(r_max), exclude_ii=True))
+
+
+def test_pmft3_two_particles(oracle):
+    # test_pmft.py:119-154 with each class's get_bin (:207-240, :272-313, :626-661)
+    pts = np.array([[-1.0, 0.0, 0.0], [1.0, 0.1, 0.0]], np.float32)
+    theta = np.zeros(2, np.float32)
+    # R12, limits (5.23,), bins (10, 20, 30)
+    nl = _two_particle_nl(oracle, Box.square(16), pts, 5.23)
+    c = oracle.pmft3_accumulate("r12", nl, (5.23,), (10, 20, 30), p_theta=theta, q_theta=theta)
+    exp = np.zeros((10, 20, 30), np.int64)
+    for q, p in ((0, 1), (1, 0)):
+        r = pts[p].astype(np.float64) - pts[q]
+        rb = int(np.floor(np.linalg.norm(r) * 10 / 5.23))
+        t1 = int(np.floor(((0 - np.arctan2(r[1], r[0])) % _TWO_PI) * 20 / _TWO_PI))
+        t2 = int(np.floor(((0 - np.arctan2(-r[1], -r[0])) % _TWO_PI) * 30 / _TWO_PI))
+        exp[rb, t1, t2] += 1
+    npt.assert_array_equal(c, exp)
+    # XYT, limits (3.6, 4.2), bins (20, 30, 40)
+    lim = np.array([3.6, 4.2])
+    nl = _two_particle_nl(oracle, Box.square(16), pts, np.linalg.norm(lim))
+    c = oracle.pmft3_accumulate("xyt", nl, (3.6, 4.2), (20, 30, 40), p_theta=theta, q_theta=theta)
+    exp = np.zeros((20, 30, 40), np.int64)
+    for q, p in ((0, 1), (1, 0)):
+        r = pts[p].astype(np.float64) - pts[q]
+        xy = np.floor((r[:2] + lim) * np.array([20, 30]) / (2 * lim)).astype(int)
+        tb = int(np.floor(((0 - np.arctan2(-r[1], -r[0])) % _TWO_PI) * 40 / _TWO_PI))
+        exp[xy[0], xy[1], tb] += 1
+    npt.assert_array_equal(c, exp)
+    # XYZ, limits (5.23, 6.23, 7.23), bins (100, 110, 120); equivalent
+    # orientations count every bond once each
+    lim = np.array([5.23, 6.23, 7.23])
+    nl = _two_particle_nl(oracle, Box.cube(25), pts, np.linalg.norm(lim))
+    quats = np.array([[1, 0, 0, 0]] * 2, np.float32)
+    c = oracle.pmft3_accumulate("xyz", nl, lim, (100, 110, 120), q_quat=quats)
+    exp = np.zeros((100, 110, 120), np.int64)
+    for q, p in ((0, 1), (1, 0)):
+        r = pts[p].astype(np.float64) - pts[q]
+        b = np.floor((r + lim) * np.array([100, 110, 120]) / (2 * lim)).astype(int)
+        exp[tuple(b)] += 1
+    npt.assert_array_equal(c, exp)
+    c2 = oracle.pmft3_accumulate("xyz", nl, lim, (100, 110, 120), q_quat=quats, equiv=[[1, 0, 0, 0]] * 2)
+    npt.assert_array_equal(c2, 2 * exp)
+    box_t = _box_tuple(Box.cube(25))
+    npt.assert_allclose(oracle.pmft3_reduce("xyz", box_t, c2, lim, 2, 2, 1, n_equiv=2),
+                        oracle.pmft3_reduce("xyz", box_t, c, lim, 2, 2, 1, n_equiv=1), rtol=1e-6)
+
+
+@pytest.mark.parametrize("angle", [0.0, np.pi / 4])
+@pytest.mark.parametrize("query_angle", [0.01, np.pi / 4 + 0.01])
+def test_pmftxyt_nontrivial_orientations(oracle, angle, query_angle):
+    # test_pmft.py:346-376: the point's angle enters t, the query point's the rotation
+    box = Box.square(6)
+    p = np.array([[-1.0, 0.0, 0.0]], np.float32)
+    q = np.array([[0.9, 0.1, 0.0]], np.float32)
+    nl = oracle.query(_box_tuple(box), p, q, dict(r_max=float(np.sqrt(8))))
+    c = oracle.pmft3_accumulate("xyt", nl, (2, 2), (4, 4, 4), p_theta=[angle], q_theta=[query_angle])
+    r = p[0].astype(np.float64) - q[0]
+    rr = _quat_rotate64(np.conj(_quat_axis_angle([0, 0, 1], query_angle) * [1, -1, -1, -1]), r)[0]
+    xy = np.floor((rr[:2] + 2) * 4 / 4).astype(int)
+    tb = int(np.floor(((angle - np.arctan2(-r[1], -r[0])) % _TWO_PI) * 4 / _TWO_PI))
+    assert tuple(np.asarray(np.where(c)).flatten()) == (xy[0], xy[1], tb)
+    assert c.sum() == 1
+
+
+def test_pmftxyz_known_answers(oracle):
+    # test_pmft.py:729-779: every query point finds the origin as its nearest neighbour
+    box = (8, 8, 8, 0, 0, 0, False)
+    points = np.array([[0, 0, 0]], np.float32)
+    qp = np.array([[1.1, 0, 0], [-1.2, 0, 0], [0, 1.3, 0], [0, -1.4, 0], [0, 0, 1.5], [0, 0, -1.6]], np.float32)
+    nl = oracle.query(box, points, qp, dict(mode="nearest", num_neighbors=1))
+    c = oracle.pmft3_accumulate("xyz", nl, (3, 3, 3), (3, 3, 3), q_quat=np.tile([1, 0, 0, 0], (6, 1)))
+    npt.assert_array_equal(c, [[[0, 0, 0], [0, 1, 0], [0, 0, 0]], [[0, 1, 0], [1, 0, 1], [0, 1, 0]],
+                               [[0, 0, 0], [0, 1, 0], [0, 0, 0]]])
+    nl = oracle.query(box, qp, points, dict(mode="nearest", num_neighbors=1))
+    c = oracle.pmft3_accumulate("xyz", nl, (3, 3, 3), (3, 3, 3), q_quat=[[1, 0, 0, 0]])
+    assert c[2, 1, 1] == 1 and c.sum() == 1
+    # test_pmft.py:781-857: the query point's orientation applies (bins (4, 6, 6), (7, 5, 6))
+    p1 = np.array([[0.1, 0.1, 0]], np.float32)
+    p2 = np.array([[1, 0, 0]], np.float32)
+    for quat in (_quat_axis_angle([0, 0, 1], 0.0), _quat_axis_angle([0, 0, 1], np.deg2rad(45))):
+        quat = quat.astype(np.float32)
+        for pa, pb in ((p1, p2), (p2, p1)):
+            nl = oracle.query(box, pa, pb, dict(mode="nearest", num_neighbors=1))
+            c = oracle.pmft3_accumulate("xyz", nl, (3, 3, 3), (12, 12, 12), q_quat=quat)
+            bv = _quat_rotate64(quat * [1, -1, -1, -1], pa.astype(np.float64) - pb)
+            want = ((bv + 3) * 4 / 2).astype(int).squeeze()
+            npt.assert_array_equal(np.asarray(np.where(c)).squeeze(), want)
+    nl = oracle.query(box, p1, p2, dict(mode="nearest", num_neighbors=1))
+    c = oracle.pmft3_accumulate("xyz", nl, (3, 3, 3), (12, 12, 12), q_quat=[[1, 0, 0, 0]])
+    npt.assert_array_equal(np.asarray(np.where(c)).squeeze(), (4, 6, 6))
+
+
+def test_pmft3_compare_with_xy(oracle):
+    # test_pmft.py:991-1052: XY counts equal XYZ's middle z plane and XYT with one t bin
+    rng = np.random.default_rng(0)
+    pts = rng.random((100, 3)).astype(np.float32)
+    pts[:, 2] = 0
+    nl2 = oracle.query((10, 10, 0, 0, 0, 0, True), pts, pts, dict(r_max=float(np.sqrt(2 * 2.5**2)), exclude_ii=True))
+    xy = oracle.pmftxy_accumulate(nl2, np.zeros(100, np.float32), 2.5, 2.5, 4, 4)
+    nl3 = oracle.query((10, 10, 10, 0, 0, 0, False), pts, pts,
+                       dict(r_max=float(np.sqrt(2 * 2.5**2 + 1)), exclude_ii=True))
+    xyz = oracle.pmft3_accumulate("xyz", nl3, (2.5, 2.5, 1), (4, 4, 4), q_quat=np.tile([1, 0, 0, 0], (100, 1)))
+    npt.assert_array_equal(xy.reshape(4, 4), xyz[:, :, 2])
+    xyt = oracle.pmft3_accumulate("xyt", nl2, (2.5, 2.5), (4, 4, 1), p_theta=np.zeros(100), q_theta=np.zeros(100))
+    npt.assert_array_equal(xy.reshape(4, 4), xyt.reshape(4, 4))
+    pxy = oracle.pmftxy_reduce((10, 10, 0, 0, 0, 0, True), xy, 2.5, 2.5, 100, 100, 1)
+    pxyt = oracle.pmft3_reduce("xyt", (10, 10, 0, 0, 0, 0, True), xyt, (2.5, 2.5), 100, 100, 1)
+    npt.assert_allclose(pxy.reshape(4, 4), pxyt.reshape(4, 4), rtol=1e-6)
+    pxyz = oracle.pmft3_reduce("xyz", (10, 10, 10, 0, 0, 0, False), xyz, (2.5, 2.5, 1), 100, 100, 1)
+    # exp(pmft) = 1 / pcf; the XYZ pcf carries the extra z bins and the box length
+    npt.assert_allclose(pxy.reshape(4, 4), pxyz[:, :, 2] / (2 * 10), rtol=1e-5)
+
+
+@pytest.mark.parametrize("kind", ["xyt", "r12", "xyz"])
+def test_pmft3_brute_force_and_ideal_gas(oracle, kind):
+    """Random systems with random orientations: every bond's bins in float64
+    equal the oracle's away from bin edges, and the ideal gas pcf averages to 1."""
+    is2D = kind != "xyz"
+    box, pts = data.make_random_system(10, 1000, is2D=is2D, seed=3)
+    bt = _box_tuple(box)
+    rng = np.random.default_rng(5)
+    theta = (rng.random(len(pts)) * 2 * np.pi).astype(np.float32)
+    quat = rng.normal(size=(len(pts), 4))
+    quat = (quat / np.linalg.norm(quat, axis=1, keepdims=True)).astype(np.float32)
+    if kind == "r12":
+        maxes, bins = (3.0,), (30, 24, 18)
+    elif kind == "xyt":
+        maxes, bins = (2.1, 2.1), (30, 28, 16)
+    else:
+        maxes, bins = (1.7, 1.7, 1.7), (20, 22, 24)
+    r_max = float(np.linalg.norm(maxes))
+    nl = oracle.query(bt, pts, pts, dict(r_max=r_max, exclude_ii=True), nthreads=8)
+    c = oracle.pmft3_accumulate(kind, nl, maxes, bins, p_theta=theta, q_theta=theta, q_quat=quat)
+    i, j = nl["ij"][:, 0], nl["ij"][:, 1]
+    v = nl["vectors"].astype(np.float64)
+    if kind == "r12":
+        lo, hi = np.zeros(3), np.array([3.0, _TWO_PI, _TWO_PI])
+        coords = np.stack([np.linalg.norm(v, axis=1), (theta[j] - np.arctan2(v[:, 1], v[:, 0])) % _TWO_PI,
+                           (theta[i] - np.arctan2(-v[:, 1], -v[:, 0])) % _TWO_PI], 1)
+    elif kind == "xyt":
+        lo, hi = np.array([-2.1, -2.1, 0]), np.array([2.1, 2.1, _TWO_PI])
+        th = theta[i].astype(np.float64)
+        x = np.cos(-th) * v[:, 0] - np.sin(-th) * v[:, 1]
+        y = np.sin(-th) * v[:, 0] + np.cos(-th) * v[:, 1]
+        coords = np.stack([x, y, (theta[j] - np.arctan2(-v[:, 1], -v[:, 0])) % _TWO_PI], 1)
+    else:
+        lo, hi = -np.array(maxes), np.array(maxes)
+        coords = _quat_rotate64(quat[i].astype(np.float64) * [1, -1, -1, -1], v)
+    t = (coords - lo) * np.array(bins) / (hi - lo)
+    inside = np.all((t >= 0) & (t < np.array(bins)), axis=1)
+    # bonds within 2e-5 bin widths of an edge (or of the histogram's bounds)
+    # may land on either side in float32 (its rounding is below 2e-6 here)
+    away = np.all(np.abs(t - np.round(t)) > 2e-5, axis=1)
+    b = np.floor(t).astype(np.int64)
+    ref = np.zeros(bins, np.int64)
+    np.add.at(ref, tuple(b[inside & away].T), 1)
+    near = np.zeros(bins, bool)
+    risky = ~away & np.all((t > -1) & (t < np.array(bins) + 1), axis=1)
+    for off in np.ndindex(3, 3, 3):
+        bb = np.clip(b[risky] + np.array(off) - 1, 0, np.array(bins) - 1)
+        near[tuple(bb.T)] = True
+    sel = ~near
+    assert sel.sum() > 0.5 * sel.size
+    npt.assert_array_equal(c[sel], ref[sel])
+    assert abs(int(c.sum()) - int(inside.sum())) <= int(risky.sum())
+    pcf = oracle.pmft3_reduce(kind, bt, c, maxes, len(pts), len(pts), 1)
+    centers = [(np.linspace(l0, h0, n + 1)[:-1] + np.linspace(l0, h0, n + 1)[1:]) / 2
+               for l0, h0, n in zip(lo, hi, bins)]
+    if kind == "r12":
+        keep = np.broadcast_to((centers[0] < 0.95 * 3.0)[:, None, None], bins)
+    elif kind == "xyt":
+        rr = np.sqrt(centers[0][:, None] ** 2 + centers[1][None, :] ** 2)
+        keep = np.broadcast_to((rr < 0.95 * 2.1)[:, :, None], bins)
+    else:
+        rr = np.sqrt(centers[0][:, None, None] ** 2 + centers[1][None, :, None] ** 2 + centers[2][None, None, :] ** 2)
+        keep = rr < 0.95 * 1.7
+    assert abs(pcf[keep].mean() - 1.0) < 0.06
