@@ -1,4 +1,5 @@
-// fa_density.hip -- LocalDensity (freud/density/LocalDensity.cc) and the PMFT
+// fa_density.hip -- LocalDensity (freud/density/LocalDensity.cc),
+// CorrelationFunction (freud/density/CorrelationFunction.cc) and the PMFT
 // family (freud/pmft/PMFTXY.cc, PMFTXYT.cc, PMFTR12.cc, PMFTXYZ.cc, PMFT.h)
 // over a NeighborList: the other consumers of the query path beside the RDF.
 //
@@ -252,6 +253,73 @@ __global__ __launch_bounds__(kBlock) void pmft3_kernel(const uint32_t* __restric
         for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
             if (s_hist3[t])
                 atomicAdd(&counts[t], s_hist3[t]);
+    }
+}
+
+// ---------------------------------------------------------------- CorrelationFunction
+// CorrelationFunction.cc:84-98: per bond, the distance bin (RegularAxis(bins,
+// 0, r_max)) gets one count and conj(values[j]) * query_values[i] in complex
+// double.  Block-local histograms in LDS up to kMaxLdsCorrBins bins, merged
+// with device atomics; the sum order over bonds is unspecified, as the
+// reference's thread-local sums are.
+constexpr uint32_t kMaxLdsCorrBins = 2048;
+
+template<bool LDS>
+__global__ __launch_bounds__(kBlock) void correlation_kernel(const uint32_t* __restrict__ ij,
+                                                             const float* __restrict__ dist, uint64_t nb,
+                                                             const double2* __restrict__ values,
+                                                             const double2* __restrict__ qvalues, uint32_t nbins,
+                                                             float r_max, float inv_width,
+                                                             uint32_t* __restrict__ counts,
+                                                             double* __restrict__ sums)
+{
+    extern __shared__ double s_corr[];
+    double* s_sum = s_corr;                                          // 2 * nbins
+    uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_corr + 2 * nbins); // nbins
+    if (LDS)
+    {
+        for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+        {
+            s_sum[2 * t] = 0.0;
+            s_sum[2 * t + 1] = 0.0;
+            s_cnt[t] = 0u;
+        }
+        __syncthreads();
+    }
+    uint64_t const stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t b = uint64_t(fa::xcd_tile(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x; b < nb; b += stride)
+    {
+        uint32_t const bin = axis_bin(dist[b], 0.0f, r_max, inv_width, nbins);
+        if (bin == kOverflow)
+            continue;
+        uint32_t const i = ij[2 * b], j = ij[2 * b + 1];
+        double2 const x = values[j], y = qvalues[i];
+        // conj(x) * y = (x.re y.re + x.im y.im, x.re y.im - x.im y.re)
+        double const re = x.x * y.x + x.y * y.y;
+        double const im = x.x * y.y - x.y * y.x;
+        if (LDS)
+        {
+            atomicAdd(&s_cnt[bin], 1u);
+            atomicAdd(&s_sum[2 * bin], re);
+            atomicAdd(&s_sum[2 * bin + 1], im);
+        }
+        else
+        {
+            atomicAdd(&counts[bin], 1u);
+            atomicAdd(&sums[2 * bin], re);
+            atomicAdd(&sums[2 * bin + 1], im);
+        }
+    }
+    if (LDS)
+    {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+            if (s_cnt[t])
+            {
+                atomicAdd(&counts[t], s_cnt[t]);
+                atomicAdd(&sums[2 * t], s_sum[2 * t]);
+                atomicAdd(&sums[2 * t + 1], s_sum[2 * t + 1]);
+            }
     }
 }
 
@@ -525,6 +593,58 @@ int fa_pmftxyz_accumulate(const fa_nlist* nl, const float* query_orientations, c
     set_axis(A, 2, n_z, -z_max, z_max);
     return launch_pmft3<PMFT_XYZ>(nl, A, nullptr, nullptr, nullptr, static_cast<const float4*>(dev[0]),
                                   static_cast<const float4*>(dev[1]), n_equiv, d_counts);
+}
+
+int fa_correlation_accumulate(const fa_nlist* nl, const double* values, const double* query_values, uint32_t bins,
+                              float r_max, uint32_t* d_counts, double* d_sums)
+{
+    if (int rc = fa_init())
+        return rc;
+    // CorrelationFunction.cc:27-35
+    if (bins == 0)
+        return fa::fail(FA_ERR_INVALID, "CorrelationFunction  requires a nonzero number of bins.");
+    if (r_max <= 0)
+        return fa::fail(FA_ERR_INVALID, "CorrelationFunction requires r_max to be positive.");
+    uint32_t const nq = nl->num_query_points, np = nl->num_points;
+    uint64_t const nb = nl->num_bonds;
+    if (nb == 0 || nq == 0)
+        return FA_OK;
+    const void* host[2] = {values, query_values};
+    size_t const bytes[2] = {2 * sizeof(double) * size_t(np), 2 * sizeof(double) * size_t(nq)};
+    void* dev[2];
+    if (int rc = upload_arrays(host, bytes, 2, dev))
+        return rc;
+    volatile float w = (r_max - 0.0f) / float(bins);
+    float const inv_width = 1.0f / w;
+    hipStream_t s = fa::stream();
+    if (bins <= kMaxLdsCorrBins)
+        FA_KLAUNCH(correlation_kernel<true>, dim3(grid_for(nb, 2048)), dim3(kBlock),
+                   (2 * sizeof(double) + sizeof(uint32_t)) * size_t(bins), s, nl->neighbors, nl->distances, nb,
+                   static_cast<const double2*>(dev[0]), static_cast<const double2*>(dev[1]), bins, r_max, inv_width,
+                   d_counts, d_sums);
+    else
+        FA_KLAUNCH(correlation_kernel<false>, dim3(grid_for(nb, 4096)), dim3(kBlock), 0, s, nl->neighbors,
+                   nl->distances, nb, static_cast<const double2*>(dev[0]), static_cast<const double2*>(dev[1]), bins,
+                   r_max, inv_width, d_counts, d_sums);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int fa_correlation_reduce(const uint32_t* counts, const double* sums, uint32_t bins, double* correlation)
+{
+    // CorrelationFunction.cc:50-60: the complex sum divided by the bin's count
+    for (uint32_t t = 0; t < bins; ++t)
+    {
+        double re = sums[2 * t], im = sums[2 * t + 1];
+        if (counts[t] != 0)
+        {
+            re /= double(counts[t]);
+            im /= double(counts[t]);
+        }
+        correlation[2 * t] = re;
+        correlation[2 * t + 1] = im;
+    }
+    return FA_OK;
 }
 
 int fa_pmft_reduce(int kind, const fa_box* box, const uint32_t* counts, const uint32_t* nbins, const float* maxes,

@@ -1,6 +1,7 @@
-"""Radial distribution function and local density: the APIs of
-``freud.density.RDF`` (freud/density.py:535-696) and
-``freud.density.LocalDensity`` (freud/density.py:418-515) on the GPU.
+"""Radial distribution function, local density and the pair correlation of
+per-point values: the APIs of ``freud.density.RDF`` (freud/density.py:535-696),
+``freud.density.LocalDensity`` (freud/density.py:418-515) and
+``freud.density.CorrelationFunction`` (freud/density.py:28-190) on the GPU.
 
 Without a NeighborList the bins are filled by the fused query + histogram
 kernel (``fa_rdf_accumulate_query``); a given NeighborList is binned from its
@@ -15,7 +16,7 @@ import numpy as np
 from freud_amd import _lib
 from freud_amd._lib import DeviceArray, check, lib
 from freud_amd.locality import _PairCompute, _SpatialHistogram1D
-from freud_amd.util import _Compute, computed_property
+from freud_amd.util import _Compute, _convert_array, computed_property
 
 
 class RDF(_SpatialHistogram1D):
@@ -206,3 +207,103 @@ class LocalDensity(_PairCompute):
 
     def __repr__(self):
         return f"freud_amd.density.LocalDensity(r_max={self.r_max}, diameter={self.diameter})"
+
+
+class CorrelationFunction(_SpatialHistogram1D):
+    """Average product conj(s_1(point)) s_2(query point) of per-point values
+    over bonds at distance r (freud/density.py:28-190,
+    freud/density/CorrelationFunction.cc).  Counts and complex double sums
+    accumulate on the device across frames (``reset=False``)."""
+
+    def __init__(self, bins, r_max):
+        # CorrelationFunction.cc:27-35
+        if bins == 0:
+            msg = "CorrelationFunction  requires a nonzero number of bins."
+            raise ValueError(msg)
+        if r_max <= 0:
+            msg = "CorrelationFunction requires r_max to be positive."
+            raise ValueError(msg)
+        self._bins = int(bins)
+        self._r_max32 = np.float32(r_max)
+        self.r_max = r_max
+        self.is_complex = False
+        self._counts = None
+        self._sums = None
+
+    def _reset(self):
+        if self._counts is None:
+            self._counts = DeviceArray.zeros(self._bins, np.uint32)
+            self._sums = DeviceArray.zeros(2 * self._bins, np.float64)
+        else:
+            self._counts.zero()
+            self._sums.zero()
+        self._reduced = None
+
+    def compute(self, system, values, query_points=None, query_values=None, neighbors=None, reset=True):
+        if reset or self._counts is None:
+            self.is_complex = False
+            self._reset()
+        nq, nlist, qargs, d_q, num_query_points = self._preprocess_arguments(system, query_points, neighbors)
+        # freud/density.py:109-128
+        self.is_complex = bool(self.is_complex or np.any(np.iscomplex(values)) or np.any(np.iscomplex(query_values)))
+        values = _convert_array(values, shape=(nq._n,), dtype=np.complex128)
+        if query_values is None:
+            query_values = values
+        else:
+            query_values = _convert_array(query_values, shape=(num_query_points,), dtype=np.complex128)
+        nlist = self._neighbor_list(nq, nlist, qargs, d_q, num_query_points)
+        check(lib().fa_correlation_accumulate(ctypes.byref(nlist._c), values.ctypes.data, query_values.ctypes.data,
+                                              self._bins, float(self._r_max32), self._counts.ptr, self._sums.ptr))
+        self._box = nq._box
+        self._reduced = None
+        return self
+
+    def _reduce(self):
+        if self._reduced is None:
+            counts = self._counts.numpy()
+            sums = self._sums.numpy()
+            out = np.empty(2 * self._bins, np.float64)
+            check(lib().fa_correlation_reduce(counts.ctypes.data, sums.ctypes.data, self._bins, out.ctypes.data))
+            self._reduced = (counts, out.view(np.complex128))
+        return self._reduced
+
+    @computed_property
+    def correlation(self):
+        out = self._reduce()[1].copy()
+        return out if self.is_complex else np.real(out)
+
+    @computed_property
+    def bin_counts(self):
+        return self._reduce()[0].copy()
+
+    @computed_property
+    def box(self):
+        import freud_amd.box
+
+        return freud_amd.box.Box.from_box(self._box)
+
+    @property
+    def bin_edges(self):
+        """RegularAxis(bins, 0, r_max) edges (Histogram.h:126-138)."""
+        width = (self._r_max32 - np.float32(0)) / np.float32(self._bins)
+        return (np.float32(0) + np.arange(self._bins + 1, dtype=np.float32) * width).astype(np.float32)
+
+    @property
+    def bin_centers(self):
+        e = self.bin_edges
+        return ((e[:-1] + e[1:]) / np.float32(2.0)).astype(np.float32)
+
+    @property
+    def bounds(self):
+        return (0.0, float(self._r_max32))
+
+    @property
+    def nbins(self):
+        return self._bins
+
+    @property
+    def bins(self):
+        return self._bins
+
+    def __repr__(self):
+        return f"freud_amd.density.CorrelationFunction(bins={self.nbins}, r_max={self.r_max})"

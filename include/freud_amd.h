@@ -284,6 +284,18 @@ int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, fl
 int fa_pmftxy_reduce(const fa_box* box, const uint32_t* counts, uint32_t n_x, uint32_t n_y, float x_max, float y_max,
                      uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, float* pcf);
 
+/* CorrelationFunction::accumulate (freud/density/CorrelationFunction.cc:84-98;
+ * binding export-CorrelationFunction.cc): per bond of nl with distance in
+ * [0, r_max) (RegularAxis(bins, 0, r_max)), d_counts[bin] += 1 and
+ * d_sums[bin] += conj(values[j]) * query_values[i].  values / query_values:
+ * host complex128 (re, im interleaved), num_points / num_query_points of them;
+ * d_sums: 2 * bins doubles (device).  The order of the sums is unspecified,
+ * as in the reference's thread-local histograms. */
+int fa_correlation_accumulate(const fa_nlist* nl, const double* values, const double* query_values, uint32_t bins,
+                              float r_max, uint32_t* d_counts, double* d_sums);
+/* CorrelationFunction::reduce (CorrelationFunction.cc:50-60): host arrays;
+ * correlation (2 * bins doubles) = sums / counts where counts != 0. */
+int fa_correlation_reduce(const uint32_t* counts, const double* sums, uint32_t bins, double* correlation);
 /* PMFTXYT::accumulate (freud/pmft/PMFTXYT.cc:76-100): per bond, (x, y)
  * rotated by minus the query point's angle (host cos / sin, as PMFTXY) and
  * t = modulusPositive(theta_j - atan2(-y, -x), 2 pi), binned into

@@ -21,6 +21,7 @@ __all__ = [
     "pmftxy_accumulate",
     "pmftxy_reduce",
     "pmft3_accumulate",
+    "correlation_accumulate",
     "pmft3_reduce",
     "box_op",
     "nearest_plane_distance",
@@ -458,3 +459,22 @@ def pmft3_reduce(kind, box, counts, maxes, n_points, n_query_points, n_frames, n
                          ctypes.c_uint32(n_query_points), ctypes.c_uint32(n_frames), ctypes.c_uint32(n_equiv),
                          _p(pcf))
     return pcf.reshape(c.shape)
+
+
+def correlation_accumulate(nl, values, query_values, bins, r_max):
+    """CorrelationFunction (CorrelationFunction.cc:50-98) over a NeighborList
+    dict: (counts, correlation) with correlation = sums / counts."""
+    lib = _load()
+    ij = np.ascontiguousarray(nl["ij"], np.uint32)
+    dist = _f32(nl["distances"])
+    v = np.ascontiguousarray(values, np.complex128)
+    q = np.ascontiguousarray(query_values, np.complex128)
+    counts = np.zeros(bins, np.uint32)
+    sums = np.zeros(2 * bins, np.float64)
+    lib.orc_correlation_accumulate(_p(ij), _p(dist), ctypes.c_uint64(len(dist)), _p(v), _p(q), ctypes.c_uint32(bins),
+                                   ctypes.c_float(r_max), _p(counts), _p(sums))
+    # complex<double> /= count divides each part by the count (a scalar)
+    nz = counts != 0
+    s2 = sums.reshape(-1, 2).copy()
+    s2[nz] = s2[nz] / counts[nz, None].astype(np.float64)
+    return counts, s2.reshape(-1).view(np.complex128)

@@ -1841,6 +1841,27 @@ void orc_pmftxy_reduce(const float* boxp, int is2D, const uint32_t* counts, uint
         pcf[t] = static_cast<float>(counts[t]) * prefactor * jacobian_factor;
 }
 
+// CorrelationFunction.cc:84-98 in bond order: counts and the complex double
+// sums of conj(values[j]) * query_values[i] per distance bin.
+void orc_correlation_accumulate(const uint32_t* ij, const float* dist, uint64_t nb, const double* values,
+                                const double* qvalues, uint32_t bins, float r_max, uint32_t* counts, double* sums)
+{
+    float const w = (r_max - 0.0f) / static_cast<float>(bins);
+    float const inv = static_cast<float>(1.0) / w;
+    for (uint64_t e = 0; e < nb; ++e)
+    {
+        long const b = regular_bin(dist[e], 0.0f, r_max, inv, bins);
+        if (b < 0)
+            continue;
+        std::complex<double> const x(values[2 * ij[2 * e + 1]], values[2 * ij[2 * e + 1] + 1]);
+        std::complex<double> const y(qvalues[2 * ij[2 * e]], qvalues[2 * ij[2 * e] + 1]);
+        std::complex<double> const p = std::conj(x) * y;
+        counts[b] += 1;
+        sums[2 * b] += p.real();
+        sums[2 * b + 1] += p.imag();
+    }
+}
+
 // PMFTXYT.cc:76-100, PMFTR12.cc:90-111, PMFTXYZ.cc:106-145 (kind 0, 1, 2).
 // n, lo, hi: the three RegularAxis(n, lo, hi).  XYT and R12 take angles
 // (p_theta per point, q_theta per query point); XYZ takes query quaternions

@@ -287,3 +287,56 @@ def test_pmftxyz_known_answers():
     npt.assert_array_equal(np.asarray(np.where(pm.bin_counts)).squeeze(), (4, 6, 6))
     pm.compute((box, p2), _quat_axis_angle([0, 0, 1], 0.0), p1, neighbors={"mode": "nearest", "num_neighbors": 1})
     npt.assert_array_equal(np.asarray(np.where(pm.bin_counts)).squeeze(), (7, 5, 6))
+
+
+# ------------------------------------------------------------------ CorrelationFunction
+def test_correlation_known_answers():
+    # test_density_correlation_function.py:247-377 of the reference
+    box = Box.cube(8)
+    points = np.array([[0, 0, 0]], np.float32)
+    qp = np.array([[0.4, 0, 0], [0.9, 0, 0], [0, 1.4, 0], [0, 1.9, 0], [0, 0, 2.4], [0, 0, 2.9]], np.float32)
+    nn = {"mode": "nearest", "num_neighbors": 1}
+    cf = freud_amd.density.CorrelationFunction(3, 3)
+    with pytest.raises(AttributeError):
+        cf.correlation
+    cf.compute((box, points), np.ones(1), qp, np.ones(6), neighbors=nn)
+    npt.assert_array_equal(cf.correlation, [1, 1, 1])
+    npt.assert_array_equal(cf.bin_counts, [2, 2, 2])
+    assert not np.iscomplexobj(cf.correlation)
+    cf.compute((box, qp), np.ones(6), points, np.ones(1), neighbors=nn)
+    npt.assert_array_equal(cf.correlation, [1, 0, 0])
+    npt.assert_array_equal(cf.bin_counts, [1, 0, 0])
+    values = [1 + 1j]
+    qv = [1 + 1j, 1 + 1j, 2 + 2j, 2 + 2j, 3 + 3j, 3 + 3j]
+    cf.compute((box, points), values, qp, qv, neighbors=nn)
+    npt.assert_array_equal(cf.correlation, [2, 4, 6])
+    cf.compute((box, points), np.conj(values), qp, qv, neighbors=nn)
+    npt.assert_array_equal(cf.correlation, [2j, 4j, 6j])
+    cf.compute((box, qp), np.conj(qv), points, values, neighbors=nn)
+    npt.assert_array_equal(cf.correlation, [2j, 0, 0])
+    npt.assert_array_equal(cf.bin_counts, [1, 0, 0])
+    with pytest.raises(ValueError):
+        freud_amd.density.CorrelationFunction(0, 3)
+    with pytest.raises(ValueError):
+        freud_amd.density.CorrelationFunction(3, -1)
+
+
+@pytest.mark.parametrize("bins", [40, 3000])
+def test_correlation_matches_oracle(oracle, bins):
+    box, pts = freud_amd.data.make_random_system(31, 4000, is2D=True, seed=1)
+    pts = pts.astype(np.float32)
+    rng = np.random.default_rng(3)
+    v = np.exp(1j * rng.random(len(pts)) * 2 * np.pi) * (1 + rng.random(len(pts)))
+    cf = freud_amd.density.CorrelationFunction(bins, 10.0).compute((box, pts), v)
+    ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=10.0, exclude_ii=True), nthreads=8)
+    c, corr = oracle.correlation_accumulate(ref_nl, v, v, bins, 10.0)
+    npt.assert_array_equal(cf.bin_counts, c)
+    # complex double sums in another order than the oracle's bond order
+    npt.assert_allclose(cf.correlation, corr, rtol=1e-11, atol=1e-13)
+    assert np.iscomplexobj(cf.correlation)
+    # reset=False adds a second frame with other values
+    w = rng.random(len(pts)) - 0.5
+    cf.compute((box, pts), w, reset=False)
+    c2, _ = oracle.correlation_accumulate(ref_nl, w, w, bins, 10.0)
+    npt.assert_array_equal(cf.bin_counts, c + c2)
+    assert cf.box == box

@@ -1033,3 +1033,58 @@ def test_pmft3_brute_force_and_ideal_gas(oracle, kind):
         rr = np.sqrt(centers[0][:, None, None] ** 2 + centers[1][None, :, None] ** 2 + centers[2][None, None, :] ** 2)
         keep = rr < 0.95 * 1.7
     assert abs(pcf[keep].mean() - 1.0) < 0.06
+
+
+# --------------------------------------------------------------------- CorrelationFunction
+def test_correlation_known_answers(oracle):
+    # test_density_correlation_function.py:247-377 of the reference
+    box = (8, 8, 8, 0, 0, 0, False)
+    points = np.array([[0, 0, 0]], np.float32)
+    qp = np.array([[0.4, 0, 0], [0.9, 0, 0], [0, 1.4, 0], [0, 1.9, 0], [0, 0, 2.4], [0, 0, 2.9]], np.float32)
+    nn = dict(mode="nearest", num_neighbors=1)
+    nl = oracle.query(box, points, qp, nn)
+    c, corr = oracle.correlation_accumulate(nl, np.ones(1), np.ones(6), 3, 3.0)
+    npt.assert_array_equal(corr, [1, 1, 1])
+    npt.assert_array_equal(c, [2, 2, 2])
+    nl2 = oracle.query(box, qp, points, nn)
+    c, corr = oracle.correlation_accumulate(nl2, np.ones(6), np.ones(1), 3, 3.0)
+    npt.assert_array_equal(corr, [1, 0, 0])
+    npt.assert_array_equal(c, [1, 0, 0])
+    values = np.array([1 + 1j])
+    qv = np.array([1 + 1j, 1 + 1j, 2 + 2j, 2 + 2j, 3 + 3j, 3 + 3j])
+    _, corr = oracle.correlation_accumulate(nl, values, qv, 3, 3.0)
+    npt.assert_array_equal(corr, [2, 4, 6])
+    _, corr = oracle.correlation_accumulate(nl, np.conj(values), qv, 3, 3.0)
+    npt.assert_array_equal(corr, [2j, 4j, 6j])
+    _, corr = oracle.correlation_accumulate(nl2, qv, values, 3, 3.0)
+    npt.assert_array_equal(corr, [2, 0, 0])
+    _, corr = oracle.correlation_accumulate(nl2, np.conj(qv), values, 3, 3.0)
+    npt.assert_array_equal(corr, [2j, 0, 0])
+
+
+def test_correlation_brute_force(oracle):
+    # counts equal the minimum-image pair counts (test_density_correlation_function.py:162-190),
+    # sums a float64 numpy restatement in bond order; constant values correlate to 1 and
+    # random phases to about 0 (:65-145)
+    box, pts = data.make_random_system(31, 1000, is2D=True, seed=1)
+    bt = _box_tuple(box)
+    nl = oracle.query(bt, pts, pts, dict(r_max=10.0, exclude_ii=True), nthreads=8)
+    rng = np.random.default_rng(2)
+    v = np.exp(1j * rng.random(len(pts)) * 2 * np.pi) * (1 + rng.random(len(pts)))
+    c, corr = oracle.correlation_accumulate(nl, v, v, 10, 10.0)
+    d = pts[None, :, :] - pts[:, None, :]
+    dl = np.linalg.norm(box.wrap(d.reshape(-1, 3)), axis=1).reshape(len(pts), len(pts))
+    np.fill_diagonal(dl, np.inf)
+    assert c.sum() == np.sum(dl < 10.0)
+    i, j = nl["ij"][:, 0], nl["ij"][:, 1]
+    b = np.floor(nl["distances"].astype(np.float64) / 1.0).astype(int)
+    ref = np.zeros(10, np.complex128)
+    np.add.at(ref, b, np.conj(v[j]) * v[i])
+    cnt = np.bincount(b, minlength=10)
+    npt.assert_array_equal(c, cnt)
+    npt.assert_allclose(corr, (ref.real / cnt) + 1j * (ref.imag / cnt), rtol=1e-12)
+    _, one = oracle.correlation_accumulate(nl, np.ones(len(pts)), np.ones(len(pts)), 10, 10.0)
+    npt.assert_array_equal(one, 1)
+    _, zero = oracle.correlation_accumulate(nl, np.exp(1j * rng.random(len(pts)) * 2 * np.pi),
+                                            np.exp(1j * rng.random(len(pts)) * 2 * np.pi), 10, 10.0)
+    assert np.all(np.abs(zero[1:]) < 0.1)
