@@ -2,13 +2,14 @@
 SolidLiquid, Hexatic, Cluster) on one AMD Instinct MI355X.
 
 The Python API mirrors freud 3.5.0's ``freud.box``, ``freud.locality``,
-``freud.density``, ``freud.order``, ``freud.pmft`` (PMFTXY) and ``freud.cluster``
+``freud.density``, ``freud.environment`` (BondOrder), ``freud.order``, ``freud.pmft`` and ``freud.cluster``
 for this path; the work
 runs in hand-written HIP kernels (``freud_amd/csrc``) behind the C-ABI declared
 in ``include/freud_amd.h``.
 """
 
-from freud_amd import batch, box, cluster, data, dcd, density, gsd, learned, locality, order, parallel, pmft
+from freud_amd import (batch, box, cluster, data, dcd, density, environment, gsd, learned, locality, order,
+                       parallel, pmft)
 from freud_amd.box import Box
 from freud_amd.locality import AABBQuery, CellQuery, LinkCell, NeighborList
 
@@ -25,6 +26,7 @@ __all__ = [
     "cluster",
     "data",
     "density",
+    "environment",
     "learned",
     "locality",
     "order",

@@ -173,6 +173,10 @@ _SIGS = {
                                         ctypes.c_void_p, ctypes.c_void_p]),
     "fa_pmftxy_accumulate": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_float,
                                             ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_bond_order_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "fa_bond_order_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                            ctypes.c_void_p]),
     "fa_correlation_accumulate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                  ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
     "fa_correlation_reduce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),

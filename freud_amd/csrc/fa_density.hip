@@ -1,6 +1,6 @@
 // fa_density.hip -- LocalDensity (freud/density/LocalDensity.cc),
-// CorrelationFunction (freud/density/CorrelationFunction.cc) and the PMFT
-// family (freud/pmft/PMFTXY.cc, PMFTXYT.cc, PMFTR12.cc, PMFTXYZ.cc, PMFT.h)
+// CorrelationFunction (freud/density/CorrelationFunction.cc), BondOrder
+// (freud/environment/BondOrder.cc) and the PMFT family (freud/pmft/PMFTXY.cc, PMFTXYT.cc, PMFTR12.cc, PMFTXYZ.cc, PMFT.h)
 // over a NeighborList: the other consumers of the query path beside the RDF.
 //
 // LocalDensity is one pass over each query point's bond segment (the bonds of
@@ -323,6 +323,82 @@ __global__ __launch_bounds__(kBlock) void correlation_kernel(const uint32_t* __r
     }
 }
 
+// ---------------------------------------------------------------- BondOrder
+// BondOrder.cc:87-136: per bond, the (possibly rotated) direction's azimuth
+// theta = modulusPositive(atan2(y, x), 2 pi) and polar angle
+// phi = acos(z / |v|), binned into n_theta x n_phi over [0, 2 pi) x [0, pi).
+// Modes (BondOrder.h): 0 bod, 1 lbod, 2 obcd, 3 oocd.  ref_q is the point's
+// orientation, q the query point's.  acos, like atan2, runs in double and is
+// rounded once (the reference's come from the host C library).
+template<int MODE, bool LDS>
+__global__ __launch_bounds__(kBlock) void bond_order_kernel(const uint32_t* __restrict__ ij,
+                                                            const float* __restrict__ vec, uint64_t nb,
+                                                            const float4* __restrict__ p_quat,
+                                                            const float4* __restrict__ q_quat, Axis2 A,
+                                                            uint32_t* __restrict__ counts)
+{
+    extern __shared__ uint32_t s_hist_bo[];
+    uint32_t const nbins = A.nx * A.ny;
+    if (LDS)
+    {
+        for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+            s_hist_bo[t] = 0u;
+        __syncthreads();
+    }
+    uint64_t const stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t b = uint64_t(fa::xcd_tile(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x; b < nb; b += stride)
+    {
+        uint32_t const i = ij[2 * b], j = ij[2 * b + 1];
+        float vx = vec[3 * b], vy = vec[3 * b + 1], vz = vec[3 * b + 2];
+        if constexpr (MODE != 0)
+        {
+            float4 ref_c = p_quat[j];
+            ref_c.y = -ref_c.y;
+            ref_c.z = -ref_c.z;
+            ref_c.w = -ref_c.w;
+            float4 const q = q_quat[i];
+            float ox, oy, oz;
+            if constexpr (MODE == 2)
+            {
+                quat_rotate(ref_c, vx, vy, vz, ox, oy, oz);
+                quat_rotate(q, ox, oy, oz, vx, vy, vz);
+            }
+            else if constexpr (MODE == 1)
+            {
+                quat_rotate(ref_c, vx, vy, vz, ox, oy, oz);
+                vx = ox;
+                vy = oy;
+                vz = oz;
+            }
+            else
+            {
+                float zx, zy, zz;
+                quat_rotate(q, 0.0f, 0.0f, 1.0f, zx, zy, zz);
+                quat_rotate(ref_c, zx, zy, zz, vx, vy, vz);
+            }
+        }
+        float const theta = modulus_positive(angle_of(vy, vx), kTwoPi);
+        float const len = sqrtf((vx * vx + vy * vy) + vz * vz);
+        float const phi = float(acos(double(vz / len)));
+        uint32_t const bt = axis_bin(theta, A.xmin, A.xmax, A.xinv, A.nx);
+        uint32_t const bp = axis_bin(phi, A.ymin, A.ymax, A.yinv, A.ny);
+        if (bt == kOverflow || bp == kOverflow)
+            continue;
+        uint32_t const bin = bt * A.ny + bp;
+        if (LDS)
+            atomicAdd(&s_hist_bo[bin], 1u);
+        else
+            atomicAdd(&counts[bin], 1u);
+    }
+    if (LDS)
+    {
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < nbins; t += blockDim.x)
+            if (s_hist_bo[t])
+                atomicAdd(&counts[t], s_hist_bo[t]);
+    }
+}
+
 // RegularAxis(n, min, max) (Histogram.h:126-138)
 void set_axis(Axis3& A, int k, uint32_t n, float mn, float mx)
 {
@@ -627,6 +703,89 @@ int fa_correlation_accumulate(const fa_nlist* nl, const double* values, const do
                    nl->distances, nb, static_cast<const double2*>(dev[0]), static_cast<const double2*>(dev[1]), bins,
                    r_max, inv_width, d_counts, d_sums);
     FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int fa_bond_order_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, int mode,
+                             uint32_t n_theta, uint32_t n_phi, uint32_t* d_counts)
+{
+    if (int rc = fa_init())
+        return rc;
+    // BondOrder.cc:16-24
+    if (n_theta < 2)
+        return fa::fail(FA_ERR_INVALID, "BondOrder requires at least 2 bins in theta.");
+    if (n_phi < 2)
+        return fa::fail(FA_ERR_INVALID, "BondOrder requires at least 2 bins in phi.");
+    if (mode < 0 || mode > 3)
+        return fa::fail(FA_ERR_INVALID, "Unknown BondOrder mode");
+    if (uint64_t(n_theta) * n_phi > 0xffffffffull)
+        return fa::fail(FA_ERR_INVALID, "BondOrder histogram too large");
+    uint32_t const nq = nl->num_query_points, np = nl->num_points;
+    uint64_t const nb = nl->num_bonds;
+    if (nb == 0 || nq == 0)
+        return FA_OK;
+    const void* host[2] = {orientations, query_orientations};
+    size_t const bytes[2] = {sizeof(float4) * size_t(np), sizeof(float4) * size_t(nq)};
+    void* dev[2];
+    if (int rc = upload_arrays(host, bytes, 2, dev))
+        return rc;
+    // RegularAxis(n_theta, 0, TWO_PI) and RegularAxis(n_phi, 0, M_PI) (float(M_PI))
+    Axis2 A;
+    A.xmin = 0.0f;
+    A.xmax = kTwoPi;
+    volatile float wt = (kTwoPi - 0.0f) / float(n_theta);
+    A.xinv = 1.0f / wt;
+    A.ymin = 0.0f;
+    A.ymax = float(M_PI);
+    volatile float wp = (float(M_PI) - 0.0f) / float(n_phi);
+    A.yinv = 1.0f / wp;
+    A.nx = n_theta;
+    A.ny = n_phi;
+    uint32_t const nbins = n_theta * n_phi;
+    auto const* pq = static_cast<const float4*>(dev[0]);
+    auto const* qq = static_cast<const float4*>(dev[1]);
+    hipStream_t s = fa::stream();
+#define FA_BO_LAUNCH(M)                                                                                        \
+    do                                                                                                         \
+    {                                                                                                          \
+        if (nbins <= kMaxLdsBins2D)                                                                            \
+            FA_KLAUNCH((bond_order_kernel<M, true>), dim3(grid_for(nb, 2048)), dim3(kBlock),                   \
+                       sizeof(uint32_t) * nbins, s, nl->neighbors, nl->vectors, nb, pq, qq, A, d_counts);      \
+        else                                                                                                   \
+            FA_KLAUNCH((bond_order_kernel<M, false>), dim3(grid_for(nb, 4096)), dim3(kBlock), 0, s,            \
+                       nl->neighbors, nl->vectors, nb, pq, qq, A, d_counts);                                   \
+    } while (0)
+    if (mode == 0)
+        FA_BO_LAUNCH(0);
+    else if (mode == 1)
+        FA_BO_LAUNCH(1);
+    else if (mode == 2)
+        FA_BO_LAUNCH(2);
+    else
+        FA_BO_LAUNCH(3);
+#undef FA_BO_LAUNCH
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int fa_bond_order_reduce(const uint32_t* counts, uint32_t n_theta, uint32_t n_phi, uint32_t n_frames, float* bond_order)
+{
+    // BondOrder.cc:26-53 (surface areas with the host C library's cos) and :68-74
+    volatile float dt = kTwoPi / float(n_theta);
+    volatile float dp = M_PI / float(n_phi);
+    for (uint32_t t = 0; t < n_theta; ++t)
+        for (uint32_t p = 0; p < n_phi; ++p)
+        {
+            volatile float phi = float(p) * dp;
+            volatile float phi2 = phi + dp;
+            volatile float c1 = std::cos(float(phi));
+            volatile float c2 = std::cos(float(phi2));
+            volatile float dc = c1 - c2;
+            volatile float sa = dt * dc;
+            uint64_t const k = uint64_t(t) * n_phi + p;
+            volatile float a = float(counts[k]) / sa;
+            bond_order[k] = a / float(n_frames);
+        }
     return FA_OK;
 }
 

@@ -284,6 +284,21 @@ int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, fl
 int fa_pmftxy_reduce(const fa_box* box, const uint32_t* counts, uint32_t n_x, uint32_t n_y, float x_max, float y_max,
                      uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, float* pcf);
 
+/* BondOrder::accumulate (freud/environment/BondOrder.cc:87-136; binding
+ * export-BondOrder.cc): per bond the direction (mode 0 bod: the bond vector;
+ * 1 lbod: rotated by conj(q_j); 2 obcd: by conj(q_j) then q_i; 3 oocd: the
+ * query point's director q_i z rotated by conj(q_j)) binned by azimuth
+ * theta in [0, 2 pi) and polar angle phi in [0, pi) into n_theta x n_phi
+ * (row-major).  orientations: host (num_points, 4) quaternions (s, x, y, z);
+ * query_orientations: host (num_query_points, 4).  The angles' atan2 and acos
+ * run in double rounded to float (the reference's come from the host C
+ * library). */
+int fa_bond_order_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, int mode,
+                             uint32_t n_theta, uint32_t n_phi, uint32_t* d_counts);
+/* BondOrder::reduce (BondOrder.cc:26-53, 68-74): bond_order (host) =
+ * count / surface area / n_frames, surface areas dt (cos phi - cos(phi + dp))
+ * with the host C library's cos, in the reference's float order. */
+int fa_bond_order_reduce(const uint32_t* counts, uint32_t n_theta, uint32_t n_phi, uint32_t n_frames, float* bond_order);
 /* CorrelationFunction::accumulate (freud/density/CorrelationFunction.cc:84-98;
  * binding export-CorrelationFunction.cc): per bond of nl with distance in
  * [0, r_max) (RegularAxis(bins, 0, r_max)), d_counts[bin] += 1 and

@@ -22,6 +22,7 @@ __all__ = [
     "pmftxy_reduce",
     "pmft3_accumulate",
     "correlation_accumulate",
+    "bond_order",
     "pmft3_reduce",
     "box_op",
     "nearest_plane_distance",
@@ -478,3 +479,24 @@ def correlation_accumulate(nl, values, query_values, bins, r_max):
     s2 = sums.reshape(-1, 2).copy()
     s2[nz] = s2[nz] / counts[nz, None].astype(np.float64)
     return counts, s2.reshape(-1).view(np.complex128)
+
+
+_BOND_ORDER_MODES = {"bod": 0, "lbod": 1, "obcd": 2, "oocd": 3}
+
+
+def bond_order(nl, p_quat, q_quat, mode, n_theta, n_phi, n_frames=1, counts=None):
+    """BondOrder (BondOrder.cc:26-136) over a NeighborList dict: (counts, bond_order)."""
+    lib = _load()
+    ij = np.ascontiguousarray(nl["ij"], np.uint32)
+    vec = _f32(nl["vectors"])
+    pq = _f32(p_quat)
+    qq = _f32(q_quat)
+    if counts is None:
+        counts = np.zeros(n_theta * n_phi, np.uint32)
+    lib.orc_bond_order_accumulate(_p(ij), _p(vec), ctypes.c_uint64(len(ij)), _p(pq), _p(qq),
+                                  _BOND_ORDER_MODES[mode], ctypes.c_uint32(n_theta), ctypes.c_uint32(n_phi),
+                                  _p(counts))
+    bo = np.empty(n_theta * n_phi, np.float32)
+    lib.orc_bond_order_reduce(_p(counts), ctypes.c_uint32(n_theta), ctypes.c_uint32(n_phi),
+                              ctypes.c_uint32(n_frames), _p(bo))
+    return counts.reshape(n_theta, n_phi), bo.reshape(n_theta, n_phi)

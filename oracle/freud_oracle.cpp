@@ -1841,6 +1841,80 @@ void orc_pmftxy_reduce(const float* boxp, int is2D, const uint32_t* counts, uint
         pcf[t] = static_cast<float>(counts[t]) * prefactor * jacobian_factor;
 }
 
+static float modulus_positive_f(float a, float b) { return std::fmod(std::fmod(a, b) + b, b); }
+
+struct quatf
+{
+    float s;
+    v3 v;
+};
+// VectorMath.h:810-822
+static v3 quat_rotate(const quatf& a, v3 b)
+{
+    float const c1 = a.s * a.s - dot(a.v, a.v);
+    v3 const cr = {a.v.y * b.z - a.v.z * b.y, a.v.z * b.x - a.v.x * b.z, a.v.x * b.y - a.v.y * b.x};
+    float const c2 = 2 * a.s;
+    float const c3 = 2 * dot(a.v, b);
+    v3 const t1 = {c1 * b.x, c1 * b.y, c1 * b.z};
+    v3 const t2 = {c2 * cr.x, c2 * cr.y, c2 * cr.z};
+    v3 const t3 = {c3 * a.v.x, c3 * a.v.y, c3 * a.v.z};
+    return add(add(t1, t2), t3);
+}
+
+// BondOrder.cc:87-136 (modes 0 bod, 1 lbod, 2 obcd, 3 oocd) and reduce
+// (BondOrder.cc:26-53, 68-74).  Quaternions (s, x, y, z).
+void orc_bond_order_accumulate(const uint32_t* ij, const float* vec, uint64_t nb, const float* p_quat,
+                               const float* q_quat, int mode, uint32_t n_theta, uint32_t n_phi, uint32_t* counts)
+{
+    float const two_pi = float(2.0 * M_PI);
+    float const pi_f = float(M_PI);
+    float const it = static_cast<float>(1.0) / ((two_pi - 0.0f) / static_cast<float>(n_theta));
+    float const ip = static_cast<float>(1.0) / ((pi_f - 0.0f) / static_cast<float>(n_phi));
+    for (uint64_t e = 0; e < nb; ++e)
+    {
+        uint32_t const i = ij[2 * e], j = ij[2 * e + 1];
+        quatf const ref_q = {p_quat[4 * j], {p_quat[4 * j + 1], p_quat[4 * j + 2], p_quat[4 * j + 3]}};
+        quatf const ref_c = {ref_q.s, {-ref_q.v.x, -ref_q.v.y, -ref_q.v.z}};
+        quatf const q = {q_quat[4 * i], {q_quat[4 * i + 1], q_quat[4 * i + 2], q_quat[4 * i + 3]}};
+        v3 v = {vec[3 * e], vec[3 * e + 1], vec[3 * e + 2]};
+        if (mode == 2)
+        {
+            v = quat_rotate(ref_c, v);
+            v = quat_rotate(q, v);
+        }
+        else if (mode == 1)
+            v = quat_rotate(ref_c, v);
+        else if (mode == 3)
+        {
+            v3 z = {0.0f, 0.0f, 1.0f};
+            z = quat_rotate(q, z);
+            v = quat_rotate(ref_c, z);
+        }
+        float theta = std::atan2(v.y, v.x);
+        theta = modulus_positive_f(theta, two_pi);
+        float const phi = std::acos(v.z / std::sqrt(dot(v, v)));
+        long const bt = regular_bin(theta, 0.0f, two_pi, it, n_theta);
+        long const bp = regular_bin(phi, 0.0f, pi_f, ip, n_phi);
+        if (bt < 0 || bp < 0)
+            continue;
+        counts[size_t(bt) * n_phi + size_t(bp)] += 1;
+    }
+}
+
+void orc_bond_order_reduce(const uint32_t* counts, uint32_t n_theta, uint32_t n_phi, uint32_t n_frames, float* bo)
+{
+    float const dt = float(2.0 * M_PI) / float(n_theta);
+    float const dp = M_PI / float(n_phi);
+    for (uint32_t t = 0; t < n_theta; ++t)
+        for (uint32_t p = 0; p < n_phi; ++p)
+        {
+            float const phi = (float) p * dp;
+            float const sa = dt * (std::cos(phi) - std::cos(phi + dp));
+            size_t const k = size_t(t) * n_phi + p;
+            bo[k] = float(counts[k]) / sa / static_cast<float>(n_frames);
+        }
+}
+
 // CorrelationFunction.cc:84-98 in bond order: counts and the complex double
 // sums of conj(values[j]) * query_values[i] per distance bin.
 void orc_correlation_accumulate(const uint32_t* ij, const float* dist, uint64_t nb, const double* values,
@@ -1866,26 +1940,6 @@ void orc_correlation_accumulate(const uint32_t* ij, const float* dist, uint64_t 
 // n, lo, hi: the three RegularAxis(n, lo, hi).  XYT and R12 take angles
 // (p_theta per point, q_theta per query point); XYZ takes query quaternions
 // (s, x, y, z) and n_equiv equivalent orientations.
-static float modulus_positive_f(float a, float b) { return std::fmod(std::fmod(a, b) + b, b); }
-
-struct quatf
-{
-    float s;
-    v3 v;
-};
-// VectorMath.h:810-822
-static v3 quat_rotate(const quatf& a, v3 b)
-{
-    float const c1 = a.s * a.s - dot(a.v, a.v);
-    v3 const cr = {a.v.y * b.z - a.v.z * b.y, a.v.z * b.x - a.v.x * b.z, a.v.x * b.y - a.v.y * b.x};
-    float const c2 = 2 * a.s;
-    float const c3 = 2 * dot(a.v, b);
-    v3 const t1 = {c1 * b.x, c1 * b.y, c1 * b.z};
-    v3 const t2 = {c2 * cr.x, c2 * cr.y, c2 * cr.z};
-    v3 const t3 = {c3 * a.v.x, c3 * a.v.y, c3 * a.v.z};
-    return add(add(t1, t2), t3);
-}
-
 void orc_pmft3_accumulate(int kind, const uint32_t* ij, const float* vec, const float* dist, uint64_t nb,
                           const float* p_theta, const float* q_theta, const float* q_quat, const float* equiv,
                           uint32_t n_equiv, const uint32_t* n, const float* lo, const float* hi, uint32_t* counts)

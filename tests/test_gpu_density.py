@@ -340,3 +340,85 @@ def test_correlation_matches_oracle(oracle, bins):
     c2, _ = oracle.correlation_accumulate(ref_nl, w, w, bins, 10.0)
     npt.assert_array_equal(cf.bin_counts, c + c2)
     assert cf.box == box
+
+
+# ------------------------------------------------------------------ BondOrder
+def _rand_quats(n, seed):
+    q = np.random.default_rng(seed).normal(size=(n, 4))
+    return (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+
+
+def test_bond_order_reference_cases():
+    # test_environment_bond_order.py:14-140 of the reference
+    box, pos = freud_amd.data.UnitCell.fcc().generate_system(4)
+    quats = np.tile(np.array([1, 0, 0, 0], np.float32), (len(pos), 1))
+    bo = freud_amd.environment.BondOrder(6)
+    with pytest.raises(AttributeError):
+        bo.bond_order
+    bo.compute((box, pos), quats, neighbors={"num_neighbors": 12, "r_max": 1.5})
+    op = bo.bond_order.copy()
+    assert np.sum(op > 0) == 12
+    assert bo.nbins == [6, 6] and bo.box == box
+    npt.assert_allclose(bo.bin_centers[0], (2 * np.arange(6) + 1) * np.pi / 6, atol=1e-6)
+    npt.assert_allclose(bo.bin_centers[1], (2 * np.arange(6) + 1) * np.pi / 12, atol=1e-6)
+    nn = {"mode": "nearest", "num_neighbors": 12, "r_guess": 1.5, "exclude_ii": True}
+    for mode in ("lbod", "obcd"):
+        b = freud_amd.environment.BondOrder(6, mode=mode)
+        b.compute((box, pos), quats, neighbors=nn, reset=False)
+        npt.assert_allclose(b.bond_order, op)
+    rq = _rand_quats(len(pos), 10893)
+    npt.assert_allclose(freud_amd.environment.BondOrder(6).compute((box, pos), rq, neighbors=nn).bond_order, op)
+    b = freud_amd.environment.BondOrder(6, mode="obcd").compute((box, pos), rq, neighbors=nn)
+    assert not np.allclose(b.bond_order, op) and np.sum(b.bond_order > 0) == b.bond_order.size
+    b = freud_amd.environment.BondOrder(6, mode="oocd").compute((box, pos), quats, neighbors=nn)
+    assert np.sum(b.bond_order > 0) == 1 and b.bond_order[0, 0] > 0
+    assert np.sum(freud_amd.environment.BondOrder(6, mode="oocd").compute(
+        (box, pos), rq, neighbors=nn).bond_order > 0) > 30
+    # points != query points
+    query_points, points = _alternating_lattice(10, np.pi / 30)
+    b = freud_amd.environment.BondOrder(bins=(35, 2))
+    b.compute((Box.square(50), points), np.tile([1.0, 0, 0, 0], (len(points), 1)), query_points,
+              np.tile([1.0, 0, 0, 0], (len(query_points), 1)),
+              neighbors={"mode": "nearest", "num_neighbors": 12, "r_guess": 1.6, "exclude_ii": False})
+    assert np.count_nonzero(b.bond_order) == 12
+    assert len(np.unique(b.bond_order)) == 2
+    with pytest.raises(NotImplementedError):
+        freud_amd.environment.BondOrder(6).compute((box, pos), quats)
+    with pytest.raises(ValueError):
+        freud_amd.environment.BondOrder(6, mode="nope")
+    with pytest.raises(ValueError):
+        freud_amd.environment.BondOrder(1)
+
+
+@pytest.mark.parametrize("mode", ["bod", "lbod", "obcd", "oocd"])
+def test_bond_order_matches_oracle(oracle, mode):
+    box, pts = freud_amd.data.make_random_system(12, 6000, seed=13)
+    pts = pts.astype(np.float32)
+    pq, qq = _rand_quats(len(pts), 1), _rand_quats(len(pts), 2)
+    bo = freud_amd.environment.BondOrder((30, 18), mode=mode)
+    qp = pts + np.float32(0.01)
+    bo.compute((box, pts), pq, qp, qq, neighbors={"r_max": 1.6})
+    ref_nl = oracle.query(_bt(box), pts, qp, dict(r_max=1.6), nthreads=8)
+    counts, ref_bo = oracle.bond_order(ref_nl, pq, qq, mode, 30, 18)
+    i, j = ref_nl["ij"][:, 0], ref_nl["ij"][:, 1]
+    v = ref_nl["vectors"].astype(np.float64)
+
+    def rot(q, x):
+        q = np.asarray(q, np.float64)
+        s, u = q[:, :1], q[:, 1:]
+        return (s * s - np.sum(u * u, 1, keepdims=True)) * x + 2 * s * np.cross(u, x) + 2 * np.sum(
+            u * x, 1, keepdims=True) * u
+
+    pc = pq[j].astype(np.float64) * [1, -1, -1, -1]
+    if mode == "lbod":
+        v = rot(pc, v)
+    elif mode == "obcd":
+        v = rot(qq[i], rot(pc, v))
+    elif mode == "oocd":
+        v = rot(pc, rot(qq[i], np.tile([0.0, 0, 1], (len(i), 1))))
+    theta = np.arctan2(v[:, 1], v[:, 0]) % _TWO_PI
+    phi = np.arccos(v[:, 2] / np.linalg.norm(v, axis=1))
+    d, r = _angle_parity(bo.bin_counts, counts, [(theta, 30), (2 * phi, 18)])
+    print(f"{mode}: {d} count differences, {r} bonds within 4 ulp of an angle edge", flush=True)
+    if d == 0:
+        npt.assert_array_equal(bo.bond_order, ref_bo)

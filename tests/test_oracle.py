@@ -1088,3 +1088,88 @@ def test_correlation_brute_force(oracle):
     _, zero = oracle.correlation_accumulate(nl, np.exp(1j * rng.random(len(pts)) * 2 * np.pi),
                                             np.exp(1j * rng.random(len(pts)) * 2 * np.pi), 10, 10.0)
     assert np.all(np.abs(zero[1:]) < 0.1)
+
+
+# --------------------------------------------------------------------- BondOrder
+def _random_quats(n, seed):
+    q = np.random.default_rng(seed).normal(size=(n, 4))
+    return (q / np.linalg.norm(q, axis=1, keepdims=True)).astype(np.float32)
+
+
+def test_bond_order_fcc(oracle):
+    # test_environment_bond_order.py:14-105 of the reference
+    box, pos = data.UnitCell.fcc().generate_system(4)
+    pos = pos.astype(np.float32)
+    bt = _box_tuple(box)
+    quats = np.tile(np.array([1, 0, 0, 0], np.float32), (len(pos), 1))
+    nl = oracle.query(bt, pos, pos, dict(num_neighbors=12, r_max=1.5, exclude_ii=True))
+    _, op = oracle.bond_order(nl, quats, quats, "bod", 6, 6)
+    assert np.sum(op > 0) == 12
+    for mode in ("lbod", "obcd"):
+        npt.assert_allclose(oracle.bond_order(nl, quats, quats, mode, 6, 6)[1], op)
+    rq = _random_quats(len(pos), 10893)
+    npt.assert_allclose(oracle.bond_order(nl, rq, rq, "bod", 6, 6)[1], op)
+    _, obcd = oracle.bond_order(nl, rq, rq, "obcd", 6, 6)
+    assert not np.allclose(obcd, op) and np.sum(obcd > 0) == obcd.size
+    _, oocd = oracle.bond_order(nl, quats, quats, "oocd", 6, 6)
+    assert np.sum(oocd > 0) == 1 and oocd[0, 0] > 0
+    assert np.sum(oracle.bond_order(nl, rq, rq, "oocd", 6, 6)[1] > 0) > 30
+
+
+def test_bond_order_points_ne_query_points(oracle):
+    # test_environment_bond_order.py:111-140: 12 occupied (theta, phi) bins, one value
+    size, angle = 10, np.pi / 30
+    c, s = np.cos(angle), np.sin(angle)
+    rot = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    p1, p2 = [], []
+    for i in range(-2, size + 2):
+        for j in range(-2, size + 2):
+            p = rot.dot(np.array([i, j, 0]))
+            if (i + j) % 2 == 0:
+                p2.append(p)
+            elif 0 <= i < size and 0 <= j < size:
+                p1.append(p)
+    query_points, points = np.array(p1, np.float32), np.array(p2, np.float32)
+    nl = oracle.query((50, 50, 0, 0, 0, 0, True), points, query_points,
+                      dict(mode="nearest", num_neighbors=12, r_guess=1.6, exclude_ii=False))
+    ident_p = np.tile(np.array([1, 0, 0, 0], np.float32), (len(points), 1))
+    ident_q = np.tile(np.array([1, 0, 0, 0], np.float32), (len(query_points), 1))
+    _, bo = oracle.bond_order(nl, ident_p, ident_q, "bod", 35, 2)
+    assert np.count_nonzero(bo) == 12
+    assert len(np.unique(bo)) == 2
+
+
+@pytest.mark.parametrize("mode", ["bod", "lbod", "obcd", "oocd"])
+def test_bond_order_brute_force(oracle, mode):
+    """float64 directions and angles give the same bins away from edges; the
+    bond order is count / (dt (cos phi - cos(phi + dp))) / frames."""
+    box, pts = data.make_random_system(8, 1500, seed=6)
+    bt = _box_tuple(box)
+    nl = oracle.query(bt, pts, pts, dict(r_max=1.6, exclude_ii=True), nthreads=8)
+    pq, qq = _random_quats(len(pts), 1), _random_quats(len(pts), 2)
+    c, bo = oracle.bond_order(nl, pq, qq, mode, 20, 12)
+    i, j = nl["ij"][:, 0], nl["ij"][:, 1]
+    v = nl["vectors"].astype(np.float64)
+    pc = pq[j].astype(np.float64) * [1, -1, -1, -1]
+    if mode == "lbod":
+        v = _quat_rotate64(pc, v)
+    elif mode == "obcd":
+        v = _quat_rotate64(qq[i].astype(np.float64), _quat_rotate64(pc, v))
+    elif mode == "oocd":
+        v = _quat_rotate64(pc, _quat_rotate64(qq[i].astype(np.float64), np.tile([0.0, 0, 1], (len(i), 1))))
+    theta = np.arctan2(v[:, 1], v[:, 0]) % (2 * np.pi)
+    phi = np.arccos(v[:, 2] / np.linalg.norm(v, axis=1))
+    t = np.stack([theta * 20 / (2 * np.pi), phi * 12 / np.pi], 1)
+    away = np.all(np.abs(t - np.round(t)) > 2e-5, axis=1)
+    b = np.minimum(np.floor(t).astype(int), [19, 11])
+    ref = np.zeros((20, 12), np.int64)
+    np.add.at(ref, tuple(b[away].T), 1)
+    near = np.zeros((20, 12), bool)
+    for off in np.ndindex(3, 3):
+        bb = np.clip(b[~away] + np.array(off) - 1, 0, [19, 11])
+        near[tuple(bb.T)] = True
+    npt.assert_array_equal(c[~near], ref[~near])
+    assert c.sum() == len(i)
+    dt, dp = 2 * np.pi / 20, np.pi / 12
+    sa = dt * (np.cos(np.arange(12) * dp) - np.cos(np.arange(12) * dp + dp))
+    npt.assert_allclose(bo, c / sa[None, :], rtol=1e-5)
