@@ -1150,8 +1150,10 @@ int launch_knn_lc(const fa::DevBox& db, const DevGrid& g, const float* qpts, uin
 }
 
 // Keys kept per point by the one-walk ball query: the bonds a point of a
-// uniform system at this density has, with slack, at most 32 (a 32 KB LDS
-// stage per block); points beyond it are walked again by the emission pass.
+// uniform system at this density has, with slack; points beyond it are walked
+// again by the emission pass.  When that exceeds 32 (a 32 KB LDS stage per
+// block) the walk only counts (0: every point is walked again, in processing
+// order, as a count pass and a fill pass would).
 constexpr uint32_t kWalkMaxSlots = kKeyBytes / (kBlock * sizeof(uint32_t));
 uint32_t ball_row_slots(const fa_nq* nq, float r_max)
 {
@@ -1160,7 +1162,9 @@ uint32_t ball_row_slots(const fa_nq* nq, float r_max)
     double const r = double(r_max);
     double const expect = nq->box.is2D ? dens * M_PI * r * r : dens * (4.0 / 3.0) * M_PI * r * r * r;
     double const c = std::ceil((expect * 1.15 + 4.0) / 4.0) * 4.0;
-    return uint32_t(std::min(std::max(c, 8.0), double(kWalkMaxSlots)));
+    if (c > double(kWalkMaxSlots))
+        return 0;
+    return uint32_t(std::max(c, 8.0));
 }
 
 int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args& a, int by_distance, fa_nlist* out)
@@ -1231,9 +1235,10 @@ int run_ball(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args
                            out->weights, out->vectors, gkeys, key_cap);
         FA_PROF_END();
     };
-    // self queries of points stored in random order emit in processing order
-    bool index_order = true;
-    if (self && n_query)
+    // self queries of points stored in random order emit in processing order,
+    // and so does a walk that kept no keys (the emission pass walks again)
+    bool index_order = C > 0;
+    if (self && n_query && index_order)
     {
         const uint32_t* sorder = nullptr;
         if (int rc = fa::spatial_order(nq->box, nq->d_points, nq->n, &sorder))
