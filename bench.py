@@ -109,6 +109,8 @@ def algorithmic_bytes(name, n, b_ball, b_knn, k=12, l=6, bins=200):
         # emission: offsets, query and neighbour positions, positions, keys in; bonds out
         "ball_emit_kernel": 4 * (n + 1) + 12 * n + 12 * n + 4 * n + 4 * b_ball + 28 * b_ball,
         "knn_aabb_kernel": 12 * n + 12 * n + 4 * n + 4 * n + 8 * k * n,
+        # staged search: cell-ordered points in; counts, positions, selections out
+        "knn_stage_kernel": 16 * n + 4 * n + 4 * n + 8 * k * n,
         "knn_write_kernel": 8 * k * n + 4 * n + 4 * (n + 1) + 12 * n + 28 * b_knn,
         "rdf_self_kernel": 16 * n + 4 * bins,
         "rdf_cluster_kernel": 16 * n + 4 * bins,

@@ -70,15 +70,19 @@ enum TunedAnalysis
 {
     TUNE_RDF_SELF = 1, // variant = a cell shape (R, F): r_max / R wide, r_max / (R F) along x
     TUNE_BALL = 2,     // variant = cells r_max / R wide, R = 1 + variant (batched ball query)
-    TUNE_KNN = 3,      // variant = cells rk, rk / 1.2, rk / 1.5, rk / 2 wide (rk: radius holding ~2k points)
+    TUNE_KNN = 3,      // variant = cells rk, rk / 1.2, rk / 1.5, rk / 2 wide (rk: radius holding ~2k points),
+                       // ring search (0-3) or staged search (4-7)
+    TUNE_KNN_REUSE = 4, // variant = ring search (0) or staged search (1) on a grid another query built
 };
-constexpr int kKnnVariants = 4;
+constexpr int kKnnCellSizes = 4;
+constexpr int kKnnVariants = 2 * kKnnCellSizes;
 // Keys of the batched analyses are offset by this (the single-frame keys are
 // small integers around 0).
 constexpr int kBatchTuneKey = 1000;
 bool tuning_enabled();
 int tuned_variant(int analysis, int key); // -1 when not tuned yet
 void record_variant(int analysis, int key, int variant);
+bool variant_preset(int analysis, int key); // set by the caller (fa_tune_set), not by the tuner
 // Number of launch variants that the tuner chooses among.
 int rdf_row_variant_count();   // single-frame self RDF
 int batch_rdf_shape_count();   // batched self RDF
@@ -338,9 +342,11 @@ template<int FORMULA> __device__ inline void stencil_interval(int R, int dim, in
 // the candidate's cell-ordered record (x, y, z, bits(j)).
 template<typename F>
 __device__ inline void visit_candidate(F& f, uint32_t j, float rx, float ry, float rz, float r_sq, int code,
-                                       const float4& p)
+                                       const float4& p, uint32_t slot)
 {
-    if constexpr (std::is_invocable_v<F&, uint32_t, float, float, float, float, int, const float4&>)
+    if constexpr (std::is_invocable_v<F&, uint32_t, float, float, float, float, int, const float4&, uint32_t>)
+        f(j, rx, ry, rz, r_sq, code, p, slot);
+    else if constexpr (std::is_invocable_v<F&, uint32_t, float, float, float, float, int, const float4&>)
         f(j, rx, ry, rz, r_sq, code, p);
     else
         f(j, rx, ry, rz, r_sq, code);
@@ -380,7 +386,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
                 float const ry = p.y - iy;
                 float const rz = pz - iz;
                 float const r_sq = (rx * rx + ry * ry) + rz * rz;
-                visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, code, p);
+                visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, code, p, t);
                 p = pn;
             }
         }
@@ -400,7 +406,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
                 float const ry = p.y - (qy + b.img[code][1]);
                 float const rz = pz - (qzz + b.img[code][2]);
                 float const r_sq = (rx * rx + ry * ry) + rz * rz;
-                visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, code, p);
+                visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, code, p, t);
             }
         }
     }
@@ -415,7 +421,7 @@ __device__ inline void visit_piece(const DevBox& b, const DevGrid& g, float qx, 
             float rx, ry, rz;
             wrap(b, p.x - qx, p.y - qy, p.z - qz, rx, ry, rz);
             float const r_sq = (rx * rx + ry * ry) + rz * rz;
-            visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, 13, p);
+            visit_candidate(f, __float_as_uint(p.w), rx, ry, rz, r_sq, 13, p, t);
             p = pn;
         }
     }
@@ -653,10 +659,10 @@ int spatial_order(const fa_box& box, const float* d_points, uint32_t n, const ui
 // Exclusive scan of n uint32 counts into n+1 offsets (offsets[n] = total);
 // returns the 64-bit total through *total (host) when total != nullptr.
 // d_out[0] = 0, d_out[t + 1] = sum of d_in[0..t]; *total (if given) = the 64-bit
-// sum, read back with one synchronisation.  If d_extra is given, the u32 it
-// points to is copied to *h_extra in the same synchronisation.
+// sum, read back with one synchronisation.  If d_extra is given, the n_extra
+// (1 or 2) u32 it points to are copied to h_extra in the same synchronisation.
 int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total,
-                       const uint32_t* d_extra = nullptr, uint32_t* h_extra = nullptr);
+                       const uint32_t* d_extra = nullptr, uint32_t* h_extra = nullptr, int n_extra = 1);
 // The same scan for code that may be captured into a HIP graph: no memset, no
 // host synchronisation; `tmp` holds scan_tmp_bytes(n) bytes.
 size_t scan_tmp_bytes(uint32_t n);

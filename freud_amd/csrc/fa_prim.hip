@@ -17,8 +17,9 @@ struct widen
 } // namespace
 
 int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64_t* total, const uint32_t* d_extra,
-                       uint32_t* h_extra)
+                       uint32_t* h_extra, int n_extra)
 {
+    size_t const extra_bytes = sizeof(uint32_t) * size_t(std::min(std::max(n_extra, 1), 2));
     hipStream_t s = stream();
     FA_HIP(hipMemsetAsync(d_out, 0, sizeof(uint32_t), s));
     if (n == 0)
@@ -27,7 +28,7 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
             *total = 0;
         if (d_extra)
         {
-            FA_HIP(hipMemcpyAsync(h_extra, d_extra, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            FA_HIP(hipMemcpyAsync(h_extra, d_extra, extra_bytes, hipMemcpyDeviceToHost, s));
             FA_HIP(hipStreamSynchronize(s));
         }
         return FA_OK;
@@ -60,12 +61,12 @@ int exclusive_scan_u32(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint64
     if (total)
         FA_HIP(hipMemcpyAsync(h_stage, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     if (d_extra)
-        FA_HIP(hipMemcpyAsync(h_stage + 1, d_extra, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        FA_HIP(hipMemcpyAsync(h_stage + 1, d_extra, extra_bytes, hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
     if (total)
         *total = h_stage[0];
     if (d_extra)
-        std::memcpy(h_extra, h_stage + 1, sizeof(uint32_t));
+        std::memcpy(h_extra, h_stage + 1, extra_bytes);
     return FA_OK;
 }
 

@@ -504,17 +504,12 @@ template<int KMAX> __device__ inline float knn_rsq_cut(const TopK<KMAX>& best, f
 // distance (where one point can appear through two images) is flagged for the
 // exhaustive kernel below.
 template<int KMAX, bool SELF>
-__global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                          uint32_t nq, uint32_t k, float rlim_sq, float rlim,
-                                                          float rmin_thr, int exclude_ii, float wmin,
-                                                          float half_min_plane, int rmax_ring,
-                                                          uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
-                                                          uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag,
-                                                          uint32_t* __restrict__ pos)
+__device__ inline void knn_ring_search(const DevBox& b, const DevGrid& g, const float* __restrict__ qpts, uint32_t p,
+                                       uint32_t k, float rlim_sq, float rlim, float rmin_thr, int exclude_ii,
+                                       float wmin, float half_min_plane, int rmax_ring, uint32_t* __restrict__ counts,
+                                       uint64_t* __restrict__ sel, uint32_t* __restrict__ flagged,
+                                       uint32_t* __restrict__ nflag, uint32_t* __restrict__ pos)
 {
-    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (p >= nq)
-        return;
     float qx, qy, qz;
     uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
     if (SELF)
@@ -578,6 +573,195 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
     counts[i] = n;
     for (uint32_t m = 0; m < n; ++m)
         sel[uint64_t(p) * k + m] = best.at(m);
+}
+
+// The ring search over every query position, or (LIST) over the processing
+// positions list[0 .. *nlist), grid-stride, for the points the staged search
+// below hands back.
+template<int KMAX, bool SELF, bool LIST>
+__global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                          uint32_t nq, uint32_t k, float rlim_sq, float rlim,
+                                                          float rmin_thr, int exclude_ii, float wmin,
+                                                          float half_min_plane, int rmax_ring,
+                                                          uint32_t* __restrict__ counts, uint64_t* __restrict__ sel,
+                                                          uint32_t* __restrict__ flagged, uint32_t* __restrict__ nflag,
+                                                          uint32_t* __restrict__ pos, const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ nlist)
+{
+    if (LIST)
+    {
+        uint32_t const nl = *nlist;
+        for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nl; t += gridDim.x * blockDim.x)
+            knn_ring_search<KMAX, SELF>(b, g, qpts, list[t], k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin,
+                                        half_min_plane, rmax_ring, counts, sel, flagged, nflag, pos);
+        return;
+    }
+    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p < nq)
+        knn_ring_search<KMAX, SELF>(b, g, qpts, p, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, half_min_plane,
+                                    rmax_ring, counts, sel, flagged, nflag, pos);
+}
+
+// Staged AABB kNN: the same rings and the same answer as the ring search, with
+// the selection moved out of the walk.  Inserting into a register top-k
+// during the walk costs every lane of a wave the insertion whenever one lane
+// inserts, which is nearly every candidate.  Here a candidate with
+// d < r_stage (a radius holding ~k + 2 sqrt(k) + 2 points at the system's
+// mean density; stage_thr is the smallest r_sq whose sqrtf reaches it) is
+// only recorded in the lane's LDS stage, as its cell-order slot and image
+// code (4 bytes, so the stage leaves room for several blocks per CU); the k
+// smallest are picked after the walk, their distances recomputed from the
+// slot with the walk's own arithmetic.  A full stage is cut to its k smallest,
+// whose k-th then bounds what may enter.  The rings end as in the ring search,
+// once k staged distances lie below the radius the walked rings cover
+// (counted as they are staged, for the first three rings).  A lane whose
+// answer may hold candidates the stage did not take (fewer than k staged
+// once the rings cover r_stage), or that needs a fourth ring or the
+// half-box radius, is handed back (`refine`) to the ring search.
+template<int KMAX, bool SELF>
+__global__ __launch_bounds__(kBlock) void knn_stage_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, uint32_t k, float rlim_sq, float rlim,
+                                                           float rmin_thr, int exclude_ii, float wmin,
+                                                           float half_min_plane, int rmax_ring, float stage_thr,
+                                                           float r_stage, uint32_t C, uint32_t* __restrict__ counts,
+                                                           uint64_t* __restrict__ sel, uint32_t* __restrict__ flagged,
+                                                           uint32_t* __restrict__ nflag, uint32_t* __restrict__ refine,
+                                                           uint32_t* __restrict__ nrefine, uint32_t* __restrict__ pos)
+{
+    // s_e[m * kBlock + tid]: staged entry m of thread tid, slot << 5 | image code
+    uint32_t* s_e = reinterpret_cast<uint32_t*>(fa_dyn_lds);
+    uint32_t const tid = threadIdx.x;
+    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
+    if (p >= nq)
+        return;
+    float qx, qy, qz;
+    uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
+    if (SELF)
+        pos[i] = p;
+    float const qzz = b.is2D ? 0.0f : qz;
+    // selection key of a staged entry: (d, j << 5 | code), r_sq as the walk computed it
+    auto key_of = [&](uint32_t e, float& r_sq) -> uint64_t {
+        uint32_t const code = e & 31u;
+        float4 const pj = g.cpos[e >> 5];
+        float const pz = b.is2D ? 0.0f : pj.z;
+        float const rx = pj.x - (qx + b.img[code][0]);
+        float const ry = pj.y - (qy + b.img[code][1]);
+        float const rz = pz - (qzz + b.img[code][2]);
+        r_sq = (rx * rx + ry * ry) + rz * rz;
+        return knn_key(r_sq, (__float_as_uint(pj.w) << 5) | code);
+    };
+    // squared radii covered after rings 1, 2, 3 (termination counts)
+    float const w1 = wmin * (1.0f - 1e-4f);
+    float const c1 = w1 * w1, c2 = (2.0f * w1) * (2.0f * w1), c3 = (3.0f * w1) * (3.0f * w1);
+    uint32_t found = 0, c = 0, in1 = 0, in2 = 0, in3 = 0;
+    bool hand_back = false;
+    float cut = INFINITY; // after a cut: candidates above it cannot be among the k smallest
+    auto reduce = [&]() {
+        TopK<KMAX> best;
+        best.init();
+        float r_sq;
+        for (uint32_t m = 0; m < c; ++m)
+            best.insert(key_of(s_e[m * kBlock + tid], r_sq));
+        uint64_t const kth = best.at(k - 1);
+        uint32_t w = 0;
+        in1 = in2 = in3 = 0;
+        for (uint32_t m = 0; m < c; ++m)
+        {
+            uint32_t const e = s_e[m * kBlock + tid];
+            if (key_of(e, r_sq) <= kth)
+            {
+                s_e[w * kBlock + tid] = e;
+                ++w;
+                in1 += r_sq < c1;
+                in2 += r_sq < c2;
+                in3 += r_sq < c3;
+            }
+        }
+        c = w;
+        float const d = knn_key_distance(kth);
+        cut = d * d * (1.0f + 0x1p-20f) + 1e-37f;
+    };
+    int R = 0;
+    for (; R <= rmax_ring; ++R)
+    {
+        fa::for_each_candidate<AABB_FORMULA>(
+            b, g, qx, qy, qz, R, true,
+            [&](uint32_t j, float, float, float, float r_sq, int code, const float4&, uint32_t slot) {
+                if (!(r_sq < rlim_sq))
+                    return;
+                if (exclude_ii && j == i)
+                    return;
+                if (!(r_sq >= rmin_thr))
+                    return;
+                ++found;
+                if (!(r_sq < stage_thr && r_sq <= cut))
+                    return;
+                if (c == C)
+                {
+                    reduce();
+                    if (!(r_sq <= cut))
+                        return;
+                }
+                s_e[c * kBlock + tid] = (slot << 5) | uint32_t(code);
+                ++c;
+                in1 += r_sq < c1;
+                in2 += r_sq < c2;
+                in3 += r_sq < c3;
+            });
+        float const complete = float(R) * w1;
+        uint32_t const in = R == 1 ? in1 : R == 2 ? in2 : R == 3 ? in3 : 0u;
+        if (c >= k && in >= k)
+            break;
+        if (c < k && found > c && complete >= r_stage)
+        {
+            hand_back = true; // the answer holds candidates at or above r_stage
+            break;
+        }
+        if (complete >= rlim)
+            break;
+        if ((complete >= half_min_plane && rlim > half_min_plane) || R == 3)
+        {
+            hand_back = true; // longer walks, and the exhaustive flag, are the ring search's
+            break;
+        }
+    }
+    uint32_t const n = min(found, k);
+    if (hand_back || c < n)
+    {
+        refine[atomicAdd(nrefine, 1u)] = p;
+        return;
+    }
+    // c >= n: the n smallest keys are staged
+    float last = 0.0f;
+    float r_sq;
+    if (c <= k)
+    {
+        // n == c: every staged entry is selected
+        for (uint32_t m = 0; m < c; ++m)
+        {
+            uint64_t const key = key_of(s_e[m * kBlock + tid], r_sq);
+            last = fmaxf(last, knn_key_distance(key));
+            sel[uint64_t(p) * k + m] = key;
+        }
+    }
+    else
+    {
+        TopK<KMAX> best;
+        best.init();
+        for (uint32_t m = 0; m < c; ++m)
+            best.insert(key_of(s_e[m * kBlock + tid], r_sq));
+        for (uint32_t m = 0; m < n; ++m)
+            sel[uint64_t(p) * k + m] = best.at(m);
+        last = knn_key_distance(best.at(n - 1));
+    }
+    // two images of one point are at least a plane distance apart (see the ring search)
+    if (rlim > half_min_plane && n > 0 && last >= half_min_plane)
+    {
+        flagged[atomicAdd(nflag, 1u)] = p;
+        counts[i] = 0;
+        return;
+    }
+    counts[i] = n;
 }
 
 // LinkCell formula kNN (LinkCell.cc:575-679): every point in cells within
@@ -1116,24 +1300,79 @@ float sqrt_threshold(float r)
     return __builtin_bit_cast(float, lo);
 }
 
+// Staged-search parameters for k neighbours at number density dens: the
+// stage radius holds ~k + 2 sqrt(k) + 2 points on average and the stage
+// ~2 standard deviations more (at most 32 entries of 4 bytes per lane).
+struct KnnStage
+{
+    float thr = 0.0f; // smallest r_sq whose sqrtf reaches the stage radius
+    float r = 0.0f;   // the stage radius
+    uint32_t C = 0;   // 0: no staged search (k > 16)
+};
+KnnStage knn_stage_params(bool is2D, double dens, uint32_t k)
+{
+    KnnStage st;
+    if (k > 16 || !(dens > 0.0))
+        return st;
+    double const lam = k + 2.0 * std::sqrt(double(k)) + 2.0;
+    double const r = is2D ? std::sqrt(lam / (M_PI * dens)) : std::cbrt(3.0 * lam / (4.0 * M_PI * dens));
+    uint32_t C = uint32_t(std::ceil((lam + 2.0 * std::sqrt(lam)) / 4.0)) * 4;
+    C = std::min<uint32_t>(32, std::max<uint32_t>(C, k + 4));
+    st.r = float(r);
+    st.thr = sqrt_threshold(st.r);
+    st.C = C;
+    return st;
+}
+
 template<int KMAX>
 int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, uint32_t nq, uint32_t k, float rlim,
-                    float r_min, int exclude_ii, float wmin, float hmp, int rmax_ring, uint32_t* counts,
-                    uint64_t* sel, uint32_t* flagged, uint32_t* nflag, const float* self_points, uint32_t self_n,
-                    uint32_t* pos)
+                    float r_min, int exclude_ii, float wmin, float hmp, int rmax_ring, const KnnStage& st,
+                    uint32_t* counts, uint64_t* sel, uint32_t* flagged, uint32_t* nflag, uint32_t* refine,
+                    uint32_t* nrefine, const float* self_points, uint32_t self_n, uint32_t* pos)
 {
     float const rlim_sq = rlim * rlim;
     float const rmin_thr = sqrt_threshold(r_min);
     bool const self = (qpts == self_points) && (nq == self_n);
-    FA_PROF_BEGIN("knn_aabb_kernel");
+    unsigned const blocks = (nq + kBlock - 1) / kBlock;
+    if (st.C == 0)
+    {
+        FA_PROF_BEGIN("knn_aabb_kernel");
+        if (self)
+            hipLaunchKernelGGL((knn_aabb_kernel<KMAX, true, false>), dim3(blocks), dim3(kBlock), 0, fa::stream(), db,
+                               g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring, counts, sel,
+                               flagged, nflag, pos, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL((knn_aabb_kernel<KMAX, false, false>), dim3(blocks), dim3(kBlock), 0, fa::stream(),
+                               db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring, counts,
+                               sel, flagged, nflag, pos, nullptr, nullptr);
+        FA_PROF_END();
+        FA_HIP(hipGetLastError());
+        return FA_OK;
+    }
+    size_t const lds = size_t(st.C) * kBlock * sizeof(uint32_t);
+    FA_PROF_BEGIN("knn_stage_kernel");
     if (self)
-        hipLaunchKernelGGL((knn_aabb_kernel<KMAX, true>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring,
-                           counts, sel, flagged, nflag, pos);
+        hipLaunchKernelGGL((knn_stage_kernel<KMAX, true>), dim3(blocks), dim3(kBlock), lds, fa::stream(), db, g, qpts,
+                           nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring, st.thr, st.r, st.C, counts, sel,
+                           flagged, nflag, refine, nrefine, pos);
     else
-        hipLaunchKernelGGL((knn_aabb_kernel<KMAX, false>), dim3((nq + kBlock - 1) / kBlock), dim3(kBlock), 0,
-                           fa::stream(), db, g, qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring,
-                           counts, sel, flagged, nflag, pos);
+        hipLaunchKernelGGL((knn_stage_kernel<KMAX, false>), dim3(blocks), dim3(kBlock), lds, fa::stream(), db, g,
+                           qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring, st.thr, st.r, st.C,
+                           counts, sel, flagged, nflag, refine, nrefine, pos);
+    FA_PROF_END();
+    FA_HIP(hipGetLastError());
+    // the handed-back points: a quarter of the full grid, grid-stride over
+    // the list (its length stays on the device)
+    unsigned const rblocks = std::max(1u, (blocks + 3) / 4);
+    FA_PROF_BEGIN("knn_refine_kernel");
+    if (self)
+        hipLaunchKernelGGL((knn_aabb_kernel<KMAX, true, true>), dim3(rblocks), dim3(kBlock), 0, fa::stream(), db, g,
+                           qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring, counts, sel,
+                           flagged, nflag, pos, refine, nrefine);
+    else
+        hipLaunchKernelGGL((knn_aabb_kernel<KMAX, false, true>), dim3(rblocks), dim3(kBlock), 0, fa::stream(), db, g,
+                           qpts, nq, k, rlim_sq, rlim, rmin_thr, exclude_ii, wmin, hmp, rmax_ring, counts, sel,
+                           flagged, nflag, pos, refine, nrefine);
     FA_PROF_END();
     FA_HIP(hipGetLastError());
     return FA_OK;
@@ -1327,17 +1566,20 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         return fa::nlist_finish_counts(out);
     }
     // scratch: selections (nq*k u64, point-major by processing position),
-    // write temp (nq*k u64), flagged list, processing position per point
+    // write temp (nq*k u64), the flagged and handed-back counts and lists,
+    // processing position per point
     void* tmp = nullptr;
     uint64_t const selb = uint64_t(n_query) * k * sizeof(uint64_t);
-    if (int rc = fa::scratch(4, 2 * selb + sizeof(uint32_t) * (2 * size_t(n_query) + 64), &tmp))
+    if (int rc = fa::scratch(4, 2 * selb + sizeof(uint32_t) * (3 * size_t(n_query) + 128), &tmp))
         return rc;
     uint64_t* sel = static_cast<uint64_t*>(tmp);
     uint64_t* wtmp = sel + uint64_t(n_query) * k;
     uint32_t* nflag = reinterpret_cast<uint32_t*>(wtmp + uint64_t(n_query) * k);
+    uint32_t* nrefine = nflag + 1;
     uint32_t* flagged = nflag + 16;
     uint32_t* pos = flagged + n_query + 48;
-    FA_HIP(hipMemsetAsync(nflag, 0, sizeof(uint32_t), s));
+    uint32_t* refine = pos + n_query + 16;
+    FA_HIP(hipMemsetAsync(nflag, 0, 2 * sizeof(uint32_t), s));
     float npd[3];
     fa::nearest_plane_distance(nq->box, npd);
     float min_pd = std::min(npd[0], npd[1]);
@@ -1346,6 +1588,8 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
     const fa::DevBox* dbp = nullptr;
     uint32_t h_nflag = 0;
     float rlim = 0.0f;
+    // the tuner entry behind a staged search (revised after it, see below)
+    int staged_analysis = 0, staged_key = 0, staged_variant = 0;
     if (nq->kind == FA_NQ_AABB)
     {
         rlim = knn_radius_limit(nq, a);
@@ -1356,8 +1600,12 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
         double const dens = double(nq->n) / std::max(double(vol), 1e-30);
         double const rk =
             nq->box.is2D ? std::sqrt(2.0 * k / (M_PI * dens)) : std::cbrt(3.0 * 2.0 * k / (4.0 * M_PI * dens));
-        static constexpr double kFrac[fa::kKnnVariants] = {1.0, 1.0 / 1.2, 1.0 / 1.5, 0.5};
-        auto run_dims = [&](const int dims[3]) -> int {
+        static constexpr double kFrac[fa::kKnnCellSizes] = {1.0, 1.0 / 1.2, 1.0 / 1.5, 0.5};
+        // the staged search (k <= 16) against the ring search: the tuner times
+        // both, as the better one depends on how regular the points are
+        KnnStage const stage_params = knn_stage_params(nq->box.is2D, dens, k);
+        auto run_dims = [&](const int dims[3], bool staged) -> int {
+            KnnStage const stage = staged ? stage_params : KnnStage{};
             if (int rc = fa::ensure_grid(nq, dims))
                 return rc;
             dbp = &nq->cur->dbox;
@@ -1365,10 +1613,10 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
             for (int d = 0; d < (nq->box.is2D ? 2 : 3); ++d)
                 wmin = std::min(wmin, npd[d] / float(dims[d]));
             int const rmax_ring = std::max(dims[0], std::max(dims[1], dims[2])) * 2 + 2;
-            FA_HIP(hipMemsetAsync(nflag, 0, sizeof(uint32_t), s));
+            FA_HIP(hipMemsetAsync(nflag, 0, 2 * sizeof(uint32_t), s));
 #define FA_KNN_LAUNCH(KM)                                                                                     \
     launch_knn_aabb<KM>(*dbp, nq->cur->dev(), qpts, n_query, k, rlim, a.r_min, a.exclude_ii, wmin, min_pd / 2.0f, \
-                        rmax_ring, out->counts, sel, flagged, nflag, nq->d_points, nq->n, pos)
+                        rmax_ring, stage, out->counts, sel, flagged, nflag, refine, nrefine, nq->d_points, nq->n, pos)
             if (k <= 8)
                 return FA_KNN_LAUNCH(8);
             if (k <= 12)
@@ -1381,11 +1629,13 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
 #undef FA_KNN_LAUNCH
         };
         auto run_variant = [&](int v) -> int {
-            float const wcell = float(std::max(rk * kFrac[v], 1e-6));
+            float const wcell = float(std::max(rk * kFrac[v % fa::kKnnCellSizes], 1e-6));
             int dims[3];
             fa::grid_dims_for(nq->box, nq->n, wcell, 1, dims);
-            return run_dims(dims);
+            return run_dims(dims, v >= fa::kKnnCellSizes);
         };
+        // variants the tuner tries: the staged ones only where they exist
+        int const n_variants = stage_params.C ? fa::kKnnVariants : fa::kKnnCellSizes;
         // A grid that another query on these points has built serves when its
         // cells lie within the range of sizes the tuner chooses from (rk / 2 to
         // rk, with 25 % slack above): every cell size selects the same
@@ -1405,19 +1655,44 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
                 reuse = gs;
         }
         int rc = FA_OK;
+        int const key = fa::tune_key(dens, rk) * 128 + int(std::min<uint32_t>(k, 127u));
         if (k <= 64 && reuse >= 0)
         {
             int const dims[3] = {nq->grid_dims[reuse][0], nq->grid_dims[reuse][1], nq->grid_dims[reuse][2]};
-            rc = run_dims(dims);
+            int mode = stage_params.C ? fa::tuned_variant(fa::TUNE_KNN_REUSE, key) : 0;
+            if (mode < 0 && fa::tuning_enabled())
+            {
+                float best = 0.0f;
+                for (int v = 0; v < 2; ++v)
+                {
+                    float ms = 0.0f;
+                    if (int rc2 = run_dims(dims, v == 1))
+                        return rc2;
+                    if (int rc2 = fa::time_launch([&] { return run_dims(dims, v == 1); }, &ms))
+                        return rc2;
+                    if (mode < 0 || ms < best)
+                    {
+                        best = ms;
+                        mode = v;
+                    }
+                }
+                fa::record_variant(fa::TUNE_KNN_REUSE, key, mode);
+            }
+            rc = run_dims(dims, mode == 1);
+            if (mode == 1 && stage_params.C)
+            {
+                staged_analysis = fa::TUNE_KNN_REUSE;
+                staged_key = key;
+                staged_variant = 0;
+            }
         }
         else if (k <= 64)
         {
-            int const key = fa::tune_key(dens, rk) * 128 + int(std::min<uint32_t>(k, 127u));
             int variant = fa::tuned_variant(fa::TUNE_KNN, key);
             if (variant < 0 && fa::tuning_enabled())
             {
                 float best = 0.0f;
-                for (int v = 0; v < fa::kKnnVariants; ++v)
+                for (int v = 0; v < n_variants; ++v)
                 {
                     float ms = 0.0f;
                     if (int rc2 = run_variant(v)) // builds the grid
@@ -1432,7 +1707,14 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
                 }
                 fa::record_variant(fa::TUNE_KNN, key, variant);
             }
-            rc = run_variant(std::max(variant, 0) % fa::kKnnVariants);
+            int const v = std::max(variant, 0) % fa::kKnnVariants;
+            rc = run_variant(v);
+            if (v >= fa::kKnnCellSizes && stage_params.C)
+            {
+                staged_analysis = fa::TUNE_KNN;
+                staged_key = key;
+                staged_variant = v - fa::kKnnCellSizes;
+            }
         }
         else
         {
@@ -1487,9 +1769,18 @@ int run_knn(fa_nq* nq, const float* qpts, uint32_t n_query, const fa_query_args&
     // queries hold count 0 until the exhaustive kernel has run; the offsets
     // are then scanned again.
     uint64_t total = 0;
+    uint32_t h_counts[2] = {0, 0}; // flagged, handed back
     if (int rc = fa::exclusive_scan_u32(out->counts, out->offsets, n_query, &total,
-                                        nq->kind == FA_NQ_AABB ? nflag : nullptr, &h_nflag))
+                                        nq->kind == FA_NQ_AABB ? nflag : nullptr, h_counts, 2))
         return rc;
+    h_nflag = h_counts[0];
+    // The staged search pays a serial ring walk for each handed-back point; on
+    // points irregular enough to hand back more than 1/256 of the queries (a
+    // liquid, say, against a crystal at the same density, which the tuner's key
+    // does not tell apart) the ring search is faster, so a tuner choice of the
+    // staged search is revised for the next call (a caller's preset is kept).
+    if (staged_analysis && h_counts[1] > n_query / 256 && !fa::variant_preset(staged_analysis, staged_key))
+        fa::record_variant(staged_analysis, staged_key, staged_variant);
     if (h_nflag)
     {
         if (int rc = exhaustive_work_check(nq, h_nflag, k))

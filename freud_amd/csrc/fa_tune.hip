@@ -9,6 +9,7 @@
 // changes speed only.  The table can be read and preset through the C-ABI
 // (fa_tune_*), which is how a caller persists tuning across processes.
 #include <map>
+#include <set>
 #include <utility>
 
 #include "fa_common.h"
@@ -17,6 +18,7 @@ namespace fa {
 namespace {
 
 std::map<std::pair<int, int>, int> g_table;
+std::set<std::pair<int, int>> g_preset; // entries set through fa_tune_set (not revised by the library)
 bool g_tune_on = true;
 hipEvent_t g_e0 = nullptr, g_e1 = nullptr;
 
@@ -36,6 +38,12 @@ int tuned_variant(int analysis, int key)
 void record_variant(int analysis, int key, int variant)
 {
     g_table[{analysis, key}] = variant;
+    g_preset.erase({analysis, key});
+}
+
+bool variant_preset(int analysis, int key)
+{
+    return g_preset.count({analysis, key}) != 0;
 }
 
 int time_launch(const std::function<int()>& launch, float* ms)
@@ -91,15 +99,22 @@ int fa_tune_get(int idx, int* analysis, int* key, int* variant)
 int fa_tune_set(int analysis, int key, int variant)
 {
     if (variant < 0)
+    {
         fa::g_table.erase({analysis, key});
+        fa::g_preset.erase({analysis, key});
+    }
     else
+    {
         fa::record_variant(analysis, key, variant);
+        fa::g_preset.insert({analysis, key});
+    }
     return FA_OK;
 }
 
 int fa_tune_reset(void)
 {
     fa::g_table.clear();
+    fa::g_preset.clear();
     return FA_OK;
 }
 
@@ -113,6 +128,8 @@ int fa_tune_variants(int analysis, int key)
         return 2;
     case fa::TUNE_KNN:
         return fa::kKnnVariants;
+    case fa::TUNE_KNN_REUSE:
+        return 2;
     default:
         return fa::fail(FA_ERR_INVALID, "unknown tuned analysis");
     }

@@ -181,9 +181,113 @@ def test_knn_after_ball_query(oracle, name, box, pts):
         nq.query(pts, dict(r_max=r_ball, exclude_ii=True)).toNeighborList()
         for k in (6, 12):
             qa = dict(num_neighbors=k, exclude_ii=True)
-            nl = nq.query(pts, qa).toNeighborList()
             ref = oracle.query(_bt(box), pts, pts, qa, nthreads=8)
-            _assert_nlist_equal(nl, ref)
+            for mode in (0, 1):  # ring and staged search on the reused grid
+                with _ForcedKnn(box, len(pts), k, 0, mode):
+                    nl = nq.query(pts, qa).toNeighborList()
+                _assert_nlist_equal(nl, ref)
+
+
+def _knn_tune_key(box, n, k):
+    dens = n / box.volume
+    rk = np.sqrt(2.0 * k / (np.pi * dens)) if box.is2D else np.cbrt(3.0 * 2.0 * k / (4.0 * np.pi * dens))
+    return int(np.floor(2.0 * np.log2(dens * rk ** 3) + 0.5)) * 128 + k
+
+
+class _ForcedKnn:
+    """Presets the autotuner's kNN choice for (box, n, k): `variant` for a
+    fresh grid (0-3 ring search, 4-7 staged search, by cell size) and
+    `reuse_mode` on a grid another query built (0 ring, 1 staged)."""
+
+    def __init__(self, box, n, k, variant, reuse_mode):
+        from freud_amd import _lib
+
+        self._lib = _lib
+        self.key = _knn_tune_key(box, n, k)
+        self.preset = {(3, self.key): variant, (4, self.key): reuse_mode}
+
+    def __enter__(self):
+        self.before = self._lib.tune_table()
+        self._lib.tune_load(self.preset)
+
+    def __exit__(self, *exc):
+        self._lib.tune_load({a: self.before.get(a, -1) for a in self.preset})
+
+
+@pytest.mark.parametrize("is2D", [False, True])
+def test_knn_staged_paths(oracle, is2D):
+    """The staged kNN search's rarer branches beside its common one: dense
+    clumps (far above the mean density, so a lane's stage fills and is reduced
+    to its k smallest, repeatedly), sparse regions (fewer than k points inside
+    the stage radius: the point is handed back to the ring search), a radius
+    limit below the stage radius (fewer than k neighbours), r_min, and query
+    points other than the points."""
+    rng = np.random.default_rng(31 + is2D)
+    L = 30.0
+    box = Box.square(L) if is2D else Box.cube(L)
+    dim = 2 if is2D else 3
+    centres = (rng.random((5, dim)) - 0.5) * L
+    clumps = np.concatenate([c + rng.standard_normal((1400, dim)) * 0.8 for c in centres])
+    spread = (rng.random((6000, dim)) - 0.5) * L
+    pts = np.concatenate([clumps, spread])
+    if is2D:
+        pts = np.concatenate([pts, np.zeros((len(pts), 1))], axis=1)
+    pts = box.wrap(pts.astype(np.float32)).astype(np.float32)
+    if is2D:
+        pts[:, 2] = 0
+    qp = box.wrap(((rng.random((3000, 3)) - 0.5) * L).astype(np.float32)).astype(np.float32)
+    if is2D:
+        qp[:, 2] = 0
+    nq = freud_amd.AABBQuery(box, pts)
+    cases = [(pts, dict(num_neighbors=k, exclude_ii=True)) for k in (4, 8, 12, 16)]
+    cases += [(pts, dict(num_neighbors=12, r_max=1.0, exclude_ii=True)),
+              (pts, dict(num_neighbors=8, r_min=0.2, exclude_ii=True)),
+              (qp, dict(num_neighbors=12)), (qp, dict(num_neighbors=6, r_max=2.0))]
+    for q, qa in cases:
+        refs = {by_d: oracle.query(_bt(box), pts, q, qa, sort_by_distance=by_d, nthreads=8) for by_d in (False, True)}
+        # every staged cell size, then the ring search, on fresh grids and on the grid already built
+        for variant, mode in ((4, 1), (5, 1), (6, 1), (7, 1), (0, 0)):
+            print(f"staged knn 2D={is2D} {qa} nq={len(q)} variant {variant} reuse mode {mode}", flush=True)
+            with _ForcedKnn(box, len(pts), qa["num_neighbors"], variant, mode):
+                for fresh in (True, False):
+                    obj = freud_amd.AABBQuery(box, pts) if fresh else nq
+                    for by_d in (False, True):
+                        nl = obj.query(q, qa).toNeighborList(sort_by_distance=by_d)
+                        _assert_nlist_equal(nl, refs[by_d])
+
+
+def test_knn_staged_revised_on_hand_backs(oracle):
+    """Points irregular enough that the staged kNN search hands many queries
+    back to the ring search (sparse points beside dense clumps): whatever the
+    autotuner chose on the first call, its entry afterwards is a ring-search
+    variant, on a fresh grid and on a reused one; results equal the oracle's
+    throughout."""
+    from freud_amd import _lib
+
+    rng = np.random.default_rng(77)
+    L = 30.0
+    box = Box.cube(L)
+    centres = (rng.random((5, 3)) - 0.5) * L
+    clumps = np.concatenate([c + rng.standard_normal((1400, 3)) * 0.8 for c in centres])
+    pts = box.wrap(np.concatenate([clumps, (rng.random((6000, 3)) - 0.5) * L]).astype(np.float32)).astype(np.float32)
+    k = 12
+    key = _knn_tune_key(box, len(pts), k)
+    qa = dict(num_neighbors=k, exclude_ii=True)
+    ref = oracle.query(_bt(box), pts, pts, qa, nthreads=8)
+    before = _lib.tune_table()
+    try:
+        _lib.tune_load({(3, key): -1, (4, key): -1})
+        nq = freud_amd.AABBQuery(box, pts)
+        for _ in range(2):
+            _assert_nlist_equal(nq.query(pts, qa).toNeighborList(), ref)
+        assert _lib.tune_table()[(3, key)] < 4
+        nq.query(pts, dict(r_max=1.6, exclude_ii=True)).toNeighborList()  # a grid the kNN can reuse
+        for _ in range(2):
+            _assert_nlist_equal(nq.query(pts, qa).toNeighborList(), ref)
+        table = _lib.tune_table()
+        assert table.get((4, key), 0) == 0
+    finally:
+        _lib.tune_load({(3, key): before.get((3, key), -1), (4, key): before.get((4, key), -1)})
 
 
 def test_knn_linkcell(oracle):
@@ -638,7 +742,8 @@ def test_spatially_ordered_paths(oracle):
 
 @pytest.mark.parametrize("name,box,pts", SYSTEMS[:3], ids=[s[0] for s in SYSTEMS[:3]])
 def test_knn_cell_variants(oracle, name, box, pts):
-    """Every kNN cell size the autotuner can pick gives the same NeighborList."""
+    """Every kNN variant the autotuner can pick (four cell sizes, ring or
+    staged search) gives the same NeighborList."""
     from freud_amd import _lib
 
     for k in (6, 12):

@@ -223,11 +223,13 @@ def test_modulus_positive_shortcut_is_exact():
 def test_tuner_variant_counts():
     """The autotuner's variant counts per analysis (no GPU work): nine
     single-frame RDF cell shapes, five batched ones (keys offset by 1000),
-    two batched ball-query cell sizes, four kNN cell sizes."""
+    two batched ball-query cell sizes, four kNN cell sizes each with the ring
+    or the staged search, and the two kNN searches on a reused grid."""
     assert _lib.tune_variants(1, 0) == 9
     assert _lib.tune_variants(1, 1003) == 5
     assert _lib.tune_variants(2, 1003) == 2
-    assert _lib.tune_variants(3, 4) == 4
+    assert _lib.tune_variants(3, 4) == 8
+    assert _lib.tune_variants(4, 4) == 2
     with pytest.raises(ValueError):
         _lib.tune_variants(9, 0)
 
