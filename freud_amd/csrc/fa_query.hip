@@ -124,7 +124,11 @@ template<typename K> __device__ inline void insertion_sort(K* a, uint32_t n)
 
 // Writes the bond of query point i (at qx, qy, qz) given its key, at bond
 // index bidx.
-template<int FORMULA, bool BYDIST>
+// NT: non-temporal stores, for a block's contiguous range of the
+// NeighborList (full lines; the output then does not push the gathered
+// positions out of L2).  Scattered segments keep ordinary stores, whose
+// partial lines merge in L2.
+template<int FORMULA, bool BYDIST, bool NT = false>
 __device__ inline void write_bond_at(const DevBox& b, float qx, float qy, float qz, const float* __restrict__ pts,
                                      uint32_t i, uint64_t key, uint64_t bidx, uint32_t* __restrict__ nb_out,
                                      float* __restrict__ d_out, float* __restrict__ w_out, float* __restrict__ v_out)
@@ -135,6 +139,16 @@ __device__ inline void write_bond_at(const DevBox& b, float qx, float qy, float 
     float rx, ry, rz;
     fa::bond_vector<FORMULA>(b, qx, qy, qz, pts[3 * j], pts[3 * j + 1], pts[3 * j + 2], code, rx, ry, rz);
     float const r_sq = (rx * rx + ry * ry) + rz * rz;
+    if (NT)
+    {
+        __builtin_nontemporal_store((uint64_t(j) << 32) | uint64_t(i), reinterpret_cast<uint64_t*>(nb_out) + bidx);
+        __builtin_nontemporal_store(sqrtf(r_sq), d_out + bidx);
+        __builtin_nontemporal_store(1.0f, w_out + bidx);
+        __builtin_nontemporal_store(rx, v_out + 3 * bidx);
+        __builtin_nontemporal_store(ry, v_out + 3 * bidx + 1);
+        __builtin_nontemporal_store(rz, v_out + 3 * bidx + 2);
+        return;
+    }
     reinterpret_cast<uint2*>(nb_out)[bidx] = make_uint2(i, j);
     d_out[bidx] = sqrtf(r_sq);
     w_out[bidx] = 1.0f;
@@ -156,7 +170,7 @@ __device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpt
 // the block's query points in processing order (exclusive offsets s_off,
 // point indices s_i, positions s_q); bond t of point lo goes to
 // offsets[s_i[lo]] + t - s_off[lo].
-template<int FORMULA, bool BYDIST, typename K>
+template<int FORMULA, bool BYDIST, typename K, bool NT = false>
 __device__ inline void store_block_bonds(const DevBox& b, const float* s_q, const float* __restrict__ pts,
                                          const K* keys, uint32_t total, uint32_t nloc, const uint32_t* s_off,
                                          const uint32_t* s_i, const uint32_t* __restrict__ offsets,
@@ -175,7 +189,7 @@ __device__ inline void store_block_bonds(const DevBox& b, const float* s_q, cons
                 hi = mid;
         }
         uint32_t const i = s_i[lo];
-        write_bond_at<FORMULA, BYDIST>(b, s_q[3 * lo], s_q[3 * lo + 1], s_q[3 * lo + 2], pts, i, uint64_t(keys[t]),
+        write_bond_at<FORMULA, BYDIST, NT>(b, s_q[3 * lo], s_q[3 * lo + 1], s_q[3 * lo + 2], pts, i, uint64_t(keys[t]),
                                        uint64_t(offsets[i]) + (t - s_off[lo]), nb_out, d_out, w_out, v_out);
     }
 }
@@ -423,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, 
     }
     __syncthreads();
     if (in_lds)
-        store_block_bonds<FORMULA, BYDIST>(b, s_q, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+        store_block_bonds<FORMULA, BYDIST, K, INDEX>(b, s_q, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
                                            w_out, v_out);
 }
 
@@ -959,7 +973,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     }
     __syncthreads();
     if (in_lds)
-        store_block_bonds<FORMULA, BYDIST>(b, s_q, pts, s_keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
+        store_block_bonds<FORMULA, BYDIST, K, INDEX>(b, s_q, pts, s_keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
                                            w_out, v_out);
 }
 
