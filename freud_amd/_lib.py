@@ -115,7 +115,8 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64),
                                        ctypes.POINTER(ctypes.c_int)]),
     "fa_sl_features": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32,
-                                      ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+                                      ctypes.POINTER(fa_nlist), ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
     "fa_sl_decide": (ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(fa_nlist), ctypes.c_void_p, ctypes.c_float,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),

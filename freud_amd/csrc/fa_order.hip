@@ -828,6 +828,8 @@ template<int L> __device__ inline void feat_l(const float2* acc, float inv_nb, c
     *wout = w / (s * sqrtf(s));
 }
 
+// LM: the largest l of the features (8 or 12; columns of larger l are 0).
+template<int LM>
 __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const float* __restrict__ pts, uint32_t n,
                                                              const uint32_t* __restrict__ nbr,
                                                              const float* __restrict__ dist,
@@ -840,7 +842,7 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
                                                              float2* __restrict__ qlm6, float* __restrict__ ql6,
                                                              uint32_t stage_cap)
 {
-    constexpr int LM = 12;
+    static_assert(LM == 8 || LM == 12, "feature sets: l <= 8 or l <= 12");
     uint32_t const tile0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
     uint32_t const t0 = tile0 + threadIdx.x;
     BondSrc const src = stage_bonds(tile0, min(blockDim.x, n - tile0), offsets, nbr, dist, fa_stage_lds,
@@ -848,9 +850,9 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
     if (t0 >= n)
         return;
     uint32_t const i = t0;
-    float2 acc[feat_base(14)];
+    float2 acc[feat_base(LM + 2)];
 #pragma unroll
-    for (int t = 0; t < feat_base(14); ++t)
+    for (int t = 0; t < feat_base(LM + 2); ++t)
         acc[t] = make_float2(0.0f, 0.0f);
     uint32_t const e0 = offsets[i], e1 = offsets[i + 1];
     for_each_bond(b, pts, i, src, nullptr, false, e0, e1, [&](float ct, float st, float cp, float sp, float) {
@@ -868,8 +870,15 @@ __global__ __launch_bounds__(kBlock) void sl_features_kernel(fa::DevBox b, const
     feat_l<4>(acc + feat_base(4), inv_nb, coef4, f + 0, f + 5);
     feat_l<6>(acc + feat_base(6), inv_nb, coef6, f + 1, f + 6);
     feat_l<8>(acc + feat_base(8), inv_nb, coef8, f + 2, f + 7);
-    feat_l<10>(acc + feat_base(10), inv_nb, coef10, f + 3, f + 8);
-    feat_l<12>(acc + feat_base(12), inv_nb, coef12, f + 4, f + 9);
+    if constexpr (LM == 12)
+    {
+        feat_l<10>(acc + feat_base(10), inv_nb, coef10, f + 3, f + 8);
+        feat_l<12>(acc + feat_base(12), inv_nb, coef12, f + 4, f + 9);
+    }
+    else
+    {
+        f[3] = f[4] = f[8] = f[9] = 0.0f;
+    }
     f[10] = total_w;
     f[11] = 0.0f;
     // q_6m and q_6 as the plain Steinhardt kernel computes them
@@ -1013,14 +1022,15 @@ __global__ __launch_bounds__(kBlock) void solid_bonds_kernel(uint32_t n, int l, 
 // `margin` away from 0 takes the classifier's label; the others (and NaN
 // logits, e.g. particles without bonds) count their solid-like bonds exactly,
 // with the arithmetic of solid_bonds_kernel.  fallback[i] marks the latter.
+// Labels from the classifier: a particle whose logit lies at least `margin`
+// from 0 takes its sign; the others are appended to `list` (the exact
+// bond-dot pass below labels them).  They are ~1 % of the particles and
+// scattered, so labelling them here would hold nearly every wave for one
+// lane's serial walk over its bonds.
 __global__ __launch_bounds__(kBlock) void sl_decide_kernel(uint32_t n, const float* __restrict__ logits, float margin,
-                                                           const uint32_t* __restrict__ nbr,
-                                                           const uint32_t* __restrict__ offsets,
-                                                           const float2* __restrict__ qlm6,
-                                                           const float* __restrict__ ql6, int normalize_q,
-                                                           float q_threshold, uint32_t solid_threshold,
                                                            uint8_t* __restrict__ solid, uint8_t* __restrict__ fallback,
-                                                           uint32_t* __restrict__ n_fallback)
+                                                           uint32_t* __restrict__ n_fallback,
+                                                           uint32_t* __restrict__ list)
 {
     uint32_t const i = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool fb = false;
@@ -1028,27 +1038,46 @@ __global__ __launch_bounds__(kBlock) void sl_decide_kernel(uint32_t n, const flo
     {
         float const y = logits[i];
         fb = !(fabsf(y) >= margin);
-        uint8_t lab;
-        if (fb)
-            lab = solid_bond_count<13>(i, 6, nbr, offsets, qlm6, ql6, normalize_q, q_threshold, nullptr) >=
-                  solid_threshold;
-        else
-            lab = y > 0.0f;
-        solid[i] = lab;
+        solid[i] = fb ? uint8_t(0) : uint8_t(y > 0.0f);
         fallback[i] = fb;
     }
-    // one atomic per workgroup that has fallbacks (a device-wide counter
-    // serialises its atomics)
-    __shared__ uint32_t s_fb;
+    // list slots: per wave within the block, then one device atomic per block
+    __shared__ uint32_t s_cnt, s_base;
     if (threadIdx.x == 0)
-        s_fb = 0;
+        s_cnt = 0;
     __syncthreads();
     uint64_t const ball = __ballot(fb);
-    if ((threadIdx.x & 63u) == 0 && ball)
-        atomicAdd(&s_fb, uint32_t(__popcll(ball)));
+    uint32_t const lane = threadIdx.x & 63u;
+    uint32_t woff = 0;
+    if (lane == 0 && ball)
+        woff = atomicAdd(&s_cnt, uint32_t(__popcll(ball)));
+    woff = __shfl(woff, 0);
     __syncthreads();
-    if (threadIdx.x == 0 && s_fb)
-        atomicAdd(n_fallback, s_fb);
+    if (threadIdx.x == 0)
+        s_base = s_cnt ? atomicAdd(n_fallback, s_cnt) : 0u;
+    __syncthreads();
+    if (fb)
+        list[s_base + woff + uint32_t(__popcll(ball & ((1ull << lane) - 1ull)))] = i;
+}
+
+// SolidLiquid.cc:51-96: the exact label of each listed particle (grid-stride
+// over list[0 .. *n_list)).
+__global__ __launch_bounds__(kBlock) void sl_exact_kernel(const uint32_t* __restrict__ list,
+                                                          const uint32_t* __restrict__ n_list,
+                                                          const uint32_t* __restrict__ nbr,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          const float2* __restrict__ qlm6,
+                                                          const float* __restrict__ ql6, int normalize_q,
+                                                          float q_threshold, uint32_t solid_threshold,
+                                                          uint8_t* __restrict__ solid)
+{
+    uint32_t const nl = *n_list;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nl; t += gridDim.x * blockDim.x)
+    {
+        uint32_t const i = list[t];
+        solid[i] = solid_bond_count<13>(i, 6, nbr, offsets, qlm6, ql6, normalize_q, q_threshold, nullptr) >=
+                   solid_threshold;
+    }
 }
 
 // SolidLiquid.cc:100-110: solid-like bonds between particles with at least
@@ -1704,11 +1733,13 @@ int fa_steinhardt(const fa_box* box, const float* d_points, uint32_t n, const fa
     return fa::steinhardt_run(box, d_points, n, nl, ls, n_l, flags, d_qlm, d_ql, d_qlave, d_qlmave, d_wl, order);
 }
 
-int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, float* d_features,
-                   float* d_qlm6, float* d_ql6)
+int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, int lmax,
+                   float* d_features, float* d_qlm6, float* d_ql6)
 {
     if (int rc = fa_init())
         return rc;
+    if (lmax != 8 && lmax != 12)
+        return fa::fail(FA_ERR_INVALID, "learned SolidLiquid features take lmax 8 or 12");
     if (int rc = ensure_ylm_coef())
         return rc;
     if (nl->num_query_points != n)
@@ -1722,9 +1753,14 @@ int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const f
     fa::DevBox const db = fa::make_devbox(*box);
     unsigned const blocks = (n + kBlock - 1) / kBlock;
     uint32_t const cap = stage_capacity(nl, n);
-    FA_KLAUNCH(sl_features_kernel, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, d_points, n,
-               nl->neighbors, nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
-               reinterpret_cast<float2*>(d_qlm6), d_ql6, cap);
+    if (lmax == 8)
+        FA_KLAUNCH(sl_features_kernel<8>, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, d_points, n,
+                   nl->neighbors, nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
+                   reinterpret_cast<float2*>(d_qlm6), d_ql6, cap);
+    else
+        FA_KLAUNCH(sl_features_kernel<12>, dim3(blocks), dim3(kBlock), size_t(cap) * 8, fa::stream(), db, d_points,
+                   n, nl->neighbors, nl->distances, nl->offsets, c[0], c[1], c[2], c[3], c[4], d_features,
+                   reinterpret_cast<float2*>(d_qlm6), d_ql6, cap);
     FA_HIP(hipGetLastError());
     return FA_OK;
 }
@@ -1737,15 +1773,24 @@ int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float ma
         return rc;
     if (nl->num_query_points != n)
         return fa::fail(FA_ERR_RUNTIME, "NeighborList found inconsistent array sizes.");
+    // the fallback count, then the list of fallback particles
     uint32_t* cnt = nullptr;
-    if (int rc = fa::scratch(15, 256, (void**) &cnt))
+    if (int rc = fa::scratch(15, 256 + sizeof(uint32_t) * size_t(n), (void**) &cnt))
         return rc;
+    uint32_t* list = cnt + 64;
     if (int rc = fa::zero_u32(cnt, 1))
         return rc;
     if (n)
-        FA_KLAUNCH(sl_decide_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, fa::stream(), n, d_logits,
-                   margin, nl->neighbors, nl->offsets, reinterpret_cast<const float2*>(d_qlm6), d_ql6, normalize_q,
-                   q_threshold, solid_threshold, d_solid, d_fallback, cnt);
+    {
+        unsigned const blocks = (n + kBlock - 1) / kBlock;
+        FA_KLAUNCH(sl_decide_kernel, dim3(blocks), dim3(kBlock), 0, fa::stream(), n, d_logits, margin, d_solid,
+                   d_fallback, cnt, list);
+        FA_HIP(hipGetLastError());
+        // an eighth of the full grid, grid-stride over the list (its length stays on the device)
+        FA_KLAUNCH(sl_exact_kernel, dim3(std::max(1u, blocks / 8)), dim3(kBlock), 0, fa::stream(), list, cnt,
+                   nl->neighbors, nl->offsets, reinterpret_cast<const float2*>(d_qlm6), d_ql6, normalize_q,
+                   q_threshold, solid_threshold, d_solid);
+    }
     FA_HIP(hipGetLastError());
     if (n_fallback)
         return fa::read_u32(cnt, n_fallback);

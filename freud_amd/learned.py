@@ -6,7 +6,9 @@ its bonds have a normalised q_6 dot product above ``q_threshold``; that needs a
 second pass over every bond gathering the neighbour's 13 q_6m.  Here one fused
 pass over the bonds computes per-particle rotation invariants (q_l and
 normalised w_l for l = 4..12 even, and the bond count; the q_6m come out of the
-same pass), an 11 -> 64 -> 64 -> 1 network on the matrix cores turns them into
+same pass; ``feature_lmax=8`` keeps l = 4, 6, 8 only, which halves the
+per-bond spherical-harmonic work), an 11 -> 64 -> 64 -> 1 network on the
+matrix cores turns them into
 a logit, and only particles whose logit lies within ``margin`` of 0 take the
 exact bond-dot path.  The network is trained on the GPU on labels that the
 exact SolidLiquid produces.
@@ -31,9 +33,14 @@ class LearnedSolidLiquid(_PairCompute):
         normalize_q (bool): normalise the bond dot products (the exact path).
         margin (float): particles with |logit| < margin take the exact path.
         seed (int): initial weights.
+        feature_lmax (int): largest l of the invariants, 8 (q_l, w_l for
+            l = 4, 6, 8) or 12 (l = 4 .. 12); the others are zero.
     """
 
-    def __init__(self, q_threshold=0.7, solid_threshold=6, normalize_q=True, margin=0.5, seed=0):
+    def __init__(self, q_threshold=0.7, solid_threshold=6, normalize_q=True, margin=0.5, seed=0, feature_lmax=8):
+        if feature_lmax not in (8, 12):
+            raise ValueError("feature_lmax must be 8 or 12")
+        self._feature_lmax = int(feature_lmax)
         self._q_threshold = float(np.float32(q_threshold))
         self._solid_threshold = int(solid_threshold)
         self._normalize_q = bool(normalize_q)
@@ -80,8 +87,8 @@ class LearnedSolidLiquid(_PairCompute):
         feat = _lib.reuse(self, "_d_feat", (n, 12), np.float32)
         qlm6 = _lib.reuse(self, "_d_qlm6", (n, 13, 2), np.float32)
         ql6 = _lib.reuse(self, "_d_ql6", (n,), np.float32)
-        check(lib().fa_sl_features(ctypes.byref(nq._cbox), nq._d_points.ptr, n, ctypes.byref(nlist._c), feat.ptr,
-                                   qlm6.ptr, ql6.ptr))
+        check(lib().fa_sl_features(ctypes.byref(nq._cbox), nq._d_points.ptr, n, ctypes.byref(nlist._c),
+                                   self._feature_lmax, feat.ptr, qlm6.ptr, ql6.ptr))
         return feat, qlm6, ql6
 
     def exact_labels(self, system, neighbors=None):
@@ -216,6 +223,11 @@ class LearnedSolidLiquid(_PairCompute):
     def nlist(self):
         return self._nlist
 
+    @property
+    def feature_lmax(self):
+        return self._feature_lmax
+
     def __repr__(self):
         return (f"freud_amd.learned.LearnedSolidLiquid(q_threshold={self._q_threshold}, "
-                f"solid_threshold={self._solid_threshold}, normalize_q={self._normalize_q}, margin={self._margin})")
+                f"solid_threshold={self._solid_threshold}, normalize_q={self._normalize_q}, margin={self._margin}, "
+                f"feature_lmax={self._feature_lmax})")

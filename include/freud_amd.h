@@ -384,10 +384,11 @@ int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* 
  * classifier below).  Per particle, over the bonds of nl (unweighted):
  * d_features (n x 12 float32) = q_l for l = 4, 6, 8, 10, 12, normalised w_l
  * for the same l (Steinhardt.cc:120-222, 291-359 definitions), the bond count,
- * and a zero pad; d_qlm6 (n x 13 complex64, m order 0..6, -1..-6) and d_ql6
- * (n) equal fa_steinhardt's q_6m and q_6 bit for bit. */
-int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, float* d_features,
-                   float* d_qlm6, float* d_ql6);
+ * and a zero pad; with lmax = 8 the columns of l = 10 and 12 are 0 (lmax is 8
+ * or 12).  d_qlm6 (n x 13 complex64, m order 0..6, -1..-6) and d_ql6 (n)
+ * equal fa_steinhardt's q_6m and q_6 bit for bit. */
+int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, int lmax,
+                   float* d_features, float* d_qlm6, float* d_ql6);
 /* Learned SolidLiquid labels (no counterpart in the reference): particles
  * whose classifier logit lies at least `margin` from 0 take the classifier's
  * label (logit > 0: solid); the others, and NaN logits, count their solid-like

@@ -22,16 +22,23 @@ def _nl(box, pts):
     return freud_amd.AABBQuery(box, pts).query(pts, dict(r_max=1.5, exclude_ii=True)).toNeighborList()
 
 
-def test_features_match_oracle(oracle):
+@pytest.mark.parametrize("lmax", [8, 12])
+def test_features_match_oracle(oracle, lmax):
     box, pts, _ = freud_amd.data.two_phase(6, seed=2)
     nl = _nl(box, pts)
-    ls = LearnedSolidLiquid()
+    ls = LearnedSolidLiquid(feature_lmax=lmax)
     f = ls.features((box, pts), neighbors=nl)
     ref_nl = oracle.query(_bt(box), pts, pts, dict(r_max=1.5, exclude_ii=True), nthreads=8)
     ref = mlp.features(_bt(box), pts, ref_nl, nthreads=8)
+    q_cols = [0, 1, 2] if lmax == 8 else [0, 1, 2, 3, 4]
+    w_cols = [c + 5 for c in q_cols]
     npt.assert_array_equal(f[:, 10], ref[:, 10])
-    npt.assert_allclose(f[:, :5], ref[:, :5], atol=1e-5)  # q_l
-    npt.assert_allclose(f[:, 5:10], ref[:, 5:10], atol=1e-4)  # normalised w_l (other summation order)
+    npt.assert_allclose(f[:, q_cols], ref[:, q_cols], atol=1e-5)  # q_l
+    npt.assert_allclose(f[:, w_cols], ref[:, w_cols], atol=1e-4)  # normalised w_l (other summation order)
+    unused = [c for c in range(10) if c not in q_cols + w_cols]
+    assert np.all(f[:, unused] == 0.0)
+    with pytest.raises(ValueError):
+        LearnedSolidLiquid(feature_lmax=10)
 
 
 def test_fallback_everywhere_is_exact_solidliquid():
