@@ -25,6 +25,8 @@
 namespace {
 
 using bf16x4 = __attribute__((ext_vector_type(4))) short;
+using s16x8 = __attribute__((ext_vector_type(8))) short;
+using bf16x8 = __attribute__((ext_vector_type(8))) __bf16;
 using f32x4 = __attribute__((ext_vector_type(4))) float;
 
 constexpr int kBlock = 256;
@@ -57,11 +59,16 @@ __device__ inline float from_bf16(uint16_t h)
     return __uint_as_float(uint32_t(h) << 16);
 }
 
-// Per-lane operands of the network for one wave.
+// Per-lane operands of the network for one wave.  Layer 1 (K = 16) uses
+// the 16x16x16 BF16 MFMA, layer 2 (K = 64) gfx950's 16x16x32 one in two
+// K blocks.  Lane (r, q) = (lane % 16, lane / 16) holds K entries 8q .. 8q+7
+// of a 16x16x32 block; K block c is ordered so that these are the hidden
+// units 16(2c) + 4q + 0..3 and 16(2c+1) + 4q + 0..3, which are the units the
+// lane holds after layer 1 (any K order gives the same sum when W2 follows it).
 struct MlpRegs
 {
     bf16x4 w1[4];     // W1 block mt
-    bf16x4 w2[4][4];  // W2 block (mt, kc)
+    s16x8 w2[4][2];   // W2 block (mt, K block c)
     float b1[4][4];   // b1[16 mt + 4q + j]
     float b2[4][4];
     float w3[4][4];
@@ -89,8 +96,12 @@ __device__ inline void load_regs(MlpRegs& R, const uint16_t* __restrict__ w1b, c
     {
         R.w1[mt] = load4(w1b + (mt * 16 + r) * kIn + 4 * q);
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc)
-            R.w2[mt][kc] = load4(w2b + (mt * 16 + r) * kH + kc * 16 + 4 * q);
+        for (int c = 0; c < 2; ++c)
+        {
+            bf16x4 const lo = load4(w2b + (mt * 16 + r) * kH + (2 * c) * 16 + 4 * q);
+            bf16x4 const hi = load4(w2b + (mt * 16 + r) * kH + (2 * c + 1) * 16 + 4 * q);
+            R.w2[mt][c] = s16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
@@ -109,22 +120,37 @@ __device__ inline void load_regs(MlpRegs& R, const uint16_t* __restrict__ w1b, c
     R.b3 = P[OFF_B3];
 }
 
-// Forward pass of the wave's 16 particles (feature rows `row`, one per r;
-// row == ~0u: padding).  Returns the logit (the same in the 4 lanes of a
+// Features 4q .. 4q+3 of feature row `row` (row == ~0u: padding; zeros past
+// the used features).  Rows are 48 bytes, so the 4 values are one aligned
+// 16-byte load.
+__device__ inline float4 mlp_features(const float* __restrict__ feat, uint32_t row)
+{
+    uint32_t const q = (threadIdx.x & 63u) >> 4;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (row != ~0u && 4 * int(q) < kFeatUsed)
+    {
+        v = *reinterpret_cast<const float4*>(feat + uint64_t(row) * kFeatStride + 4 * q);
+        static_assert(kFeatUsed == 11 && kFeatStride == 12, "feature 11 is the row's pad");
+        if (q == 2)
+            v.w = 0.0f;
+    }
+    return v;
+}
+
+// Forward pass of the wave's 16 particles (raw features from mlp_features,
+// one particle per r).  Returns the logit (the same in the 4 lanes of a
 // particle).  With z1/z2 set, stores the pre-activations of sample `slot`.
-__device__ inline float mlp_wave(const MlpRegs& R, const float* __restrict__ feat, uint32_t row, float* z1, float* z2,
-                                 uint32_t slot)
+__device__ inline float mlp_wave(const MlpRegs& R, float4 v, float* z1, float* z2, uint32_t slot)
 {
     uint32_t const lane = threadIdx.x & 63u, q = lane >> 4;
     bf16x4 xb;
+    float const vf[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j)
     {
         int const f = 4 * int(q) + j;
-        float v = 0.0f;
-        if (row != ~0u && f < kFeatUsed)
-            v = feat[uint64_t(row) * kFeatStride + f];
-        xb[j] = short(to_bf16((v - R.mu[j]) * R.isig[j]));
+        float const x = f < kFeatUsed ? vf[j] : 0.0f;
+        xb[j] = short(to_bf16((x - R.mu[j]) * R.isig[j]));
     }
     f32x4 const zero = {0.0f, 0.0f, 0.0f, 0.0f};
     bf16x4 hb[4];
@@ -147,8 +173,13 @@ __device__ inline float mlp_wave(const MlpRegs& R, const float* __restrict__ fea
     {
         f32x4 d = zero;
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc)
-            d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(R.w2[mt][kc], hb[kc], d, 0, 0, 0);
+        for (int c = 0; c < 2; ++c)
+        {
+            s16x8 const hk = {hb[2 * c][0],     hb[2 * c][1],     hb[2 * c][2],     hb[2 * c][3],
+                              hb[2 * c + 1][0], hb[2 * c + 1][1], hb[2 * c + 1][2], hb[2 * c + 1][3]};
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, R.w2[mt][c]),
+                                                         __builtin_bit_cast(bf16x8, hk), d, 0, 0, 0);
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
@@ -175,10 +206,19 @@ __global__ __launch_bounds__(kBlock) void mlp_infer_kernel(const uint16_t* __res
     uint32_t const lane = threadIdx.x & 63u;
     uint32_t const waves = gridDim.x * (kBlock / 64);
     uint32_t const ngroups = (n + 15) / 16;
-    for (uint32_t g = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); g < ngroups; g += waves)
+    uint32_t g = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    auto row_of = [&](uint32_t gg) {
+        uint32_t const p = gg * 16 + (lane & 15u);
+        return gg < ngroups && p < n ? p : ~0u;
+    };
+    // the next group's features are loaded before the current group's layers run
+    float4 next = mlp_features(feat, row_of(g));
+    for (; g < ngroups; g += waves)
     {
+        float4 const cur = next;
+        next = mlp_features(feat, row_of(g + waves));
         uint32_t const p = g * 16 + (lane & 15u);
-        float const y = mlp_wave(R, feat, p < n ? p : ~0u, nullptr, nullptr, 0);
+        float const y = mlp_wave(R, cur, nullptr, nullptr, 0);
         if (lane < 16 && p < n)
             logits[p] = y;
     }
@@ -202,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void mlp_train_fwd_kernel(const uint16_t* _
     if (g * 16 >= nb)
         return;
     bool const ok = b < nb;
-    float const v = mlp_wave(R, feat, ok ? idx[b] : ~0u, ok ? z1 : nullptr, ok ? z2 : nullptr, b);
+    float const v = mlp_wave(R, mlp_features(feat, ok ? idx[b] : ~0u), ok ? z1 : nullptr, ok ? z2 : nullptr, b);
     if (lane < 16 && ok)
         y[b] = v;
 }
