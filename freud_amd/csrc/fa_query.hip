@@ -166,6 +166,22 @@ __device__ inline void write_bond(const DevBox& b, const float* __restrict__ qpt
                                    w_out, v_out);
 }
 
+// The point lo (< nloc) of a block whose key segment [s_off[lo], s_off[lo + 1])
+// holds element t < s_off[nloc].
+__device__ inline uint32_t segment_of(const uint32_t* s_off, uint32_t nloc, uint32_t t)
+{
+    uint32_t lo = 0, hi = nloc;
+    while (hi - lo > 1)
+    {
+        uint32_t const mid = (lo + hi) >> 1;
+        if (s_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // Block-cooperative store of a block's bonds: keys[t] for t < total belong to
 // the block's query points in processing order (exclusive offsets s_off,
 // point indices s_i, positions s_q); bond t of point lo goes to
@@ -179,15 +195,7 @@ __device__ inline void store_block_bonds(const DevBox& b, const float* s_q, cons
 {
     for (uint32_t t = threadIdx.x; t < total; t += kBlock)
     {
-        uint32_t lo = 0, hi = nloc;
-        while (hi - lo > 1)
-        {
-            uint32_t const mid = (lo + hi) >> 1;
-            if (s_off[mid] <= t)
-                lo = mid;
-            else
-                hi = mid;
-        }
+        uint32_t const lo = segment_of(s_off, nloc, t);
         uint32_t const i = s_i[lo];
         write_bond_at<FORMULA, BYDIST, NT>(b, s_q[3 * lo], s_q[3 * lo + 1], s_q[3 * lo + 2], pts, i, uint64_t(keys[t]),
                                        uint64_t(offsets[i]) + (t - s_off[lo]), nb_out, d_out, w_out, v_out);
@@ -359,6 +367,7 @@ __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, 
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
+    __shared__ uint32_t s_row[kBlock];
     __shared__ float s_q[3 * kBlock];
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
     uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
@@ -391,26 +400,48 @@ __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, 
     __syncthreads();
     bool const in_lds = total <= key_cap;
     K* keys = in_lds ? s_keys : reinterpret_cast<K*>(gkeys_raw);
+    auto key_of = [&](uint32_t jc, float x, float y, float z) -> K {
+        if (BYDIST)
+        {
+            uint32_t const j = jc >> 5;
+            float rx, ry, rz;
+            fa::bond_vector<FORMULA>(b, x, y, z, pts[3 * j], pts[3 * j + 1], pts[3 * j + 2], int(jc & 31u), rx, ry,
+                                     rz);
+            float const r_sq = (rx * rx + ry * ry) + rz * rz;
+            return K((uint64_t(float_order_bits(sqrtf(r_sq))) << 32) | jc);
+        }
+        return K(jc);
+    };
+    bool const coop = in_lds && C > 0;
+    if (coop)
+    {
+        // The staged key rows into LDS, element t by thread t: one load
+        // instruction covers whole rows (a lane reading its own row key by key
+        // lets the rows of a CU's blocks fall out of L2 between its loads).
+        // Points past C bonds (no row) are walked again below.
+        if (tid < nloc)
+            s_row[tid] = cnt <= C ? pos[i] : ~0u;
+        __syncthreads();
+        for (uint32_t t = tid; t < total; t += kBlock)
+        {
+            uint32_t const lo = segment_of(s_off, nloc, t);
+            uint32_t const r0 = s_row[lo];
+            if (r0 != ~0u)
+                s_keys[t] = key_of(rows[r0 + (t - s_off[lo])], s_q[3 * lo], s_q[3 * lo + 1], s_q[3 * lo + 2]);
+        }
+        __syncthreads();
+    }
     if (tid < nloc)
     {
         uint32_t const beg = in_lds ? excl : offsets[i];
-        auto key_of = [&](uint32_t jc) -> K {
-            if (BYDIST)
-            {
-                uint32_t const j = jc >> 5;
-                float rx, ry, rz;
-                fa::bond_vector<FORMULA>(b, qx, qy, qz, pts[3 * j], pts[3 * j + 1], pts[3 * j + 2], int(jc & 31u),
-                                         rx, ry, rz);
-                float const r_sq = (rx * rx + ry * ry) + rz * rz;
-                return K((uint64_t(float_order_bits(sqrtf(r_sq))) << 32) | jc);
-            }
-            return K(jc);
-        };
         if (cnt <= C)
         {
-            const uint32_t* row = rows + pos[i];
-            for (uint32_t m = 0; m < cnt; ++m)
-                keys[beg + m] = key_of(row[m]);
+            if (!coop)
+            {
+                const uint32_t* row = rows + pos[i];
+                for (uint32_t m = 0; m < cnt; ++m)
+                    keys[beg + m] = key_of(row[m], qx, qy, qz);
+            }
         }
         else
         {
@@ -921,6 +952,7 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_i[kBlock];
     __shared__ uint32_t s_tmp[kBlock];
+    __shared__ uint32_t s_row[kBlock];
     __shared__ float s_q[3 * kBlock];
     uint32_t const tid = threadIdx.x;
     uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
@@ -952,24 +984,40 @@ __global__ __launch_bounds__(kBlock) void knn_write_kernel(DevBox b, DevGrid g, 
         s_off[nloc] = total;
     __syncthreads();
     bool const in_lds = total <= CAP;
-    if (tid < nloc)
+    auto key_of = [&](uint64_t sk) -> K {
+        uint32_t const jc = uint32_t(sk);
+        if (BYDIST)
+            return K((uint64_t(float_order_bits(knn_key_distance(sk))) << 32) | jc);
+        return K(jc);
+    };
+    if (in_lds)
     {
-        K* keys = in_lds ? s_keys + excl : reinterpret_cast<K*>(tmp + uint64_t(i) * k);
+        // The block's selection rows into LDS, element t by thread t: one load
+        // instruction covers whole rows.  (Each lane reading its own row in k
+        // steps keeps ~12 MB of rows per XCD half read, far beyond its L2, and
+        // fetched them about four times.)
+        if (tid < nloc)
+            s_row[tid] = INDEX && SELF ? pos[i] : p0 + tid;
+        __syncthreads();
+        for (uint32_t t = tid; t < total; t += kBlock)
+        {
+            uint32_t const lo = segment_of(s_off, nloc, t);
+            s_keys[t] = key_of(sel[uint64_t(s_row[lo]) * k + (t - s_off[lo])]);
+        }
+        __syncthreads();
+        if (tid < nloc)
+            insertion_sort(s_keys + excl, n);
+    }
+    else if (tid < nloc)
+    {
+        K* keys = reinterpret_cast<K*>(tmp + uint64_t(i) * k);
         const uint64_t* row = sel + uint64_t(INDEX && SELF ? pos[i] : p0 + tid) * k;
         for (uint32_t m = 0; m < n; ++m)
-        {
-            uint64_t const sk = row[m];
-            uint32_t const jc = uint32_t(sk);
-            if (BYDIST)
-                keys[m] = K((uint64_t(float_order_bits(knn_key_distance(sk))) << 32) | jc);
-            else
-                keys[m] = K(jc);
-        }
+            keys[m] = key_of(row[m]);
         insertion_sort(keys, n);
-        if (!in_lds)
-            for (uint32_t m = 0; m < n; ++m)
-                write_bond<FORMULA, BYDIST>(b, qpts, pts, i, uint64_t(keys[m]), uint64_t(offsets[i]) + m, nb_out,
-                                            d_out, w_out, v_out);
+        for (uint32_t m = 0; m < n; ++m)
+            write_bond<FORMULA, BYDIST>(b, qpts, pts, i, uint64_t(keys[m]), uint64_t(offsets[i]) + m, nb_out, d_out,
+                                        w_out, v_out);
     }
     __syncthreads();
     if (in_lds)
