@@ -676,137 +676,139 @@ __global__ __launch_bounds__(kBlock) void knn_stage_kernel(DevBox b, DevGrid g, 
     // s_e[m * kBlock + tid]: staged entry m of thread tid, slot << 5 | image code
     uint32_t* s_e = reinterpret_cast<uint32_t*>(fa_dyn_lds);
     uint32_t const tid = threadIdx.x;
-    uint32_t const p = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
-    if (p >= nq)
-        return;
-    float qx, qy, qz;
-    uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
-    if (SELF)
-        pos[i] = p;
-    float const qzz = b.is2D ? 0.0f : qz;
-    // selection key of a staged entry: (d, j << 5 | code), r_sq as the walk computed it
-    auto key_of = [&](uint32_t e, float& r_sq) -> uint64_t {
-        uint32_t const code = e & 31u;
-        float4 const pj = g.cpos[e >> 5];
-        float const pz = b.is2D ? 0.0f : pj.z;
-        float const rx = pj.x - (qx + b.img[code][0]);
-        float const ry = pj.y - (qy + b.img[code][1]);
-        float const rz = pz - (qzz + b.img[code][2]);
-        r_sq = (rx * rx + ry * ry) + rz * rz;
-        return knn_key(r_sq, (__float_as_uint(pj.w) << 5) | code);
-    };
-    // squared radii covered after rings 1, 2, 3 (termination counts)
-    float const w1 = wmin * (1.0f - 1e-4f);
-    float const c1 = w1 * w1, c2 = (2.0f * w1) * (2.0f * w1), c3 = (3.0f * w1) * (3.0f * w1);
-    uint32_t found = 0, c = 0, in1 = 0, in2 = 0, in3 = 0;
-    bool hand_back = false;
-    float cut = INFINITY; // after a cut: candidates above it cannot be among the k smallest
-    auto reduce = [&]() {
-        TopK<KMAX> best;
-        best.init();
-        float r_sq;
-        for (uint32_t m = 0; m < c; ++m)
-            best.insert(key_of(s_e[m * kBlock + tid], r_sq));
-        uint64_t const kth = best.at(k - 1);
-        uint32_t w = 0;
-        in1 = in2 = in3 = 0;
-        for (uint32_t m = 0; m < c; ++m)
-        {
-            uint32_t const e = s_e[m * kBlock + tid];
-            if (key_of(e, r_sq) <= kth)
+    uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * blockDim.x;
+    uint32_t const p = p0 + tid;
+    TopK<KMAX> best; // the lane's selection, ascending
+    best.init();
+    if (p < nq)
+    {
+        float qx, qy, qz;
+        uint32_t const i = query_at<SELF>(g, qpts, p, qx, qy, qz);
+        if (SELF)
+            pos[i] = p;
+        float const qzz = b.is2D ? 0.0f : qz;
+        // selection key of a staged entry: (d, j << 5 | code), r_sq as the walk computed it
+        auto key_of = [&](uint32_t e, float& r_sq) -> uint64_t {
+            uint32_t const code = e & 31u;
+            float4 const pj = g.cpos[e >> 5];
+            float const pz = b.is2D ? 0.0f : pj.z;
+            float const rx = pj.x - (qx + b.img[code][0]);
+            float const ry = pj.y - (qy + b.img[code][1]);
+            float const rz = pz - (qzz + b.img[code][2]);
+            r_sq = (rx * rx + ry * ry) + rz * rz;
+            return knn_key(r_sq, (__float_as_uint(pj.w) << 5) | code);
+        };
+        // squared radii covered after rings 1, 2, 3 (termination counts)
+        float const w1 = wmin * (1.0f - 1e-4f);
+        float const c1 = w1 * w1, c2 = (2.0f * w1) * (2.0f * w1), c3 = (3.0f * w1) * (3.0f * w1);
+        uint32_t found = 0, c = 0, in1 = 0, in2 = 0, in3 = 0;
+        bool hand_back = false;
+        float cut = INFINITY; // after a cut: candidates above it cannot be among the k smallest
+        auto reduce = [&]() {
+            TopK<KMAX> best;
+            best.init();
+            float r_sq;
+            for (uint32_t m = 0; m < c; ++m)
+                best.insert(key_of(s_e[m * kBlock + tid], r_sq));
+            uint64_t const kth = best.at(k - 1);
+            uint32_t w = 0;
+            in1 = in2 = in3 = 0;
+            for (uint32_t m = 0; m < c; ++m)
             {
-                s_e[w * kBlock + tid] = e;
-                ++w;
-                in1 += r_sq < c1;
-                in2 += r_sq < c2;
-                in3 += r_sq < c3;
+                uint32_t const e = s_e[m * kBlock + tid];
+                if (key_of(e, r_sq) <= kth)
+                {
+                    s_e[w * kBlock + tid] = e;
+                    ++w;
+                    in1 += r_sq < c1;
+                    in2 += r_sq < c2;
+                    in3 += r_sq < c3;
+                }
+            }
+            c = w;
+            float const d = knn_key_distance(kth);
+            cut = d * d * (1.0f + 0x1p-20f) + 1e-37f;
+        };
+        int R = 0;
+        for (; R <= rmax_ring; ++R)
+        {
+            fa::for_each_candidate<AABB_FORMULA>(
+                b, g, qx, qy, qz, R, true,
+                [&](uint32_t j, float, float, float, float r_sq, int code, const float4&, uint32_t slot) {
+                    if (!(r_sq < rlim_sq))
+                        return;
+                    if (exclude_ii && j == i)
+                        return;
+                    if (!(r_sq >= rmin_thr))
+                        return;
+                    ++found;
+                    if (!(r_sq < stage_thr && r_sq <= cut))
+                        return;
+                    if (c == C)
+                    {
+                        reduce();
+                        if (!(r_sq <= cut))
+                            return;
+                    }
+                    s_e[c * kBlock + tid] = (slot << 5) | uint32_t(code);
+                    ++c;
+                    in1 += r_sq < c1;
+                    in2 += r_sq < c2;
+                    in3 += r_sq < c3;
+                });
+            float const complete = float(R) * w1;
+            uint32_t const in = R == 1 ? in1 : R == 2 ? in2 : R == 3 ? in3 : 0u;
+            if (c >= k && in >= k)
+                break;
+            if (c < k && found > c && complete >= r_stage)
+            {
+                hand_back = true; // the answer holds candidates at or above r_stage
+                break;
+            }
+            if (complete >= rlim)
+                break;
+            if ((complete >= half_min_plane && rlim > half_min_plane) || R == 3)
+            {
+                hand_back = true; // longer walks, and the exhaustive flag, are the ring search's
+                break;
             }
         }
-        c = w;
-        float const d = knn_key_distance(kth);
-        cut = d * d * (1.0f + 0x1p-20f) + 1e-37f;
-    };
-    int R = 0;
-    for (; R <= rmax_ring; ++R)
-    {
-        fa::for_each_candidate<AABB_FORMULA>(
-            b, g, qx, qy, qz, R, true,
-            [&](uint32_t j, float, float, float, float r_sq, int code, const float4&, uint32_t slot) {
-                if (!(r_sq < rlim_sq))
-                    return;
-                if (exclude_ii && j == i)
-                    return;
-                if (!(r_sq >= rmin_thr))
-                    return;
-                ++found;
-                if (!(r_sq < stage_thr && r_sq <= cut))
-                    return;
-                if (c == C)
-                {
-                    reduce();
-                    if (!(r_sq <= cut))
-                        return;
-                }
-                s_e[c * kBlock + tid] = (slot << 5) | uint32_t(code);
-                ++c;
-                in1 += r_sq < c1;
-                in2 += r_sq < c2;
-                in3 += r_sq < c3;
-            });
-        float const complete = float(R) * w1;
-        uint32_t const in = R == 1 ? in1 : R == 2 ? in2 : R == 3 ? in3 : 0u;
-        if (c >= k && in >= k)
-            break;
-        if (c < k && found > c && complete >= r_stage)
+        uint32_t const n = min(found, k);
+        if (hand_back || c < n)
+            refine[atomicAdd(nrefine, 1u)] = p;
+        else
         {
-            hand_back = true; // the answer holds candidates at or above r_stage
-            break;
-        }
-        if (complete >= rlim)
-            break;
-        if ((complete >= half_min_plane && rlim > half_min_plane) || R == 3)
-        {
-            hand_back = true; // longer walks, and the exhaustive flag, are the ring search's
-            break;
+            // c >= n: the n smallest keys are staged
+            float r_sq;
+            for (uint32_t m = 0; m < c; ++m)
+                best.insert(key_of(s_e[m * kBlock + tid], r_sq));
+            float const last = n ? knn_key_distance(best.at(n - 1)) : 0.0f;
+            // two images of one point are at least a plane distance apart (see the ring search)
+            if (rlim > half_min_plane && n > 0 && last >= half_min_plane)
+            {
+                flagged[atomicAdd(nflag, 1u)] = p;
+                counts[i] = 0;
+            }
+            else
+                counts[i] = n;
         }
     }
-    uint32_t const n = min(found, k);
-    if (hand_back || c < n)
-    {
-        refine[atomicAdd(nrefine, 1u)] = p;
-        return;
-    }
-    // c >= n: the n smallest keys are staged
-    float last = 0.0f;
-    float r_sq;
-    if (c <= k)
-    {
-        // n == c: every staged entry is selected
-        for (uint32_t m = 0; m < c; ++m)
-        {
-            uint64_t const key = key_of(s_e[m * kBlock + tid], r_sq);
-            last = fmaxf(last, knn_key_distance(key));
-            sel[uint64_t(p) * k + m] = key;
-        }
-    }
-    else
-    {
-        TopK<KMAX> best;
-        best.init();
-        for (uint32_t m = 0; m < c; ++m)
-            best.insert(key_of(s_e[m * kBlock + tid], r_sq));
-        for (uint32_t m = 0; m < n; ++m)
-            sel[uint64_t(p) * k + m] = best.at(m);
-        last = knn_key_distance(best.at(n - 1));
-    }
-    // two images of one point are at least a plane distance apart (see the ring search)
-    if (rlim > half_min_plane && n > 0 && last >= half_min_plane)
-    {
-        flagged[atomicAdd(nflag, 1u)] = p;
-        counts[i] = 0;
-        return;
-    }
-    counts[i] = n;
+    // The block's selection rows (positions p0 .. p0 + 255) are one contiguous
+    // range of `sel`: staged in LDS once the stage is free, then stored with
+    // coalesced writes.  Entries past a lane's count, and the rows of lanes
+    // handed back or flagged, carry no result (the searches that take those
+    // lanes rewrite their rows later on the stream; entries past a count are
+    // never read).
+    __syncthreads();
+    uint64_t* s_out = reinterpret_cast<uint64_t*>(fa_dyn_lds);
+#pragma unroll
+    for (int m = 0; m < KMAX; ++m)
+        if (uint32_t(m) < k)
+            s_out[tid * k + m] = best.v[m];
+    __syncthreads();
+    uint32_t const nrow = min(uint32_t(kBlock), nq - p0) * k;
+    for (uint32_t t = tid; t < nrow; t += kBlock)
+        sel[uint64_t(p0) * k + t] = s_out[t];
 }
 
 // LinkCell formula kNN (LinkCell.cc:575-679): every point in cells within
@@ -1411,7 +1413,8 @@ int launch_knn_aabb(const fa::DevBox& db, const DevGrid& g, const float* qpts, u
         FA_HIP(hipGetLastError());
         return FA_OK;
     }
-    size_t const lds = size_t(st.C) * kBlock * sizeof(uint32_t);
+    // the stage, and afterwards the block's selection rows
+    size_t const lds = std::max(size_t(st.C) * kBlock * sizeof(uint32_t), size_t(kBlock) * k * sizeof(uint64_t));
     FA_PROF_BEGIN("knn_stage_kernel");
     if (self)
         hipLaunchKernelGGL((knn_stage_kernel<KMAX, true>), dim3(blocks), dim3(kBlock), lds, fa::stream(), db, g, qpts,
