@@ -107,6 +107,9 @@ struct fa_batch
     float* d_wts = nullptr;
     float* d_vec = nullptr;
     void* d_gkeys = nullptr;
+    uint32_t ball_C = 0;        // keys per point the walk stages (0: count and fill)
+    uint32_t* d_pos = nullptr;  // total: where each point's keys start in d_rows
+    uint32_t* d_rows = nullptr; // ntiles * kTile * ball_C packed keys
 
     // RDF
     int rdf_R = 2;
@@ -315,7 +318,14 @@ int enqueue_ball(fa_batch* b)
 {
     if (int rc = fa::batch_grid_build(b->gb, b->d_tile_frame, b->ntiles, b->d_points, b->d_scan, b->scan_bytes))
         return rc;
-    if (int rc = fa::batch_ball_count(b->gb, b->d_tile_frame, b->ntiles, b->plan.ball_r_max, b->ball_R, b->d_counts))
+    if (b->ball_C)
+    {
+        if (int rc = fa::batch_ball_walk(b->gb, b->d_tile_frame, b->ntiles, b->plan.ball_r_max, b->ball_R, b->ball_C,
+                                         b->d_counts, b->d_pos, b->d_rows))
+            return rc;
+    }
+    else if (int rc = fa::batch_ball_count(b->gb, b->d_tile_frame, b->ntiles, b->plan.ball_r_max, b->ball_R,
+                                           b->d_counts))
         return rc;
     if (int rc = fa::exclusive_scan_u32_async(b->d_counts, b->d_offsets, b->total, b->d_scan, b->scan_bytes))
         return rc;
@@ -324,9 +334,16 @@ int enqueue_ball(fa_batch* b)
     if (b->total)
         FA_KLAUNCH(sum_counts_kernel, dim3(std::min<uint32_t>((b->total + 255) / 256, 1024)), dim3(256), 0, s,
                    b->d_counts, b->total, b->d_total_bonds);
-    if (int rc = fa::batch_ball_fill(b->gb, b->d_tile_frame, b->ntiles, b->d_points, b->plan.ball_r_max, b->ball_R,
-                                     b->d_offsets, b->d_total_bonds, b->capacity, b->d_nb, b->d_dist, b->d_wts,
-                                     b->d_vec, b->d_gkeys))
+    if (b->ball_C)
+    {
+        if (int rc = fa::batch_ball_emit(b->gb, b->d_tile_frame, b->ntiles, b->d_points, b->plan.ball_r_max,
+                                         b->ball_R, b->ball_C, b->d_pos, b->d_rows, b->d_offsets, b->d_total_bonds,
+                                         b->capacity, b->d_nb, b->d_dist, b->d_wts, b->d_vec, b->d_gkeys))
+            return rc;
+    }
+    else if (int rc = fa::batch_ball_fill(b->gb, b->d_tile_frame, b->ntiles, b->d_points, b->plan.ball_r_max,
+                                          b->ball_R, b->d_offsets, b->d_total_bonds, b->capacity, b->d_nb, b->d_dist,
+                                          b->d_wts, b->d_vec, b->d_gkeys))
         return rc;
     FA_KLAUNCH(frame_bonds_kernel, dim3((b->n_frames + 255) / 256), dim3(256), 0, s, b->gb.frames, b->n_frames,
                b->d_offsets, b->d_frame_bonds);
@@ -513,6 +530,8 @@ fa_batch::~fa_batch()
     fa::device_free(d_wts);
     fa::device_free(d_vec);
     fa::device_free(d_gkeys);
+    fa::device_free(d_pos);
+    fa::device_free(d_rows);
     fa::device_free(rs.nclus);
     fa::device_free(rs.table);
     fa::device_free(d_rdf);
@@ -641,6 +660,29 @@ int fa_batch_create(uint32_t n_frames, const fa_box* boxes, const uint64_t* poin
         }
         if (int rc = alloc_bonds(b, uint64_t(est * 1.25) + 1024))
             return fail_out(rc);
+        // keys the one-walk query stages per point: enough for the densest
+        // frame (none when a frame needs more than the 32 an LDS stage holds)
+        uint32_t C = 0;
+        for (uint32_t f = 0; f < n_frames; ++f)
+        {
+            double const v = std::max(double(fa::box_volume(boxes[f])), 1e-30);
+            uint32_t const c = fa::batch_ball_slots(double(point_offsets[f + 1] - point_offsets[f]) / v,
+                                                    boxes[f].is2D, float(r));
+            if (c == 0)
+            {
+                C = 0;
+                break;
+            }
+            C = std::max(C, c);
+        }
+        if (C && b->ntiles)
+        {
+            if (int rc = fa::device_alloc((void**) &b->d_pos, sizeof(uint32_t) * np))
+                return fail_out(rc);
+            if (int rc = fa::device_alloc((void**) &b->d_rows, sizeof(uint32_t) * size_t(b->ntiles) * fa::kTile * C))
+                return fail_out(rc);
+            b->ball_C = C;
+        }
     }
     *out = b;
     return FA_OK;

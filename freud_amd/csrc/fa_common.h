@@ -730,6 +730,18 @@ int batch_ball_count(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t 
 int batch_ball_fill(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, float r_max,
                     int R, const uint32_t* d_offsets, const unsigned long long* d_total, uint64_t capacity,
                     uint32_t* nb, float* dist, float* wts, float* vec, void* gkeys);
+// The ball query in one walk: the walk records each point's bond count
+// (d_counts), its first C keys packed per tile into d_rows (ntiles * kTile * C
+// slots) and where they start (d_pos); the emission (after the counts are
+// scanned into d_offsets) sorts and writes them, walking again only points
+// past C bonds.  C from batch_ball_slots (0: use count and fill instead).
+uint32_t batch_ball_slots(double max_points_per_volume, bool is2D, float r_max);
+int batch_ball_walk(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, float r_max, int R, uint32_t C,
+                    uint32_t* d_counts, uint32_t* d_pos, uint32_t* d_rows);
+int batch_ball_emit(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, float r_max,
+                    int R, uint32_t C, const uint32_t* d_pos, const uint32_t* d_rows, const uint32_t* d_offsets,
+                    const unsigned long long* d_total, uint64_t capacity, uint32_t* nb, float* dist, float* wts,
+                    float* vec, void* gkeys);
 struct BatchRdfScratch
 {
     uint32_t* nclus;  // total_rows + 1

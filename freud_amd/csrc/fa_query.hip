@@ -29,6 +29,8 @@ using fa::LINKCELL_FORMULA;
 
 constexpr int kBlock = 256;
 constexpr int kKeyBytes = 32768; // largest LDS key staging per block
+// keys per point that the one-walk ball query stages in LDS (32 KB per block)
+constexpr uint32_t kWalkMaxSlots = kKeyBytes / (kBlock * sizeof(uint32_t));
 extern __shared__ __align__(16) unsigned char fa_dyn_lds[];
 
 // Dynamic LDS bytes for a fill block's keys: 1.3 x the mean bonds of kBlock
@@ -286,20 +288,21 @@ __device__ inline void ball_fill_body(const DevBox& b, const DevGrid& g, const f
 // full-line stores (written in cell order they would land as ~13-bond
 // segments scattered over partly written lines).  A point with more than C
 // bonds walks its stencil again inside the emission pass.
+// One block's share of the walk: processing positions [p0, p0 + nloc), keys
+// packed into rows[region ..] (region: the block's kBlock * C slots); counts
+// and pos are indexed by point index (a frame of a batch passes them advanced
+// to its first point; pos values are absolute slots of rows).
 template<int FORMULA, bool SELF>
-__global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                           uint32_t nq, float r_max_sq, float r_min_sq,
-                                                           int exclude_ii, int R, fa::CQGrid cq, uint32_t C,
-                                                           uint32_t* __restrict__ counts, uint32_t* __restrict__ pos,
-                                                           uint32_t* __restrict__ rows)
+__device__ inline void ball_walk_body(const DevBox& b, const DevGrid& g, const float* __restrict__ qpts, uint32_t p0,
+                                      uint32_t nloc, uint32_t region, float r_max_sq, float r_min_sq,
+                                      int exclude_ii, int R, const fa::CQGrid& cq, uint32_t C,
+                                      uint32_t* __restrict__ counts, uint32_t* __restrict__ pos,
+                                      uint32_t* __restrict__ rows)
 {
     // s_rows[m * kBlock + tid]: key m of thread tid (consecutive lanes, consecutive banks)
     uint32_t* s_rows = reinterpret_cast<uint32_t*>(fa_dyn_lds);
     __shared__ uint32_t s_off[kBlock + 1];
     __shared__ uint32_t s_tmp[kBlock];
-    uint32_t const tile = fa::xcd_tile(blockIdx.x, gridDim.x);
-    uint32_t const p0 = tile * kBlock;
-    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
     uint32_t const tid = threadIdx.x;
     uint32_t c = 0, i = 0;
     if (tid < nloc)
@@ -319,7 +322,6 @@ __global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, 
     uint32_t const kept = min(c, C);
     uint32_t excl;
     uint32_t const total = block_exclusive_scan(kept, s_tmp, excl);
-    uint32_t const region = tile * kBlock * C; // this block's slots in rows
     if (tid < nloc)
     {
         s_off[tid] = excl;
@@ -344,6 +346,19 @@ __global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, 
     }
 }
 
+template<int FORMULA, bool SELF>
+__global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, float r_max_sq, float r_min_sq,
+                                                           int exclude_ii, int R, fa::CQGrid cq, uint32_t C,
+                                                           uint32_t* __restrict__ counts, uint32_t* __restrict__ pos,
+                                                           uint32_t* __restrict__ rows)
+{
+    uint32_t const tile = fa::xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t const p0 = tile * kBlock;
+    ball_walk_body<FORMULA, SELF>(b, g, qpts, p0, min(uint32_t(kBlock), nq - p0), tile * kBlock * C, r_max_sq,
+                                  r_min_sq, exclude_ii, R, cq, C, counts, pos, rows);
+}
+
 // Emission: with INDEX, query points [p0, p0 + 256) of the caller's order,
 // whose bonds are one contiguous range; otherwise the walk's processing order
 // (the block's keys are its own packed region, and its neighbours' positions
@@ -351,16 +366,18 @@ __global__ __launch_bounds__(kBlock) void ball_walk_kernel(DevBox b, DevGrid g, 
 // Keys come from the point's row (or, past C bonds, from a second walk), are
 // sorted per point (by j and image, or by distance), and the block writes its
 // bonds.
+// One block's share of the emission: query positions [p0, p0 + nloc) (point
+// indices with INDEX, else processing positions).  Point indices are relative
+// to qpts, pts, pos and offsets (a frame of a batch passes them advanced to its
+// first point); bond indices and rows slots are absolute.
 template<int FORMULA, bool BYDIST, bool SELF, bool INDEX>
-__global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
-                                                           uint32_t nq, const float* __restrict__ pts,
-                                                           float r_max_sq, float r_min_sq, int exclude_ii, int R,
-                                                           fa::CQGrid cq, uint32_t C, const uint32_t* __restrict__ pos,
-                                                           const uint32_t* __restrict__ rows,
-                                                           const uint32_t* __restrict__ offsets,
-                                                           uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
-                                                           float* __restrict__ w_out, float* __restrict__ v_out,
-                                                           void* __restrict__ gkeys_raw, uint32_t key_cap)
+__device__ inline void ball_emit_body(const DevBox& b, const DevGrid& g, const float* __restrict__ qpts, uint32_t p0,
+                                      uint32_t nloc, const float* __restrict__ pts, float r_max_sq, float r_min_sq,
+                                      int exclude_ii, int R, const fa::CQGrid& cq, uint32_t C,
+                                      const uint32_t* __restrict__ pos, const uint32_t* __restrict__ rows,
+                                      const uint32_t* __restrict__ offsets, uint32_t* __restrict__ nb_out,
+                                      float* __restrict__ d_out, float* __restrict__ w_out,
+                                      float* __restrict__ v_out, void* __restrict__ gkeys_raw, uint32_t key_cap)
 {
     using K = typename std::conditional<BYDIST, uint64_t, uint32_t>::type;
     K* s_keys = reinterpret_cast<K*>(fa_dyn_lds);
@@ -369,8 +386,6 @@ __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, 
     __shared__ uint32_t s_tmp[kBlock];
     __shared__ uint32_t s_row[kBlock];
     __shared__ float s_q[3 * kBlock];
-    uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
-    uint32_t const nloc = min(uint32_t(kBlock), nq - p0);
     uint32_t const tid = threadIdx.x;
     uint32_t i = p0 + tid;
     float qx = 0.0f, qy = 0.0f, qz = 0.0f;
@@ -468,8 +483,25 @@ __global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, 
     }
     __syncthreads();
     if (in_lds)
-        store_block_bonds<FORMULA, BYDIST, K, INDEX>(b, s_q, pts, keys, total, nloc, s_off, s_i, offsets, nb_out, d_out,
-                                           w_out, v_out);
+        store_block_bonds<FORMULA, BYDIST, K, INDEX>(b, s_q, pts, keys, total, nloc, s_off, s_i, offsets, nb_out,
+                                                     d_out, w_out, v_out);
+}
+
+template<int FORMULA, bool BYDIST, bool SELF, bool INDEX>
+__global__ __launch_bounds__(kBlock) void ball_emit_kernel(DevBox b, DevGrid g, const float* __restrict__ qpts,
+                                                           uint32_t nq, const float* __restrict__ pts,
+                                                           float r_max_sq, float r_min_sq, int exclude_ii, int R,
+                                                           fa::CQGrid cq, uint32_t C, const uint32_t* __restrict__ pos,
+                                                           const uint32_t* __restrict__ rows,
+                                                           const uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ nb_out, float* __restrict__ d_out,
+                                                           float* __restrict__ w_out, float* __restrict__ v_out,
+                                                           void* __restrict__ gkeys_raw, uint32_t key_cap)
+{
+    uint32_t const p0 = fa::xcd_tile(blockIdx.x, gridDim.x) * kBlock;
+    ball_emit_body<FORMULA, BYDIST, SELF, INDEX>(b, g, qpts, p0, min(uint32_t(kBlock), nq - p0), pts, r_max_sq,
+                                                 r_min_sq, exclude_ii, R, cq, C, pos, rows, offsets, nb_out, d_out,
+                                                 w_out, v_out, gkeys_raw, key_cap);
 }
 
 // Segments with the reference's convention (0 for query points without bonds).
@@ -1096,6 +1128,52 @@ __global__ __launch_bounds__(kBlock) void bball_fill_kernel(fa::BatchGrid G, con
                                               gkeys, key_cap);
 }
 
+// The batched one-walk ball query (tile t of the batch, its frame's grid and
+// cell order; see ball_walk_body and ball_emit_body).  The emission runs in the
+// walk's processing order, where each tile's keys are one packed region and
+// its neighbours' positions stay in cache (the frames' points may be stored in
+// any order).
+__global__ __launch_bounds__(kBlock) void bball_walk_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                            uint32_t ntiles, float r_max_sq, int R, uint32_t C,
+                                                            uint32_t* __restrict__ counts, uint32_t* __restrict__ pos,
+                                                            uint32_t* __restrict__ rows)
+{
+    uint32_t const t = fa::xcd_tile(blockIdx.x, ntiles);
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const local0 = (t - F.tile_base) * fa::kTile;
+    DevGrid const g = fa::frame_grid(F, G);
+    ball_walk_body<AABB_FORMULA, true>(F.box, g, nullptr, F.pt_base + local0, min(fa::kTile, F.n - local0),
+                                       t * fa::kTile * C, r_max_sq, 0.0f, 1, R, fa::CQGrid{}, C, counts + F.pt_base,
+                                       pos + F.pt_base, rows);
+}
+
+__global__ __launch_bounds__(kBlock) void bball_emit_kernel(fa::BatchGrid G, const uint32_t* __restrict__ tile_frame,
+                                                            uint32_t ntiles, const float* __restrict__ pts,
+                                                            float r_max_sq, int R, uint32_t C,
+                                                            const uint32_t* __restrict__ pos,
+                                                            const uint32_t* __restrict__ rows,
+                                                            const uint32_t* __restrict__ offsets,
+                                                            const unsigned long long* __restrict__ total_bonds,
+                                                            uint64_t capacity, uint32_t* __restrict__ nb_out,
+                                                            float* __restrict__ d_out, float* __restrict__ w_out,
+                                                            float* __restrict__ v_out, void* __restrict__ gkeys,
+                                                            uint32_t key_cap)
+{
+    // a run whose bonds exceed the allocation writes nothing (the host sees
+    // the 64-bit total and runs again with room for it)
+    if (*total_bonds > capacity)
+        return;
+    uint32_t const t = fa::xcd_tile(blockIdx.x, ntiles);
+    const fa::BatchFrame& F = G.frames[tile_frame[t]];
+    uint32_t const local0 = (t - F.tile_base) * fa::kTile;
+    DevGrid const g = fa::frame_grid(F, G);
+    const float* fp = pts + 3 * size_t(F.pt_base);
+    ball_emit_body<AABB_FORMULA, false, true, false>(F.box, g, fp, F.pt_base + local0, min(fa::kTile, F.n - local0),
+                                                     fp, r_max_sq, 0.0f, 1, R, fa::CQGrid{}, C, pos + F.pt_base, rows,
+                                                     offsets + F.pt_base, nb_out, d_out, w_out, v_out, gkeys,
+                                                     key_cap);
+}
+
 } // namespace
 
 namespace fa {
@@ -1124,6 +1202,41 @@ int batch_ball_fill(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t n
     return FA_OK;
 }
 
+
+uint32_t batch_ball_slots(double max_points_per_volume, bool is2D, float r_max)
+{
+    double const r = double(r_max);
+    double const expect = is2D ? max_points_per_volume * M_PI * r * r
+                               : max_points_per_volume * (4.0 / 3.0) * M_PI * r * r * r;
+    double const c = std::ceil((expect * 1.15 + 4.0) / 4.0) * 4.0;
+    if (c > double(kWalkMaxSlots))
+        return 0;
+    return uint32_t(std::max(c, 8.0));
+}
+
+int batch_ball_walk(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, float r_max, int R, uint32_t C,
+                    uint32_t* d_counts, uint32_t* d_pos, uint32_t* d_rows)
+{
+    if (ntiles)
+        FA_KLAUNCH(bball_walk_kernel, dim3(ntiles), dim3(kBlock), size_t(C) * kBlock * sizeof(uint32_t), stream(), G,
+                   d_tile_frame, ntiles, r_max * r_max, R, C, d_counts, d_pos, d_rows);
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
+
+int batch_ball_emit(const BatchGrid& G, const uint32_t* d_tile_frame, uint32_t ntiles, const float* d_pts, float r_max,
+                    int R, uint32_t C, const uint32_t* d_pos, const uint32_t* d_rows, const uint32_t* d_offsets,
+                    const unsigned long long* d_total, uint64_t capacity, uint32_t* nb, float* dist, float* wts,
+                    float* vec, void* gkeys)
+{
+    size_t const kb = fill_key_bytes(capacity, G.total_points, sizeof(uint32_t));
+    if (ntiles)
+        FA_KLAUNCH(bball_emit_kernel, dim3(ntiles), dim3(kBlock), kb, stream(), G, d_tile_frame, ntiles, d_pts,
+                   r_max * r_max, R, C, d_pos, d_rows, d_offsets, d_total, capacity, nb, dist, wts, vec, gkeys,
+                   uint32_t(kb / sizeof(uint32_t)));
+    FA_HIP(hipGetLastError());
+    return FA_OK;
+}
 
 int nlist_reserve(fa_nlist* nl, uint64_t nb, uint32_t nq)
 {
@@ -1458,7 +1571,6 @@ int launch_knn_lc(const fa::DevBox& db, const DevGrid& g, const float* qpts, uin
 // again by the emission pass.  When that exceeds 32 (a 32 KB LDS stage per
 // block) the walk only counts (0: every point is walked again, in processing
 // order, as a count pass and a fill pass would).
-constexpr uint32_t kWalkMaxSlots = kKeyBytes / (kBlock * sizeof(uint32_t));
 uint32_t ball_row_slots(const fa_nq* nq, float r_max)
 {
     double const vol = std::max(double(fa::box_volume(nq->box)), 1e-30);
