@@ -683,7 +683,7 @@ __global__ __launch_bounds__(kBlock) void knn_aabb_kernel(DevBox b, DevGrid g, c
 // the selection moved out of the walk.  Inserting into a register top-k
 // during the walk costs every lane of a wave the insertion whenever one lane
 // inserts, which is nearly every candidate.  Here a candidate with
-// d < r_stage (a radius holding ~k + 2 sqrt(k) + 2 points at the system's
+// d < r_stage (a radius holding ~k + sqrt(k) + 1 points at the system's
 // mean density; stage_thr is the smallest r_sq whose sqrtf reaches it) is
 // only recorded in the lane's LDS stage, as its cell-order slot and image
 // code (4 bytes, so the stage leaves room for several blocks per CU); the k
@@ -1478,8 +1478,10 @@ float sqrt_threshold(float r)
 }
 
 // Staged-search parameters for k neighbours at number density dens: the
-// stage radius holds ~k + 2 sqrt(k) + 2 points on average and the stage
-// ~2 standard deviations more (at most 32 entries of 4 bytes per lane).
+// stage radius holds ~k + sqrt(k) + 1 points on average (regular points, for
+// which the staged search is chosen, rarely have fewer than k inside it; each
+// staged point beyond k costs a selection step) and the stage ~2 standard
+// deviations more (at most 32 entries of 4 bytes per lane).
 struct KnnStage
 {
     float thr = 0.0f; // smallest r_sq whose sqrtf reaches the stage radius
@@ -1491,7 +1493,7 @@ KnnStage knn_stage_params(bool is2D, double dens, uint32_t k)
     KnnStage st;
     if (k > 16 || !(dens > 0.0))
         return st;
-    double const lam = k + 2.0 * std::sqrt(double(k)) + 2.0;
+    double const lam = k + std::sqrt(double(k)) + 1.0;
     double const r = is2D ? std::sqrt(lam / (M_PI * dens)) : std::cbrt(3.0 * lam / (4.0 * M_PI * dens));
     uint32_t C = uint32_t(std::ceil((lam + 2.0 * std::sqrt(lam)) / 4.0)) * 4;
     C = std::min<uint32_t>(32, std::max<uint32_t>(C, k + 4));
