@@ -118,6 +118,26 @@ def test_batch_bond_overflow_reruns(oracle):
     _check(b, frames, oracle)
 
 
+def test_batch_dense_ball_query(oracle):
+    """A ball radius at which the frames expect more bonds per point than the
+    walk's key stage holds (r_max 2.2 at density ~1: ~45 bonds): the batch
+    counts and fills instead of staging keys.  NeighborLists against the
+    oracle, eager and graph-replayed."""
+    r = 2.2
+    frames = _frames(seed=5)
+    b = TrajectoryBatch(frames, ball_r_max=r, rdf_bins=BINS, rdf_r_max=R_RDF, l=L_Q)
+    for use_graph in (False, True):
+        b.compute(use_graph=use_graph)
+        nb = b.num_bonds
+        for f, (box, pts) in enumerate(frames):
+            ref = oracle.query(_bt(box), pts, pts, dict(r_max=r, exclude_ii=True), nthreads=8)
+            nl = b.neighbor_list(f)
+            npt.assert_array_equal(nl[:], ref["ij"], err_msg=f"frame {f}")
+            npt.assert_array_equal(nl.distances, ref["distances"], err_msg=f"frame {f}")
+            npt.assert_array_equal(nl.vectors, ref["vectors"], err_msg=f"frame {f}")
+            assert int(nb[f]) == len(ref["ij"])
+
+
 def test_batch_matches_single_frame_api():
     frames = _frames(seed=7)
     b = _make(frames).compute()
