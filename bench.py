@@ -673,17 +673,126 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def run_filters(args):
+    """FilterSANN and FilterRAD (closed and open) over the ball NeighborList
+    (r_max 1.9: ~26 candidates per point, 27M bonds) of the 1M-point frame.
+    A step runs the three filters; value = candidate bonds filtered per second."""
+    from freud_amd import _lib
+    from freud_amd._lib import DeviceArray
+
+    L, lib, check = _lib, _lib.lib(), _lib.check
+    box, pts = make_workload()
+    n = len(pts)
+    d_pts = DeviceArray.from_numpy(pts)
+    cb = L.make_box(box)
+    nl = L.fa_nlist()
+    h = ctypes.c_void_p()
+    check(lib.fa_nq_create(ctypes.byref(cb), d_pts.ptr, n, L.NQ_AABB, 0.0, ctypes.byref(h)))
+    try:
+        ball = L.fa_query_args(L.MODE_BALL, 0xFFFFFFFF, 1.9, 0.0, -1.0, -1.0, 1)
+        check(lib.fa_query(h, d_pts.ptr, n, ctypes.byref(ball), 0, ctypes.byref(nl)))
+    finally:
+        lib.fa_nq_destroy(h)
+    nb = int(nl.num_bonds)
+    outs = [L.fa_nlist() for _ in range(3)]
+    d_unf = DeviceArray((n,), np.uint8)
+    nu = ctypes.c_uint32()
+
+    def step():
+        check(lib.fa_filter_sann(ctypes.byref(nl), ctypes.byref(outs[0]), d_unf.ptr, ctypes.byref(nu)))
+        for closed, out in ((1, outs[1]), (0, outs[2])):
+            check(lib.fa_filter_rad(ctypes.byref(cb), d_pts.ptr, n, d_pts.ptr, None, ctypes.byref(nl), closed,
+                                    ctypes.byref(out), d_unf.ptr, ctypes.byref(nu)))
+
+    for w in range(max(args.warmup, 1)):
+        step()
+        L.synchronize()
+    L.profile_reset()
+    L.profile_enable(True)
+    L.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    L.synchronize()
+    t1 = time.perf_counter()
+    L.profile_enable(False)
+    stats = L.profile_stats()
+    ms = (t1 - t0) * 1e3 / args.steps
+    kept = [int(o.num_bonds) for o in outs]
+    print(f"[bench] filters {ms:.3f} ms/step over {nb} candidate bonds: kept SANN {kept[0]}, RAD-closed {kept[1]}, "
+          f"RAD-open {kept[2]}", file=sys.stderr, flush=True)
+    kernels = {}
+    for name, (cnt, tot) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        kernels[name] = {"launches_per_step": cnt / args.steps, "avg_ms": round(tot / cnt, 4),
+                         "share": round(tot / (ms * args.steps), 4)}
+    dom = max(stats.items(), key=lambda kv: kv[1][1])[0]
+    cnt, tot = stats[dom]
+    avg_ms = tot / cnt
+    # compulsory bytes of a count pass: per candidate bond (i, j) 8 + d 4 + w 4
+    # + its 2-byte code, and for RAD the point p_j 12; per query point its
+    # offset 4, kept count 4, unfilled byte 1, and for RAD q_i 12
+    rad = "true" in dom
+    ab = nb * (8 + 4 + 4 + 2 + (12 if rad else 0)) + n * (4 + 4 + 1 + (12 if rad else 0))
+    achieved = ab / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom, "avg_ms": round(avg_ms, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the oracle (one core, its distance sort included) on the lists of the first 32,768 query points
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import api as oracle
+
+        m = min(32768, n)
+        end = int(L.copy_d2h(nl.offsets + 4 * m, (1,), np.uint32)[0])
+        sample = {"ij": L.copy_d2h(nl.neighbors, (end, 2), np.uint32),
+                  "distances": L.copy_d2h(nl.distances, (end,), np.float32),
+                  "weights": L.copy_d2h(nl.weights, (end,), np.float32),
+                  "vectors": L.copy_d2h(nl.vectors, (end, 3), np.float32)}
+        bt = (box.Lx, box.Ly, box.Lz, box.xy, box.xz, box.yz, box.is2D)
+        c0 = time.perf_counter()
+        oracle.filter_sann(sample, m)
+        oracle.filter_rad(bt, pts, pts[:m], sample, terminate_after_blocked=True)
+        oracle.filter_rad(bt, pts, pts[:m], sample, terminate_after_blocked=False)
+        c1 = time.perf_counter()
+        cpu = {"value": 3 * end / (c1 - c0), "unit": "bonds/s", "cores": 1, "kind": "port",
+               "sample": f"the three filters over the {end} candidate bonds of the first {m} query points"}
+    out = {
+        "metric": "candidate bonds filtered/s (FilterSANN + FilterRAD closed + open)",
+        "value": 3 * nb / (ms / 1e3),
+        "unit": "bonds/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 in the SANN test)",
+        "data": "synthetic (seeded jittered FCC)",
+        "config": {"workload": "NeighborList filters: N=1048576 jittered FCC (a=1.5874, sigma 0.05a), AABB ball "
+                               "query r_max=1.9 excluding i=j; FilterSANN, FilterRAD closed, FilterRAD open",
+                   "n_points": n, "n_candidate_bonds": nb, "kept": {"sann": kept[0], "rad_closed": kept[1],
+                                                                     "rad_open": kept[2]},
+                   "kernels": kernels},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    for o in outs + [nl]:
+        lib.fa_nlist_free(ctypes.byref(o))
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("frame", "batched", "stress", "learned"), default="frame",
+    ap.add_argument("--workload", choices=("frame", "batched", "stress", "learned", "filters"), default="frame",
                     help="frame: one 1M-point frame (the headline line); batched: a 256-frame trajectory of uneven "
                          "frames, batched + graph-replayed against the per-frame loop; stress: 500k triclinic "
                          "(l = 4..12) + 250k 2D hexagonal (Hexatic); learned: 2M-particle two-phase SolidLiquid, "
-                         "exact vs learned")
+                         "exact vs learned; filters: FilterSANN / FilterRAD over the 1M-point frame's r=1.9 list")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -702,6 +811,8 @@ def main():
         return run_stress(args)
     if args.workload == "learned":
         return run_learned(args)
+    if args.workload == "filters":
+        return run_filters(args)
     if args.copy_mib > 0:
         nbytes = args.copy_mib << 20
         src = DeviceArray((nbytes // 4,), np.uint32)

@@ -164,6 +164,11 @@ _SIGS = {
     "fa_nlist_filter_r": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_float, ctypes.c_float]),
     "fa_nlist_sort": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.c_int]),
     "fa_nlist_copy": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.POINTER(fa_nlist)]),
+    "fa_filter_sann": (ctypes.c_int, [ctypes.POINTER(fa_nlist), ctypes.POINTER(fa_nlist), ctypes.c_void_p,
+                                      ctypes.POINTER(ctypes.c_uint32)]),
+    "fa_filter_rad": (ctypes.c_int, [ctypes.POINTER(fa_box), ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.POINTER(fa_nlist), ctypes.c_int, ctypes.POINTER(fa_nlist),
+                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]),
     "fa_rdf_accumulate_query": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float,
                                                ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_float,
                                                ctypes.c_float, ctypes.c_void_p]),

@@ -13,7 +13,7 @@ import numpy as np
 import freud_amd.box
 from freud_amd import _lib
 from freud_amd._lib import DeviceArray, check, lib
-from freud_amd.util import NO_DEFAULT_QUERY_ARGS_MESSAGE, _Compute, _convert_array, _convert_box
+from freud_amd.util import NO_DEFAULT_QUERY_ARGS_MESSAGE, _Compute, _convert_array, _convert_box, computed_property
 
 _DEFAULT_K = 0xFFFFFFFF
 
@@ -542,3 +542,128 @@ class _SpatialHistogram(_PairCompute):
 
 class _SpatialHistogram1D(_SpatialHistogram):
     pass
+
+
+class Filter(_PairCompute):
+    """Filter an existing NeighborList (freud/locality.py:1271-1336).
+
+    ``compute`` builds the unfiltered list from ``neighbors`` (a NeighborList,
+    query arguments, or every pair when None) and keeps the subset of its bonds
+    the derived filter selects; ``filtered_nlist`` is sorted by distance.  The
+    filtering runs on the device (fa_filter_sann / fa_filter_rad).
+    """
+
+    def __init__(self):
+        msg = "The Filter class is abstract and should not be instantiated directly."
+        raise RuntimeError(msg)
+
+    def _setup_filter(self, allow_incomplete_shell):
+        _Compute.__init__(self)
+        self._allow_incomplete_shell = bool(allow_incomplete_shell)
+        self._filtered = None
+        self._unfiltered = None
+
+    def _preprocess_arguments(self, system, query_points=None, neighbors=None):
+        """Every pair (without i == i for self queries) when neighbors is None."""
+        nq = NeighborQuery.from_system(system)
+        if neighbors is None:
+            neighbors = NeighborList.all_pairs(nq, query_points, query_points is None)
+        return super()._preprocess_arguments(nq, query_points, neighbors)
+
+    def compute(self, system, neighbors=None, query_points=None):
+        nq, nlist, qargs, d_q, num_query_points = self._preprocess_arguments(system, query_points, neighbors)
+        nlist = self._neighbor_list(nq, nlist, qargs, d_q, num_query_points)
+        self._filter(nq, nlist, d_q, None)
+        return self
+
+    def _run_filter(self, nlist, call):
+        """call(out, d_unfilled, n_unfilled) runs the C-ABI filter; then the
+        unfilled shells are reported as Filter.h:80-102 does."""
+        out = NeighborList()
+        nqp = nlist.num_query_points
+        d_unfilled = DeviceArray((max(nqp, 1),), np.uint8)
+        n_unfilled = ctypes.c_uint32()
+        check(call(out, d_unfilled, n_unfilled))
+        self._unfiltered = nlist
+        if n_unfilled.value:
+            idx = np.flatnonzero(d_unfilled.numpy()[:nqp])
+            msg = "Query point indices " + ", ".join(str(int(i)) for i in idx) + " do not have full neighbor shells."
+            if not self._allow_incomplete_shell:
+                raise RuntimeError(msg)
+            print("WARNING: " + msg)
+        self._filtered = out
+
+    @computed_property
+    def filtered_nlist(self):
+        """:class:`NeighborList`: the filtered neighbor list."""
+        return self._filtered
+
+    @computed_property
+    def unfiltered_nlist(self):
+        """:class:`NeighborList`: the unfiltered neighbor list."""
+        return self._unfiltered
+
+
+class FilterSANN(Filter):
+    """Solid Angle Nearest Neighbour filter (freud/locality.py:1339-1376,
+    FilterSANN.cc:22-104): per query point, the closest bonds whose solid
+    angles first sum to 4 pi; kept bonds keep their vectors and weights."""
+
+    def __init__(self, allow_incomplete_shell=False):
+        self._setup_filter(allow_incomplete_shell)
+
+    def _filter(self, nq, nlist, d_q, radii):
+        self._run_filter(nlist, lambda out, d_u, n_u: lib().fa_filter_sann(
+            ctypes.byref(nlist._c), ctypes.byref(out._c), d_u.ptr, ctypes.byref(n_u)))
+
+    def __repr__(self):
+        return f"freud_amd.locality.{type(self).__name__}(allow_incomplete_shell={self._allow_incomplete_shell})"
+
+
+class FilterRAD(Filter):
+    """Relative Angular Distance filter (freud/locality.py:1379-1502,
+    FilterRAD.cc:22-131): drops each neighbour j blocked by a closer k;
+    RAD-closed (terminate_after_blocked) also drops everything past the first
+    blocked neighbour.  Kept bonds carry box.wrap(q_i - p_j) and weight 1."""
+
+    def __init__(self, allow_incomplete_shell=False, terminate_after_blocked=True):
+        self._setup_filter(allow_incomplete_shell)
+        self._terminate_after_blocked = bool(terminate_after_blocked)
+
+    def compute(self, system, neighbors=None, query_points=None, points_radii=None, query_points_radii=None):
+        nq, nlist, qargs, d_q, num_query_points = self._preprocess_arguments(system, query_points, neighbors)
+        # freud/locality.py:1472-1496
+        if points_radii is None:
+            points_radii = np.ones(nq._n, dtype=np.float32)
+        points_radii = _convert_array(points_radii, shape=(nq._n,))
+        if query_points_radii is None:
+            query_points_radii = np.ones(num_query_points, dtype=np.float32)
+        query_points_radii = _convert_array(query_points_radii, shape=(num_query_points,))
+        if not np.all(np.isfinite(points_radii)):
+            msg = "points_radii must contain only finite values."
+            raise ValueError(msg)
+        if not np.all(points_radii > 0):
+            msg = "points_radii must contain only values greater than 0."
+            raise ValueError(msg)
+        if not np.all(np.isfinite(query_points_radii)):
+            msg = "query_points_radii must contain only finite values."
+            raise ValueError(msg)
+        if not np.all(query_points_radii > 0):
+            msg = "query_points_radii must contain only values greater than 0."
+            raise ValueError(msg)
+        nlist = self._neighbor_list(nq, nlist, qargs, d_q, num_query_points)
+        # the blocking test reads the radii of points only (FilterRAD.cc:26)
+        self._filter(nq, nlist, d_q, points_radii)
+        return self
+
+    def _filter(self, nq, nlist, d_q, radii):
+        d_r = None if radii is None or np.all(radii == 1) else DeviceArray.from_numpy(radii)
+        self._run_filter(nlist, lambda out, d_u, n_u: lib().fa_filter_rad(
+            ctypes.byref(nq._cbox), nq._d_points.ptr, nq._n, d_q.ptr, None if d_r is None else d_r.ptr,
+            ctypes.byref(nlist._c), int(self._terminate_after_blocked), ctypes.byref(out._c), d_u.ptr,
+            ctypes.byref(n_u)))
+
+    def __repr__(self):
+        return (f"freud_amd.locality.{type(self).__name__}(allow_incomplete_shell={self._allow_incomplete_shell}, "
+                f"terminate_after_blocked={self._terminate_after_blocked})")
+

@@ -78,12 +78,13 @@ class _Compute:
 
 
 def _check_shape(arr, shape):
+    # the messages of freud/util.py:126-139, which callers match on
     if len(shape) != arr.ndim:
-        raise ValueError(f"expected an array with {len(shape)} dimension(s), got shape {arr.shape}")
-    for axis, (want, have) in enumerate(zip(shape, arr.shape)):
+        raise ValueError(f"array.ndim = {arr.ndim}; expected ndim = {len(shape)}")
+    for want, have in zip(shape, arr.shape):
         if want is not None and want != have:
             expect = "(" + ", ".join("..." if s is None else str(s) for s in shape) + ")"
-            raise ValueError(f"axis {axis} of an array of shape {arr.shape} does not match the expected {expect}")
+            raise ValueError(f"array.shape= {arr.shape}; expected shape = {expect}")
 
 
 def _convert_array(array, shape=None, dtype=np.float32, requirements=("C",), allow_copy=True):

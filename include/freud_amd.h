@@ -223,6 +223,27 @@ int fa_nlist_sort(fa_nlist* nl, int by_distance);
 /* NeighborList::copy (NeighborList.cc:360-370) */
 int fa_nlist_copy(const fa_nlist* src, fa_nlist* dst);
 
+/* NeighborList filters over an existing list nl (left unchanged: the
+ * reference's unfiltered_nlist).  *out (a different list; zeroed or holding
+ * buffers of an earlier list) receives the kept bonds in (query point,
+ * distance, point, weight) order, with num_query_points / num_points one past
+ * the largest kept indices (NeighborList.cc:130-173; an empty result keeps
+ * nl's).  d_unfilled (nl->num_query_points device bytes) gets 1 for each query
+ * point whose shell the list could not fill and *n_unfilled their number: the
+ * reference raises or warns over these indices (Filter.h:80-102), which the
+ * caller does from the mask.
+ * FilterSANN::compute (freud/locality/FilterSANN.cc:22-104): kept bonds keep
+ * their vectors and weights. */
+int fa_filter_sann(const fa_nlist* nl, fa_nlist* out, uint8_t* d_unfilled, uint32_t* n_unfilled);
+/* FilterRAD::compute (freud/locality/FilterRAD.cc:22-131): d_points (n_points
+ * x 3) and d_query_points (nl->num_query_points x 3) are the positions nl
+ * indexes; d_points_radii (n_points, NULL = all 1) the per-point radii.  Kept
+ * bonds carry box.wrap(q_i - p_j) and weight 1.  terminate_after_blocked
+ * nonzero is RAD-closed, zero RAD-open. */
+int fa_filter_rad(const fa_box* box, const float* d_points, uint32_t n_points, const float* d_query_points,
+                  const float* d_points_radii, const fa_nlist* nl, int terminate_after_blocked, fa_nlist* out,
+                  uint8_t* d_unfilled, uint32_t* n_unfilled);
+
 /* ---------------------------------------------------------------- density
  * RDF::accumulate without a NeighborList (freud/density/RDF.cc:101-110 via
  * BondHistogramCompute::accumulateGeneral, BondHistogramCompute.h:115-126): a

@@ -39,6 +39,8 @@ __all__ = [
     "solidliquid",
     "cluster",
     "hexatic",
+    "filter_sann",
+    "filter_rad",
     "OracleError",
 ]
 
@@ -500,3 +502,54 @@ def bond_order(nl, p_quat, q_quat, mode, n_theta, n_phi, n_frames=1, counts=None
     lib.orc_bond_order_reduce(_p(counts), ctypes.c_uint32(n_theta), ctypes.c_uint32(n_phi),
                               ctypes.c_uint32(n_frames), _p(bo))
     return counts.reshape(n_theta, n_phi), bo.reshape(n_theta, n_phi)
+
+
+def _sorted_by_distance(nlist):
+    """NeighborList::sort(true) (NeighborList.cc:371-399): order (i, d, j, w)."""
+    ij = np.ascontiguousarray(nlist["ij"], dtype=np.uint32).reshape(-1, 2)
+    d = _f32(nlist["distances"])
+    w = _f32(nlist["weights"])
+    v = _f32(nlist["vectors"]).reshape(-1, 3)
+    order = np.lexsort((w, ij[:, 1], d, ij[:, 0]))
+    return (np.ascontiguousarray(ij[order]), np.ascontiguousarray(d[order]), np.ascontiguousarray(w[order]),
+            np.ascontiguousarray(v[order]))
+
+
+def _filtered(ij, d, w, v, keep, unfilled):
+    """The NeighborList the filters build from their bonds (NeighborList.cc:130-173):
+    num_query_points / num_points are one past the largest kept indices."""
+    keep = keep.astype(bool)
+    out = {"ij": ij[keep], "distances": d[keep], "weights": w[keep], "vectors": v[keep]}
+    if keep.any():
+        out["num_query_points"] = int(out["ij"][:, 0].max()) + 1
+        out["num_points"] = int(out["ij"][:, 1].max()) + 1
+    return out, unfilled.astype(bool)
+
+
+def filter_sann(nlist, num_query_points):
+    """FilterSANN.cc:22-104 -> (filtered nlist dict, unfilled mask per query point)."""
+    lib = _load()
+    ij, d, w, v = _sorted_by_distance(nlist)
+    keep = np.zeros(len(d), np.uint8)
+    unfilled = np.zeros(num_query_points, np.uint8)
+    lib.orc_filter_sann(ctypes.c_uint32(num_query_points), _p(ij), _p(d), ctypes.c_uint64(len(d)), _p(keep),
+                        _p(unfilled))
+    return _filtered(ij, d, w, v, keep, unfilled)
+
+
+def filter_rad(box, points, query_points, nlist, terminate_after_blocked=True, points_radii=None):
+    """FilterRAD.cc:22-131 -> (filtered nlist dict with vectors wrap(q_i - p_j) and
+    weights 1, unfilled mask per query point)."""
+    lib = _load()
+    b, is2d = _boxp(box)
+    pts = _f32(points).reshape(-1, 3)
+    qpts = _f32(query_points).reshape(-1, 3)
+    radii = None if points_radii is None else _f32(points_radii).reshape(-1)
+    ij, d, w, v = _sorted_by_distance(nlist)
+    keep = np.zeros(len(d), np.uint8)
+    vec = np.zeros((len(d), 3), np.float32)
+    unfilled = np.zeros(len(qpts), np.uint8)
+    lib.orc_filter_rad(_p(b), is2d, _p(pts), _p(qpts), None if radii is None else _p(radii),
+                       ctypes.c_uint32(len(qpts)), _p(ij), _p(d), ctypes.c_uint64(len(d)),
+                       int(bool(terminate_after_blocked)), _p(keep), _p(vec), _p(unfilled))
+    return _filtered(ij, d, np.ones_like(w), vec, keep, unfilled)

@@ -2050,3 +2050,99 @@ void orc_pmft3_reduce(int kind, const float* boxp, int is2D, const uint32_t* cou
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- NeighborList filters
+extern "C" {
+
+// FilterSANN.cc:22-104 over a NeighborList sorted by (query point, distance,
+// point) (NeighborList::sort(true), NeighborBond.h:97-112): keep[e] marks the
+// SANN bonds (the first 3 of each query point, then while the distance sum
+// over (m - 2), evaluated in double as the reference's 2.0 literal makes it,
+// exceeds the next distance); unfilled[i] = 1 where the list runs out first.
+void orc_filter_sann(uint32_t nq, const uint32_t* ij, const float* dist, uint64_t nb, uint8_t* keep,
+                     uint8_t* unfilled)
+{
+    uint64_t e = 0;
+    for (uint32_t i = 0; i < nq; ++i)
+    {
+        uint64_t const first = e;
+        while (e < nb && ij[2 * e] == i)
+            ++e;
+        uint32_t const n = uint32_t(e - first);
+        uint32_t m = 0;
+        float sum = 0.0f;
+        for (; m < 3 && m < n; ++m)
+        {
+            sum += dist[first + m];
+            keep[first + m] = 1;
+        }
+        while (m < n && (double(sum) / (double(float(m)) - 2.0)) > double(dist[first + m]))
+        {
+            sum += dist[first + m];
+            keep[first + m] = 1;
+            ++m;
+        }
+        for (uint32_t t = m; t < n; ++t)
+            keep[first + t] = 0;
+        unfilled[i] = m == n ? 1 : 0;
+    }
+}
+
+// FilterRAD.cc:22-131 over the same sorted list: neighbour j of query point i
+// is blocked by a closer k when
+//   r_j^2 |v2|^2 d_j d_k < r_k^2 (v1 . v2) |v1|^2,  v1 = wrap(q_i - p_j), v2 = wrap(q_i - p_k)
+// (products left to right as written there); kept bonds carry v1 (vec) and
+// weight 1.  terminate: RAD-closed (stop at the first blocked neighbour);
+// unfilled[i] = 1 when RAD-closed never met a blocked neighbour.  radii may
+// be null (all 1).
+void orc_filter_rad(const float* boxp, int is2D, const float* pts_, const float* qpts_, const float* radii,
+                    uint32_t nq, const uint32_t* ij, const float* dist, uint64_t nb, int terminate, uint8_t* keep,
+                    float* vec, uint8_t* unfilled)
+{
+    Box const b = make_box(boxp, is2D);
+    const v3* pts = reinterpret_cast<const v3*>(pts_);
+    const v3* qpts = reinterpret_cast<const v3*>(qpts_);
+    auto rad = [&](uint32_t j) { return radii ? radii[j] : 1.0f; };
+    uint64_t e = 0;
+    for (uint32_t i = 0; i < nq; ++i)
+    {
+        uint64_t const first = e;
+        while (e < nb && ij[2 * e] == i)
+            ++e;
+        uint32_t const n = uint32_t(e - first);
+        for (uint32_t t = 0; t < n; ++t)
+            keep[first + t] = 0;
+        bool good = true;
+        for (uint32_t j = 0; j < n; ++j)
+        {
+            uint32_t const pj = ij[2 * (first + j) + 1];
+            v3 const v1 = wrap(b, sub(qpts[i], pts[pj]));
+            good = true;
+            for (uint32_t k = 0; k < j; ++k)
+            {
+                uint32_t const pk = ij[2 * (first + k) + 1];
+                v3 const v2 = wrap(b, sub(qpts[i], pts[pk]));
+                float const rj = rad(pj), rk = rad(pk);
+                float const rj_sq = rj * rj, rk_sq = rk * rk;
+                if ((rj_sq * dot(v2, v2) * dist[first + j] * dist[first + k]) < (rk_sq * dot(v1, v2) * dot(v1, v1)))
+                {
+                    good = false;
+                    break;
+                }
+            }
+            if (good)
+            {
+                keep[first + j] = 1;
+                vec[3 * (first + j)] = v1.x;
+                vec[3 * (first + j) + 1] = v1.y;
+                vec[3 * (first + j) + 2] = v1.z;
+            }
+            else if (terminate)
+                break;
+        }
+        unfilled[i] = (good && terminate) ? 1 : 0;
+    }
+}
+
+} // extern "C"
+
