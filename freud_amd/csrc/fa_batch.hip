@@ -552,6 +552,7 @@ extern "C" {
 int fa_batch_create(uint32_t n_frames, const fa_box* boxes, const uint64_t* point_offsets, const fa_batch_plan* plan,
                     fa_batch** out)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (!out || !plan || (n_frames && (!boxes || !point_offsets)))
@@ -690,12 +691,14 @@ int fa_batch_create(uint32_t n_frames, const fa_box* boxes, const uint64_t* poin
 
 int fa_batch_destroy(fa_batch* b)
 {
+    FA_API_LOCK;
     delete b;
     return FA_OK;
 }
 
 int fa_batch_points(fa_batch* b, float** d_points)
 {
+    FA_API_LOCK;
     if (!b || !d_points)
         return fa::fail(FA_ERR_INVALID, "null argument");
     *d_points = b->d_points;
@@ -704,6 +707,7 @@ int fa_batch_points(fa_batch* b, float** d_points)
 
 int fa_batch_run(fa_batch* b, int use_graph)
 {
+    FA_API_LOCK;
     if (!b)
         return fa::fail(FA_ERR_INVALID, "null batch");
     if (b->total == 0)
@@ -748,6 +752,7 @@ int fa_batch_run(fa_batch* b, int use_graph)
 
 int fa_batch_num_bonds(fa_batch* b, uint64_t* per_frame)
 {
+    FA_API_LOCK;
     if (!b || !per_frame)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (!do_ball(b))
@@ -764,6 +769,7 @@ int fa_batch_num_bonds(fa_batch* b, uint64_t* per_frame)
 
 int fa_batch_nlist(fa_batch* b, uint32_t frame, fa_nlist* out)
 {
+    FA_API_LOCK;
     if (!b || !out)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (frame >= b->n_frames)
@@ -800,6 +806,7 @@ int fa_batch_nlist(fa_batch* b, uint32_t frame, fa_nlist* out)
 
 int fa_batch_rdf_counts(fa_batch* b, uint32_t** d_counts)
 {
+    FA_API_LOCK;
     if (!b || !d_counts)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (!do_rdf(b))
@@ -810,6 +817,7 @@ int fa_batch_rdf_counts(fa_batch* b, uint32_t** d_counts)
 
 int fa_batch_ql(fa_batch* b, float** d_ql, float** d_qlm, float* order)
 {
+    FA_API_LOCK;
     if (!b)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (!do_ql(b))
@@ -828,6 +836,7 @@ int fa_batch_ql(fa_batch* b, float** d_ql, float** d_qlm, float* order)
 
 int fa_batch_config(fa_batch* b, int* ball_R, int* rdf_R, int* rdf_F, uint64_t* capacity, int* graph_ready)
 {
+    FA_API_LOCK;
     if (!b)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (ball_R)

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -26,6 +27,14 @@ namespace fa {
 // std::invalid_argument, std::runtime_error, std::domain_error, std::out_of_range).
 void set_error(int code, const std::string& msg);
 int fail(int code, const std::string& msg);
+
+// Library calls share process-wide state (scratch buffers, the spatial-order
+// cache, the pinned read-back buffer, the autotuner table): every C-ABI entry
+// point holds this lock while it runs, so calls from several host threads run
+// one after another.  Recursive, as an entry point may call another.  GPU work
+// stays ordered on the library's one stream.
+std::recursive_mutex& api_mutex();
+#define FA_API_LOCK std::lock_guard<std::recursive_mutex> fa_api_lock_(::fa::api_mutex())
 
 #define FA_HIP(call)                                                                        \
     do                                                                                      \

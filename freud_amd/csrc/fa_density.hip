@@ -476,6 +476,7 @@ extern "C" {
 int fa_local_density(const fa_nlist* nl, float r_max, float diameter, int is2D, float* d_density,
                      float* d_num_neighbors)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // LocalDensity.cc:26-37
@@ -501,6 +502,7 @@ int fa_local_density(const fa_nlist* nl, float r_max, float diameter, int is2D, 
 int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, float x_max, float y_max,
                          uint32_t n_x, uint32_t n_y, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // PMFTXY.cc:25-41
@@ -562,6 +564,7 @@ int fa_pmftxy_accumulate(const fa_nlist* nl, const float* query_orientations, fl
 int fa_pmftxyt_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, float x_max,
                           float y_max, uint32_t n_x, uint32_t n_y, uint32_t n_t, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // PMFTXYT.cc:27-46
@@ -605,6 +608,7 @@ int fa_pmftxyt_accumulate(const fa_nlist* nl, const float* orientations, const f
 int fa_pmftr12_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, float r_max,
                           uint32_t n_r, uint32_t n_t1, uint32_t n_t2, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // PMFTR12.cc:29-46
@@ -638,6 +642,7 @@ int fa_pmftxyz_accumulate(const fa_nlist* nl, const float* query_orientations, c
                           uint32_t n_equiv, float x_max, float y_max, float z_max, uint32_t n_x, uint32_t n_y,
                           uint32_t n_z, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // PMFTXYZ.cc:22-48
@@ -674,6 +679,7 @@ int fa_pmftxyz_accumulate(const fa_nlist* nl, const float* query_orientations, c
 int fa_correlation_accumulate(const fa_nlist* nl, const double* values, const double* query_values, uint32_t bins,
                               float r_max, uint32_t* d_counts, double* d_sums)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // CorrelationFunction.cc:27-35
@@ -709,6 +715,7 @@ int fa_correlation_accumulate(const fa_nlist* nl, const double* values, const do
 int fa_bond_order_accumulate(const fa_nlist* nl, const float* orientations, const float* query_orientations, int mode,
                              uint32_t n_theta, uint32_t n_phi, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // BondOrder.cc:16-24
@@ -770,6 +777,7 @@ int fa_bond_order_accumulate(const fa_nlist* nl, const float* orientations, cons
 
 int fa_bond_order_reduce(const uint32_t* counts, uint32_t n_theta, uint32_t n_phi, uint32_t n_frames, float* bond_order)
 {
+    FA_API_LOCK;
     // BondOrder.cc:26-53 (surface areas with the host C library's cos) and :68-74
     volatile float dt = kTwoPi / float(n_theta);
     volatile float dp = M_PI / float(n_phi);
@@ -791,6 +799,7 @@ int fa_bond_order_reduce(const uint32_t* counts, uint32_t n_theta, uint32_t n_ph
 
 int fa_correlation_reduce(const uint32_t* counts, const double* sums, uint32_t bins, double* correlation)
 {
+    FA_API_LOCK;
     // CorrelationFunction.cc:50-60: the complex sum divided by the bin's count
     for (uint32_t t = 0; t < bins; ++t)
     {
@@ -809,6 +818,7 @@ int fa_correlation_reduce(const uint32_t* counts, const double* sums, uint32_t b
 int fa_pmft_reduce(int kind, const fa_box* box, const uint32_t* counts, const uint32_t* nbins, const float* maxes,
                    uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, uint32_t n_equiv, float* pcf)
 {
+    FA_API_LOCK;
     // PMFT.h:76-87 (XYT, R12) and PMFTXYZ.cc:92-103, in the reference's float order
     uint64_t const total = uint64_t(nbins[0]) * nbins[1] * nbins[2];
     float const inv_num_dens = fa::box_volume(*box) / float(n_query_points);
@@ -875,6 +885,7 @@ int fa_pmft_reduce(int kind, const fa_box* box, const uint32_t* counts, const ui
 int fa_pmftxy_reduce(const fa_box* box, const uint32_t* counts, uint32_t n_x, uint32_t n_y, float x_max, float y_max,
                      uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, float* pcf)
 {
+    FA_API_LOCK;
     // PMFTXY.cc:48-52,62-66 and PMFT.h:76-87, float operation order of the reference
     volatile float dx = float(2.0) * x_max / float(n_x);
     volatile float dy = float(2.0) * y_max / float(n_y);

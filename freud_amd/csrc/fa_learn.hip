@@ -465,6 +465,7 @@ extern "C" {
 
 int fa_mlp_create(uint64_t seed, fa_mlp** out)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     auto* m = new fa_mlp();
@@ -516,17 +517,20 @@ int fa_mlp_create(uint64_t seed, fa_mlp** out)
 
 int fa_mlp_destroy(fa_mlp* m)
 {
+    FA_API_LOCK;
     delete m;
     return FA_OK;
 }
 
 int fa_mlp_num_params(void)
 {
+    FA_API_LOCK;
     return kParams;
 }
 
 int fa_mlp_set_params(fa_mlp* m, const float* params)
 {
+    FA_API_LOCK;
     if (!m || !params)
         return fa::fail(FA_ERR_INVALID, "null argument");
     FA_HIP(hipMemcpyAsync(m->P, params, sizeof(float) * kParams, hipMemcpyHostToDevice, fa::stream()));
@@ -537,6 +541,7 @@ int fa_mlp_set_params(fa_mlp* m, const float* params)
 
 int fa_mlp_get_params(fa_mlp* m, float* params)
 {
+    FA_API_LOCK;
     if (!m || !params)
         return fa::fail(FA_ERR_INVALID, "null argument");
     FA_HIP(hipMemcpyAsync(params, m->P, sizeof(float) * kParams, hipMemcpyDeviceToHost, fa::stream()));
@@ -546,6 +551,7 @@ int fa_mlp_get_params(fa_mlp* m, float* params)
 
 int fa_mlp_set_norm(fa_mlp* m, const float* mu, const float* sigma)
 {
+    FA_API_LOCK;
     if (!m || !mu || !sigma)
         return fa::fail(FA_ERR_INVALID, "null argument");
     std::vector<float> nrm(2 * kIn, 0.0f);
@@ -563,6 +569,7 @@ int fa_mlp_set_norm(fa_mlp* m, const float* mu, const float* sigma)
 
 int fa_mlp_forward(fa_mlp* m, const float* d_features, uint32_t n, float* d_logits)
 {
+    FA_API_LOCK;
     if (!m)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (n == 0)
@@ -585,6 +592,7 @@ int fa_mlp_forward(fa_mlp* m, const float* d_features, uint32_t n, float* d_logi
 int fa_mlp_gradient(fa_mlp* m, const float* d_features, const uint8_t* d_labels, const uint32_t* d_idx,
                     uint32_t batch, float* grads, float* loss)
 {
+    FA_API_LOCK;
     if (!m || !grads)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (batch == 0)
@@ -607,6 +615,7 @@ int fa_mlp_gradient(fa_mlp* m, const float* d_features, const uint8_t* d_labels,
 int fa_mlp_train(fa_mlp* m, const float* d_features, const uint8_t* d_labels, uint32_t n, uint32_t epochs,
                  uint32_t batch, float lr, uint64_t seed, float* epoch_loss)
 {
+    FA_API_LOCK;
     if (!m)
         return fa::fail(FA_ERR_INVALID, "null argument");
     if (n == 0 || batch == 0)

@@ -200,6 +200,7 @@ extern "C" {
 int fa_nlist_from_arrays(uint32_t num_query_points, uint32_t num_points, const uint32_t* d_qi, const uint32_t* d_pj,
                          const float* d_vectors, const float* d_weights, uint64_t num_bonds, fa_nlist* out)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     hipStream_t s = fa::stream();
@@ -249,6 +250,7 @@ int fa_nlist_from_arrays(uint32_t num_query_points, uint32_t num_points, const u
 
 int fa_nlist_filter(fa_nlist* nl, const uint8_t* d_keep)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     void* tmp = nullptr;
@@ -269,6 +271,7 @@ int fa_nlist_filter(fa_nlist* nl, const uint8_t* d_keep)
 
 int fa_nlist_filter_r(fa_nlist* nl, float r_max, float r_min)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // NeighborList.cc:288-299
@@ -296,6 +299,7 @@ int fa_nlist_filter_r(fa_nlist* nl, float r_max, float r_min)
 
 int fa_nlist_sort(fa_nlist* nl, int by_distance)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     uint64_t const nb = nl->num_bonds;
@@ -344,6 +348,7 @@ int fa_nlist_sort(fa_nlist* nl, int by_distance)
 
 int fa_nlist_copy(const fa_nlist* src, fa_nlist* dst)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (int rc = fa::nlist_reserve(dst, src->num_bonds, src->num_query_points))
@@ -1277,6 +1282,7 @@ extern "C" {
 
 int fa_filter_sann(const fa_nlist* nl, fa_nlist* out, uint8_t* d_unfilled, uint32_t* n_unfilled)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (int rc = filter_check(nl, out, d_unfilled, n_unfilled))
@@ -1316,6 +1322,7 @@ int fa_filter_rad(const fa_box* box, const float* d_points, uint32_t n_points, c
                   const float* d_points_radii, const fa_nlist* nl, int terminate_after_blocked, fa_nlist* out,
                   uint8_t* d_unfilled, uint32_t* n_unfilled)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (!box || !nl)

@@ -1728,6 +1728,7 @@ int fa_steinhardt(const fa_box* box, const float* d_points, uint32_t n, const fa
                   int n_l, int flags, float* d_qlm, float* d_ql, float* d_qlave, float* d_qlmave, float* d_wl,
                   float* order)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     return fa::steinhardt_run(box, d_points, n, nl, ls, n_l, flags, d_qlm, d_ql, d_qlave, d_qlmave, d_wl, order);
@@ -1736,6 +1737,7 @@ int fa_steinhardt(const fa_box* box, const float* d_points, uint32_t n, const fa
 int fa_sl_features(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, int lmax,
                    float* d_features, float* d_qlm6, float* d_ql6)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (lmax != 8 && lmax != 12)
@@ -1769,6 +1771,7 @@ int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float ma
                  const float* d_ql6, int normalize_q, float q_threshold, uint32_t solid_threshold, uint8_t* d_solid,
                  uint8_t* d_fallback, uint32_t* n_fallback)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (nl->num_query_points != n)
@@ -1800,6 +1803,7 @@ int fa_sl_decide(uint32_t n, const fa_nlist* nl, const float* d_logits, float ma
 int fa_cluster(uint32_t n, const fa_nlist* nl, const uint8_t* d_keep, uint32_t* d_cluster_idx,
                uint32_t* d_cluster_sizes, uint32_t* num_clusters)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     hipStream_t s = fa::stream();
@@ -1856,6 +1860,7 @@ int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const f
                    float* d_ql_ij, uint32_t* d_num_connections, uint32_t* d_cluster_idx, uint32_t* d_cluster_sizes,
                    uint32_t* num_clusters)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // SolidLiquid.cc:20-29
@@ -1889,6 +1894,7 @@ int fa_solidliquid(const fa_box* box, const float* d_points, uint32_t n, const f
 int fa_hexatic(const fa_box* box, const float* d_points, uint32_t n, const fa_nlist* nl, uint32_t k, int weighted,
                float* d_psi)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // Box.h:571-577 enforce2D

@@ -2040,6 +2040,7 @@ extern "C" {
 
 int fa_nq_create(const fa_box* box, const float* d_points, uint32_t n, int kind, float cell_width, fa_nq** out)
 {
+    FA_API_LOCK;
     *out = nullptr;
     if (int rc = fa_init())
         return rc;
@@ -2082,6 +2083,7 @@ int fa_nq_create(const fa_box* box, const float* d_points, uint32_t n, int kind,
 
 int fa_nq_destroy(fa_nq* nq)
 {
+    FA_API_LOCK;
     if (nq)
     {
         (void) hipStreamSynchronize(fa::stream());
@@ -2092,12 +2094,14 @@ int fa_nq_destroy(fa_nq* nq)
 
 int fa_nq_cell_width(const fa_nq* nq, float* width)
 {
+    FA_API_LOCK;
     *width = nq->cell_width;
     return FA_OK;
 }
 
 int fa_cellquery_grid(const fa_box* box, float r_cut, int32_t* n_xyz, float* min_pos, float* inverse_width)
 {
+    FA_API_LOCK;
     // CellQuery.cc:60-63
     if (!(r_cut > 0))
         return fa::fail(FA_ERR_RUNTIME, "CellQuery::buildGrid called with invalid r_cut (must be positive).");
@@ -2114,6 +2118,7 @@ int fa_cellquery_grid(const fa_box* box, float r_cut, int32_t* n_xyz, float* min
 int fa_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, const fa_query_args* args,
              int sort_by_distance, fa_nlist* out)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     fa_query_args a = *args;
@@ -2130,6 +2135,7 @@ int fa_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, const fa_
 
 int fa_nlist_free(fa_nlist* nl)
 {
+    FA_API_LOCK;
     if (!nl)
         return FA_OK;
     (void) hipStreamSynchronize(fa::stream());
@@ -2147,6 +2153,7 @@ int fa_nlist_free(fa_nlist* nl)
 int fa_nlist_all_pairs(const fa_box* box, const float* d_points, uint32_t n_points, const float* d_query_points,
                        uint32_t n_query, int exclude_ii, fa_nlist* out)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     uint32_t const nii = exclude_ii ? std::min(n_points, n_query) : 0;

@@ -517,6 +517,24 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
         qsy[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa.y), 8 * k));
         qsz[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pa.z), 8 * k));
     }
+    // The cluster's extent (orthorhombic boxes): a stencil row or cell whose
+    // slab lies farther than r_max from every query point of the cluster holds
+    // no pair.  Cell c along axis k covers [lo_k + c w_k, lo_k + (c + 1) w_k)
+    // up to the rounding of the fractional coordinate, which the margins
+    // (1e-5 of the box length) cover; unwrapped cell indices place periodic
+    // images in the same frame.
+    float xmn = qsx[0], xmx = qsx[0], ymn = qsy[0], ymx = qsy[0], zmn = qsz[0], zmx = qsz[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+    { // qsx is NaN beyond na (ignored by fminf/fmaxf); qsy, qsz repeat point na - 1
+        xmn = fminf(xmn, qsx[k]);
+        xmx = fmaxf(xmx, qsx[k]);
+        ymn = fminf(ymn, qsy[k]);
+        ymx = fmaxf(ymx, qsy[k]);
+        zmn = fminf(zmn, qsz[k]);
+        zmx = fmaxf(zmx, qsz[k]);
+    }
+    float const mgx = 1e-5f * b.L[0], mgy = 1e-5f * b.L[1], mgz = 1e-5f * b.L[2];
     // chunk loads through a buffer resource: wave-uniform chunk offset in an
     // SGPR, the lane's offset in a VGPR (point arrays stay below 4 GiB)
     uint32_t const voff = bl * uint32_t(sizeof(float4));
@@ -549,15 +567,30 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
             int const dz = rt / kRowsY - Rz, dy = rt % kRowsY - R;
             int const uz = cz0 + dz, uy = cy0 + dy;
             int const sz = fa::floor_div(uz, dim2), sy = fa::floor_div(uy, dim1);
-            float const gy = float(max(0, abs(dy) - 1)) * wy, gz = float(max(0, abs(dz) - 1)) * wz;
+            float gy = float(max(0, abs(dy) - 1)) * wy, gz = float(max(0, abs(dz) - 1)) * wz;
             int xr = RX;
             bool reach = true;
+            // cell range along x from the cluster's x extent (all cells otherwise)
+            int ux_lo = -dim0, ux_hi = 2 * dim0 - 1;
             if (ortho)
             {
+                float const ys = b.lo[1] + float(uy) * g.cw[1];
+                gy = fmaxf(gy, fmaxf(ys - ymx, ymn - (ys + g.cw[1])) - mgy);
+                if constexpr (!IS2D)
+                {
+                    float const zs = b.lo[2] + float(uz) * g.cw[2];
+                    gz = fmaxf(gz, fmaxf(zs - zmx, zmn - (zs + g.cw[2])) - mgz);
+                }
                 float const rem = r_lim * r_lim - (gy * gy + gz * gz);
                 reach = rem > 0.0f;
                 if (reach && wx > 0.0f)
-                    xr = min(RX, int(sqrtf(rem) / wx) + 1);
+                {
+                    float const h = sqrtf(rem);
+                    xr = min(RX, int(h / wx) + 1);
+                    float const ix = 1.0f / g.cw[0];
+                    ux_lo = int(fmaxf(floorf((xmn - h - mgx - b.lo[0]) * ix), float(-dim0)));
+                    ux_hi = int(fminf(floorf((xmx + h + mgx - b.lo[0]) * ix), float(2 * dim0 - 1)));
+                }
             }
             else
                 reach = fmaxf(gy, gz) < r_lim;
@@ -566,8 +599,8 @@ __device__ inline void rows_body(const DevBox& b, const DevGrid& g, uint32_t row
                 uint32_t const rowbase = uint32_t((uz - sz * dim2) * dim1 + (uy - sy * dim1)) * uint32_t(dim0);
                 // shifts beyond one box length are not images the reference visits,
                 // so the clipped range has at most 3 pieces (shift -1, 0, 1)
-                int ux = max(cx_lo - xr, -dim0);
-                int const uxend = min(cx_hi + xr, 2 * dim0 - 1);
+                int ux = max(max(cx_lo - xr, -dim0), ux_lo);
+                int const uxend = min(min(cx_hi + xr, 2 * dim0 - 1), ux_hi);
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
                 {
@@ -1323,6 +1356,7 @@ extern "C" {
 
 int fa_rdf_bin_table(uint32_t bins, float r_min, float r_max, float* out, uint64_t cap)
 {
+    FA_API_LOCK;
     if (bins == 0 || !(r_max > r_min))
         return fa::fail(FA_ERR_INVALID, "bin table needs bins > 0 and r_max > r_min");
     std::vector<float> const tab = build_table(bins, r_min, r_max);
@@ -1338,6 +1372,7 @@ int fa_rdf_bin_table(uint32_t bins, float r_min, float r_max, float* out, uint64
 int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_query, float q_r_max, float q_r_min,
                             int exclude_ii, uint32_t bins, float r_min, float r_max, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     // RDF.cc:27-42 argument checks happen in the constructor (host side)
@@ -1454,6 +1489,7 @@ int fa_rdf_accumulate_query(fa_nq* nq, const float* d_query_points, uint32_t n_q
 int fa_rdf_accumulate_self_noalloc(const fa_box* box, const float* d_points, uint32_t n, float q_r_max,
                                    float q_r_min, uint32_t bins, float r_min, float r_max, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     int dev = 0;
@@ -1574,6 +1610,7 @@ int fa_rdf_accumulate_self_noalloc(const fa_box* box, const float* d_points, uin
 
 int fa_rdf_accumulate_nlist(const fa_nlist* nl, uint32_t bins, float r_min, float r_max, uint32_t* d_counts)
 {
+    FA_API_LOCK;
     if (int rc = fa_init())
         return rc;
     if (nl->num_bonds == 0)
@@ -1598,6 +1635,7 @@ int fa_rdf_reduce(const fa_box* box, const uint32_t* counts, uint32_t bins, floa
                   uint32_t n_points, uint32_t n_query_points, uint32_t n_frames, int finite_size, float* rdf,
                   float* n_r)
 {
+    FA_API_LOCK;
     if (bins == 0)
         return fa::fail(FA_ERR_INVALID, "RDF requires a nonzero number of bins.");
     float const width = (r_max - r_min) / float(bins);

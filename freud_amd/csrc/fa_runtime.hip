@@ -11,6 +11,12 @@
 
 namespace fa {
 
+std::recursive_mutex& api_mutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 namespace {
 thread_local std::string g_err;
 std::mutex g_mutex;
@@ -343,10 +349,15 @@ extern "C" {
 
 const char* fa_last_error(void) { return fa::g_err.c_str(); }
 
-int fa_init(void) { return fa::ensure_init(); }
+int fa_init(void)
+{
+    FA_API_LOCK;
+    return fa::ensure_init();
+}
 
 int fa_device_name(char* buf, int len)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     hipDeviceProp_t prop;
@@ -357,6 +368,7 @@ int fa_device_name(char* buf, int len)
 
 int fa_malloc(void** d_ptr, size_t bytes)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     return fa::device_alloc(d_ptr, bytes);
@@ -364,12 +376,14 @@ int fa_malloc(void** d_ptr, size_t bytes)
 
 int fa_free(void* d_ptr)
 {
+    FA_API_LOCK;
     fa::device_free(d_ptr);
     return FA_OK;
 }
 
 int fa_memcpy_h2d(void* d_dst, const void* src, size_t bytes)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     if (bytes == 0)
@@ -381,6 +395,7 @@ int fa_memcpy_h2d(void* d_dst, const void* src, size_t bytes)
 
 int fa_memcpy_d2h(void* dst, const void* d_src, size_t bytes)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     if (bytes == 0)
@@ -392,6 +407,7 @@ int fa_memcpy_d2h(void* dst, const void* d_src, size_t bytes)
 
 int fa_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     if (bytes == 0)
@@ -402,6 +418,7 @@ int fa_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes)
 
 int fa_memset(void* d_dst, int value, size_t bytes)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     if (bytes == 0)
@@ -412,6 +429,7 @@ int fa_memset(void* d_dst, int value, size_t bytes)
 
 int fa_synchronize(void)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     FA_HIP(hipStreamSynchronize(fa::stream()));
@@ -420,6 +438,7 @@ int fa_synchronize(void)
 
 int fa_timer_start(void)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     FA_HIP(hipEventRecord(fa::g_t0, fa::stream()));
@@ -428,6 +447,7 @@ int fa_timer_start(void)
 
 int fa_timer_stop(float* ms)
 {
+    FA_API_LOCK;
     FA_HIP(hipEventRecord(fa::g_t1, fa::stream()));
     FA_HIP(hipEventSynchronize(fa::g_t1));
     FA_HIP(hipEventElapsedTime(ms, fa::g_t0, fa::g_t1));
@@ -436,6 +456,7 @@ int fa_timer_stop(float* ms)
 
 int fa_profile_enable(int on)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     fa::collect();
@@ -445,6 +466,7 @@ int fa_profile_enable(int on)
 
 int fa_profile_reset(void)
 {
+    FA_API_LOCK;
     fa::collect();
     fa::g_stats.clear();
     return FA_OK;
@@ -452,12 +474,14 @@ int fa_profile_reset(void)
 
 int fa_profile_count(void)
 {
+    FA_API_LOCK;
     fa::collect();
     return int(fa::g_stats.size());
 }
 
 int fa_profile_get(int idx, char* name, int len, uint64_t* launches, double* total_ms)
 {
+    FA_API_LOCK;
     fa::collect();
     if (idx < 0 || idx >= int(fa::g_stats.size()))
         return fa::fail(FA_ERR_RANGE, "profile index out of range");
@@ -469,6 +493,7 @@ int fa_profile_get(int idx, char* name, int len, uint64_t* launches, double* tot
 
 int fa_box_transform(const fa_box* box, int op, const float* d_in, float* d_out, uint64_t n)
 {
+    FA_API_LOCK;
     if (int rc = fa::ensure_init())
         return rc;
     if (n == 0)

@@ -75,17 +75,20 @@ extern "C" {
 
 int fa_tune_enable(int on)
 {
+    FA_API_LOCK;
     fa::g_tune_on = on != 0;
     return FA_OK;
 }
 
 int fa_tune_count(void)
 {
+    FA_API_LOCK;
     return int(fa::g_table.size());
 }
 
 int fa_tune_get(int idx, int* analysis, int* key, int* variant)
 {
+    FA_API_LOCK;
     if (idx < 0 || idx >= int(fa::g_table.size()))
         return fa::fail(FA_ERR_RANGE, "tuning entry index out of range");
     auto it = fa::g_table.begin();
@@ -98,6 +101,7 @@ int fa_tune_get(int idx, int* analysis, int* key, int* variant)
 
 int fa_tune_set(int analysis, int key, int variant)
 {
+    FA_API_LOCK;
     if (variant < 0)
     {
         fa::g_table.erase({analysis, key});
@@ -113,6 +117,7 @@ int fa_tune_set(int analysis, int key, int variant)
 
 int fa_tune_reset(void)
 {
+    FA_API_LOCK;
     fa::g_table.clear();
     fa::g_preset.clear();
     return FA_OK;
@@ -120,6 +125,7 @@ int fa_tune_reset(void)
 
 int fa_tune_variants(int analysis, int key)
 {
+    FA_API_LOCK;
     switch (analysis)
     {
     case fa::TUNE_RDF_SELF:
