@@ -432,7 +432,7 @@ def run_batched(args):
         "kernels": kernels,
         "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return _emit(out, args)
 
 
 def run_stress(args):
@@ -542,7 +542,7 @@ def run_stress(args):
         "kernels": kernels,
         "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return _emit(out, args)
 
 
 def run_learned(args):
@@ -670,7 +670,7 @@ def run_learned(args):
                          "kernel": "mlp_infer_kernel", "avg_ms": mlp_ms},
         "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return _emit(out, args)
 
 
 def run_filters(args):
@@ -779,7 +779,41 @@ def run_filters(args):
     }
     for o in outs + [nl]:
         lib.fa_nlist_free(ctypes.byref(o))
-    print(json.dumps(out))
+    return _emit(out, args)
+
+
+def _emit(out, args):
+    """Print a workload's JSON line, or hand it back when it runs as a
+    secondary workload of the default frame run."""
+    if not getattr(args, "secondary", False):
+        print(json.dumps(out))
+    return out
+
+
+def _secondary_lines(args):
+    """The batched, stress, learned and filters workloads, run after the frame
+    in the default invocation (the one that also times the CPU baseline), so
+    one run of bench.py measures every configuration.  Each entry is that
+    workload's own line without its per-kernel table, or the error it raised."""
+    import copy
+
+    res = {}
+    for name, fn in (("batched", run_batched), ("stress", run_stress), ("learned", run_learned),
+                     ("filters", run_filters)):
+        a = copy.copy(args)
+        a.secondary = True
+        a.no_cpu_baseline = True
+        t0 = time.perf_counter()
+        print(f"[bench] secondary workload {name}", file=sys.stderr, flush=True)
+        try:
+            out = fn(a)
+            out = {k: v for k, v in out.items() if k != "kernels"}
+        except Exception as e:  # the headline line is still printed
+            out = {"error": f"{type(e).__name__}: {e}"}
+        out["wall_s"] = round(time.perf_counter() - t0, 1)
+        print(f"[bench] secondary workload {name} done in {out['wall_s']} s", file=sys.stderr, flush=True)
+        res[name] = out
+    return res
 
 
 def main():
@@ -793,6 +827,10 @@ def main():
                          "frames, batched + graph-replayed against the per-frame loop; stress: 500k triclinic "
                          "(l = 4..12) + 250k 2D hexagonal (Hexatic); learned: 2M-particle two-phase SolidLiquid, "
                          "exact vs learned; filters: FilterSANN / FilterRAD over the 1M-point frame's r=1.9 list")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="default frame run only: skip the batched, stress, learned and filters workloads that it "
+                         "otherwise runs after the frame and reports under 'secondary' (they are skipped anyway "
+                         "with --no-cpu-baseline or --copy-mib)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -912,6 +950,8 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
     }
+    if not (args.no_cpu_baseline or args.no_secondary or args.copy_mib > 0):
+        out["secondary"] = _secondary_lines(args)
     print(json.dumps(out))
 
 
